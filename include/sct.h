@@ -1,0 +1,180 @@
+/*
+ * sct.h -- C ABI of the MI355X-native segmentctscans hot path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status
+ * (SCT_OK on success); the message of the last failure on a context is
+ * returned by sct_ctx_last_error().  The C++ header segmentctscans.hpp wraps
+ * these calls in the reference's own C++ signatures and turns status codes
+ * back into the reference's exceptions / null returns.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   include/cuda-context.h:11-37             context, upload/download, memory query
+ *   include/cuda-details/device-field-impl.h DeviceField<T> (voxels + mapping tables)
+ *   include/layout_cylinder.h:152-195        cylinder mapping constructor
+ *   include/field.h:37-45                    get_stride / prev_stride
+ *   potts-model/potts-min-partition.h:8-16   potts_min_partition<K>
+ *   totalvarmin/chambolle.h:5-6              chambolle
+ *   include/calc_hist.h:8-12                 calc_hist (both overloads)
+ *   include/min_max.h:7                      min_max
+ *   include/grad_magn.h:5-6                  grad_magn
+ *   uv-hist/uv-hist.h:18-44                  uv_hist (both overloads)
+ *
+ * Threading: one HIP stream per context; a context is not thread-safe.
+ * Every call that returns host values synchronises the stream, as the
+ * reference does; sct_grad_magn is asynchronous like grad_magn.cpp:60-79.
+ */
+#ifndef SCT_H
+#define SCT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define SCT_OK          0
+#define SCT_ERR_INVALID 1  /* bad argument (reference: validation_error)            */
+#define SCT_ERR_HIP     2  /* HIP runtime failure (reference: cuda::cuda_error)      */
+#define SCT_ERR_SIZE    3  /* host/device span mismatch (cuda-context.cpp:69-71)     */
+#define SCT_ERR_NOMEM   4  /* slab cannot hold the halo: the reference's null return
+                              (potts-min-partition.cpp:571-577)                      */
+
+/* ---- element types ------------------------------------------------------ */
+#define SCT_F32 0
+#define SCT_U16 1
+
+typedef struct sct_ctx sct_ctx;
+typedef struct sct_field sct_field;
+typedef struct sct_potts_solver sct_potts_solver;
+
+/* Cylinder layout (layout_cylinder::mapping, layout_cylinder.h:29-112).
+ * offset(n, row, col) = base[col] + (row - start_row[col]) * nstride + n. */
+typedef struct sct_layout {
+    uint32_t n_images, height, width;   /* extents: n (contiguous), rows, cols */
+    uint32_t nstride;                   /* elements per (row, col) line         */
+    uint32_t col_begin, col_end;
+    const uint32_t *start_row;          /* [width]                              */
+    const uint32_t *base;               /* [width + 1]; base[width] = span      */
+} sct_layout;
+
+/* ---- host-side layout helpers (no device needed) ----------------------- */
+
+/* layout_cylinder::mapping_storage(extents, xc, yc, r, nstride),
+ * layout_cylinder.h:152-195.  start_row[width], base[width + 1]. */
+int sct_cylinder_layout(uint32_t n_images, uint32_t height, uint32_t width,
+                        double xc, double yc, double r, uint32_t nstride,
+                        uint32_t *start_row, uint32_t *base,
+                        uint32_t *col_begin, uint32_t *col_end, uint64_t *data_size);
+/* Field<float>::get_stride / prev_stride, field.h:37-45 (128-byte rows). */
+uint32_t sct_get_stride(uint32_t n);
+uint32_t sct_prev_stride(uint32_t n);
+
+/* ---- context: cuda-context.h:11-14, cuda-context.cpp:14-20 ------------- */
+int  sct_ctx_create(int device, sct_ctx **out);
+void sct_ctx_destroy(sct_ctx *ctx);
+const char *sct_ctx_last_error(const sct_ctx *ctx);
+/* The context's hipStream_t (for callers that time or order work on it). */
+void *sct_ctx_stream(sct_ctx *ctx);
+int  sct_ctx_synchronize(sct_ctx *ctx);
+/* device_memory(), cuda-context.cpp:22-41: free + pool (reserved - used). */
+int  sct_device_memory(sct_ctx *ctx, size_t *available, size_t *total);
+
+/* ---- fields: cuda-context.h:24-31, device-field-impl.h:14-54 ----------- */
+/* DeviceField(mapping): uninitialised voxels. */
+int  sct_field_create(sct_ctx *ctx, const sct_layout *layout, int dtype, sct_field **out);
+/* upload(ctx, mdspan): copies span elements from host. */
+int  sct_upload(sct_ctx *ctx, const sct_layout *layout, int dtype, const void *host,
+                sct_field **out);
+/* download(ctx, field, mdspan): span_elems must equal the field's span
+ * (SCT_ERR_SIZE otherwise, as cuda-context.cpp:69-71 throws); synchronises. */
+int  sct_download(sct_ctx *ctx, const sct_field *field, void *host, size_t span_elems);
+void sct_field_destroy(sct_field *field);
+void *sct_field_data(sct_field *field);       /* device pointer to the voxels   */
+size_t sct_field_span(const sct_field *field); /* required_span_size()          */
+uint64_t sct_field_data_size(const sct_field *field); /* in-mask voxels         */
+
+/* ---- Potts: potts-min-partition.h:8-16 ---------------------------------- */
+#define SCT_STORAGE_F32  0
+#define SCT_STORAGE_BF16 1   /* Xi and V-bar stored as bfloat16, arithmetic FP32 */
+#define SCT_STORAGE_F16  2   /* Xi and V-bar stored as IEEE half                 */
+
+#define SCT_KERNEL_AUTO    0
+#define SCT_KERNEL_TWOPASS 1  /* reference-shaped: Xi pass, then V pass       */
+#define SCT_KERNEL_FUSED   2  /* one pass per iteration, z-marching LDS tiles */
+
+typedef struct sct_potts_opts {
+    uint32_t size;          /* sizeof(sct_potts_opts), for forward compatibility  */
+    int32_t  storage;       /* SCT_STORAGE_*                                      */
+    int32_t  kernel;        /* SCT_KERNEL_*                                       */
+    int32_t  slab_w;        /* >0: force this slab width (halo = iters)           */
+    int64_t  mem_available; /* >=0: pretend device_memory().available is this     */
+    float   *dq_trace;      /* optional host [iters]: max_x sum_k (dV_k)^2 per
+                               iteration (last slab when several)                 */
+    double  *gap_trace;     /* optional host [iters / gap_every]: relative
+                               primal-dual gap, FP64 (extension)                  */
+    int32_t  gap_every;     /* iterations between gap evaluations (0: never)      */
+    int32_t  reserved;
+} sct_potts_opts;
+
+void sct_potts_opts_init(sct_potts_opts *opts);
+
+/* potts_min_partition<K>(ctx, g, cpu_map, ci, lambda, tau1, tau2, iters):
+ * K in [2, 6]; *out receives a new field (same layout as g) holding c[argmax]
+ * per voxel.  SCT_ERR_NOMEM when a slab cannot hold 2*iters halo planes. */
+int sct_potts_min_partition(sct_ctx *ctx, const sct_field *g, int K, const float *c,
+                            float lambda, float tau1, float tau2, int iters,
+                            const sct_potts_opts *opts, sct_field **out);
+
+/* Stepping interface over the same solver (single slab; extension): create
+ * runs init_v (potts-min-partition.cpp:641-644), iterate runs more primal-dual
+ * iterations, labels runs update_kernel (:491-538) into res. */
+int  sct_potts_solver_create(sct_ctx *ctx, const sct_field *g, int K, const float *c,
+                             float lambda, float tau1, float tau2,
+                             const sct_potts_opts *opts, sct_potts_solver **out);
+int  sct_potts_solver_iterate(sct_potts_solver *s, int iters);
+int  sct_potts_solver_labels(sct_potts_solver *s, sct_field *res);
+/* Downloads V ((K-1) x span) and Xi (3(K-1) x span), component-major, in the
+ * input layout; either pointer may be NULL.  Synchronises. */
+int  sct_potts_solver_state(sct_potts_solver *s, float *V, float *xi);
+int  sct_potts_solver_dq(sct_potts_solver *s, float *dq, int n);
+void sct_potts_solver_destroy(sct_potts_solver *s);
+
+/* ---- Chambolle TV: chambolle.h:5-6 (g overwritten) ---------------------- */
+int sct_chambolle(sct_ctx *ctx, sct_field *g, float lambda, float tau, int iters,
+                  float *dq_trace);
+
+/* ---- reductions ---------------------------------------------------------- */
+/* min_max.h:7 */
+int sct_min_max(sct_ctx *ctx, const sct_field *u, float *mn, float *mx);
+/* calc_hist.h:11-12 with nbins widened from uint16_t to uint32_t (65536 bins
+ * are not expressible in the reference's type).  counts[nbins] receives the
+ * integer counts, density[nbins] (optional) count / (data_size * bin_width)
+ * as calc_hist.cpp:131-137.  delta != 0 adds uniform dither in
+ * [-delta/2, delta/2) from Philox4x32-10 keyed by seed. */
+int sct_calc_hist(sct_ctx *ctx, const sct_field *u, float range_start, float range_end,
+                  uint32_t nbins, float delta, uint64_t seed, uint32_t *counts,
+                  double *density);
+/* calc_hist.h:8-9: range from min_max (calc_hist.cpp:83-92). */
+int sct_calc_hist_auto(sct_ctx *ctx, const sct_field *u, uint32_t nbins, float delta,
+                       uint64_t seed, uint32_t *counts, double *density,
+                       float *range_start, float *range_end);
+/* grad_magn.h:5-6; asynchronous. */
+int sct_grad_magn(sct_ctx *ctx, const sct_field *u, sct_field **v);
+/* uv-hist-kernel.cpp:31-71: min/max over strictly positive v. */
+int sct_grad_min_max(sct_ctx *ctx, const sct_field *v, float *mn, float *mx);
+/* uv-hist.h:37-44: counts[(nbins - 1) * stride + nbins], stride = nbins
+ * rounded up to 32 (returned in *stride). */
+int sct_uv_hist(sct_ctx *ctx, const sct_field *u, const sct_field *v,
+                float u_start, float u_end, float v_start, float v_end,
+                uint32_t nbins, uint32_t *counts, uint32_t *stride);
+/* uv-hist.h:30-35: ranges from min_max(u) and grad_min_max(v);
+ * ranges[4] = {u_start, u_end, v_start, v_end}. */
+int sct_uv_hist_auto(sct_ctx *ctx, const sct_field *u, const sct_field *v, uint32_t nbins,
+                     uint32_t *counts, uint32_t *stride, float *ranges);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

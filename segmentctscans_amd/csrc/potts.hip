@@ -1,0 +1,540 @@
+// potts.hip -- Chambolle-Cremers-Pock Potts solver (potts-model/potts-min-partition.cpp).
+//
+// State per voxel, component-major in a work layout of `lines` lines of
+// `wstride` floats (the input's lines, n re-based to the slab start):
+//   Xi    3(K-1) components   (the K-th is -sum, never stored)
+//   V     K-1                 (the K-th is 1 - sum)
+//   V-bar K-1
+// The data term lambda (u - c_i)^2 is recomputed from u every iteration.
+#include "sct_internal.hpp"
+#include "potts_math.hpp"
+#include "potts.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+using namespace sct;
+
+namespace {
+
+struct WorkGeom {
+    const LineNbr *nbr;
+    uint32_t lines;      // number of lines
+    uint32_t len;        // slab length along n
+    uint32_t chunks;     // ceil(len / 64)
+    uint32_t wstride;    // work line stride
+    uint64_t wspan;      // lines * wstride (component stride)
+    uint32_t in_stride;  // input line stride
+    uint32_t s0;         // slab start in input coordinates
+};
+
+// One wave = 64 consecutive n of one line.
+__device__ __forceinline__ bool wave_voxel(const WorkGeom &g, uint32_t &line, uint32_t &n)
+{
+    const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    line = uint32_t(gw / g.chunks);
+    n = uint32_t(gw % g.chunks) * kWave + (threadIdx.x % kWave);
+    return line < g.lines && n < g.len;
+}
+
+template <int K>
+struct Labels {
+    float c[K];
+};
+
+// init_v_kernel, potts-min-partition.cpp:56-94.
+template <int K>
+__global__ __launch_bounds__(256) void potts_init_kernel(WorkGeom g, const float *__restrict__ u,
+                                                         float *__restrict__ V, float *__restrict__ vbar,
+                                                         Labels<K> lab)
+{
+    uint32_t line, n;
+    if (!wave_voxel(g, line, n)) return;
+    const float x = u[uint64_t(line) * g.in_stride + g.s0 + n];
+    const int cl = nearest_label<K>(x, lab.c);
+    const uint64_t w = uint64_t(line) * g.wstride + n;
+#pragma unroll
+    for (int i = 0; i < K - 1; ++i) {
+        const float o = (i == cl) ? 1.0f : 0.0f;
+        V[i * g.wspan + w] = o;
+        vbar[i * g.wspan + w] = o;
+    }
+}
+
+// grad_kernel, potts-min-partition.cpp:96-247: Xi <- Pi_K(Xi + tau2 grad V-bar), in place.
+template <int K>
+__global__ __launch_bounds__(256) void potts_xi_kernel(WorkGeom g, float *__restrict__ xi,
+                                                       const float *__restrict__ vbar, float tau2,
+                                                       DykstraLimits lim)
+{
+    uint32_t line, n;
+    if (!wave_voxel(g, line, n)) return;
+    const LineNbr nb = g.nbr[line];
+    const uint64_t w = uint64_t(line) * g.wstride + n;
+    const bool nx[3] = {n + 1 < g.len, nb.nrow >= 0, nb.ncol >= 0};
+    const uint64_t wn[3] = {w + 1, uint64_t(nb.nrow) * g.wstride + n, uint64_t(nb.ncol) * g.wstride + n};
+    float vb[K - 1], vbn[3][K - 1], x[K - 1][3];
+#pragma unroll
+    for (int i = 0; i < K - 1; ++i) {
+        vb[i] = vbar[i * g.wspan + w];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            vbn[a][i] = nx[a] ? vbar[i * g.wspan + wn[a]] : 0.0f;
+            x[i][a] = xi[(3 * i + a) * g.wspan + w];
+        }
+    }
+    xi_update<K>(vb, vbn, nx, x, tau2, lim);
+#pragma unroll
+    for (int i = 0; i < K - 1; ++i)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) xi[(3 * i + a) * g.wspan + w] = x[i][a];
+}
+
+// div_kernel, potts-min-partition.cpp:349-488: V <- Pi_S(V + tau1 (div Xi - G)),
+// V-bar <- 2 V+ - V.  With TRACE, the block maximum of sum_k dV_k^2 goes to *dq.
+template <int K, bool TRACE>
+__global__ __launch_bounds__(256) void potts_v_kernel(WorkGeom g, const float *__restrict__ xi,
+                                                      float *__restrict__ V, float *__restrict__ vbar,
+                                                      const float *__restrict__ u, Labels<K> lab,
+                                                      float lambda, float tau1, float *dq)
+{
+    uint32_t line, n;
+    float d = 0.0f;
+    if (wave_voxel(g, line, n)) {
+        const LineNbr nb = g.nbr[line];
+        const uint64_t w = uint64_t(line) * g.wstride + n;
+        const bool px[3] = {n > 0, nb.prow >= 0, nb.pcol >= 0};
+        const bool nx[3] = {n + 1 < g.len, nb.nrow >= 0, nb.ncol >= 0};
+        const uint64_t wp[3] = {w - 1, uint64_t(nb.prow) * g.wstride + n, uint64_t(nb.pcol) * g.wstride + n};
+        float xo[K - 1][3], xp[3][K - 1], v[K - 1], vb[K - 1];
+#pragma unroll
+        for (int i = 0; i < K - 1; ++i) {
+            v[i] = V[i * g.wspan + w];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                xo[i][a] = xi[(3 * i + a) * g.wspan + w];
+                xp[a][i] = px[a] ? xi[(3 * i + a) * g.wspan + wp[a]] : 0.0f;
+            }
+        }
+        const float uu = u[uint64_t(line) * g.in_stride + g.s0 + n];
+        d = v_update<K>(xo, xp, px, nx, v, vb, uu, lab.c, lambda, tau1);
+#pragma unroll
+        for (int i = 0; i < K - 1; ++i) {
+            V[i * g.wspan + w] = v[i];
+            vbar[i * g.wspan + w] = vb[i];
+        }
+    }
+    if constexpr (TRACE) {
+        __shared__ float red[4];
+        d = wave_max(d);
+        if ((threadIdx.x % kWave) == 0) red[threadIdx.x / kWave] = d;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float m = red[0];
+            for (int i = 1; i < int(blockDim.x / kWave); ++i) m = fmaxf(m, red[i]);
+            atomic_max_nonneg(dq, m);
+        }
+    }
+}
+
+// update_kernel, potts-min-partition.cpp:491-538: res = c[argmax V] on [r0, r1).
+template <int K>
+__global__ __launch_bounds__(256) void potts_labels_kernel(WorkGeom g, const float *__restrict__ V,
+                                                           float *__restrict__ res, Labels<K> lab,
+                                                           uint32_t r0, uint32_t r1)
+{
+    uint32_t line, n;
+    if (!wave_voxel(g, line, n)) return;
+    const uint32_t na = g.s0 + n;
+    if (na < r0 || na >= r1) return;
+    const uint64_t w = uint64_t(line) * g.wstride + n;
+    float v[K - 1];
+#pragma unroll
+    for (int i = 0; i < K - 1; ++i) v[i] = V[i * g.wspan + w];
+    res[uint64_t(line) * g.in_stride + na] = lab.c[argmax_label<K>(v)];
+}
+
+uint32_t grid_for(const WorkGeom &g)
+{
+    const uint64_t waves = uint64_t(g.lines) * g.chunks;
+    return uint32_t((waves + 3) / 4);
+}
+
+// sqrtf-free thresholds (see DykstraLimits).
+DykstraLimits dykstra_limits()
+{
+    static DykstraLimits lim = [] {
+        auto bits = [](float f) { uint32_t b; std::memcpy(&b, &f, 4); return b; };
+        auto flt = [](uint32_t b) { float f; std::memcpy(&f, &b, 4); return f; };
+        // t1: largest x >= 0 with sqrtf(x) <= 1.
+        uint32_t lo = 0, hi = bits(4.0f);
+        while (hi - lo > 1) {
+            uint32_t mid = lo + (hi - lo) / 2;
+            if (std::sqrt(flt(mid)) <= 1.0f) lo = mid; else hi = mid;
+        }
+        DykstraLimits l;
+        l.t1 = flt(lo);
+        // stol: smallest x >= 0 with (double)sqrtf(x) >= 1e-6.
+        lo = 0;
+        hi = bits(1.0f);
+        while (hi - lo > 1) {
+            uint32_t mid = lo + (hi - lo) / 2;
+            if (double(std::sqrt(flt(mid))) >= 1e-6) hi = mid; else lo = mid;
+        }
+        l.stol = flt(hi);
+        return l;
+    }();
+    return lim;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Solver object
+// ---------------------------------------------------------------------------
+struct sct_potts_solver {
+    sct_ctx *ctx = nullptr;
+    const sct_field *g = nullptr;
+    int K = 0;
+    float c[6] = {};
+    float lambda = 0, tau1 = 0, tau2 = 0;
+    sct_potts_opts opts{};
+    int kernel = SCT_KERNEL_TWOPASS;
+    uint32_t s0 = 0, s1 = 0, wstride = 0;
+    uint64_t wspan = 0;
+    float *xi = nullptr, *V = nullptr, *vbar = nullptr;
+    float *d_dq = nullptr;
+    int dq_cap = 0;
+    int it = 0;   // iterations done on this slab
+};
+
+namespace {
+
+WorkGeom make_geom(const sct_potts_solver *s)
+{
+    const Geometry &geo = s->g->geo;
+    WorkGeom g;
+    g.nbr = geo.d_nbr;
+    g.lines = geo.lines;
+    g.len = s->s1 - s->s0;
+    g.chunks = (g.len + kWave - 1) / kWave;
+    g.wstride = s->wstride;
+    g.wspan = s->wspan;
+    g.in_stride = geo.nstride;
+    g.s0 = s->s0;
+    return g;
+}
+
+template <int K>
+int launch_init(sct_potts_solver *s)
+{
+    const WorkGeom g = make_geom(s);
+    Labels<K> lab;
+    for (int i = 0; i < K; ++i) lab.c[i] = s->c[i];
+    if (g.lines && g.len) {
+        potts_init_kernel<K><<<grid_for(g), 256, 0, s->ctx->stream>>>(g, (const float *)s->g->data, s->V, s->vbar, lab);
+        SCT_HIP(s->ctx, hipGetLastError());
+    }
+    SCT_HIP(s->ctx, hipMemsetAsync(s->xi, 0, sizeof(float) * 3 * (K - 1) * s->wspan, s->ctx->stream));
+    return SCT_OK;
+}
+
+template <int K>
+int launch_iterate(sct_potts_solver *s, int iters, float *dq_dev)
+{
+    const WorkGeom g = make_geom(s);
+    if (!g.lines || !g.len) return SCT_OK;
+    Labels<K> lab;
+    for (int i = 0; i < K; ++i) lab.c[i] = s->c[i];
+    const DykstraLimits lim = dykstra_limits();
+    const uint32_t grid = grid_for(g);
+    for (int i = 0; i < iters; ++i) {
+        potts_xi_kernel<K><<<grid, 256, 0, s->ctx->stream>>>(g, s->xi, s->vbar, s->tau2, lim);
+        if (dq_dev)
+            potts_v_kernel<K, true><<<grid, 256, 0, s->ctx->stream>>>(g, s->xi, s->V, s->vbar, (const float *)s->g->data,
+                                                                     lab, s->lambda, s->tau1, dq_dev + i);
+        else
+            potts_v_kernel<K, false><<<grid, 256, 0, s->ctx->stream>>>(g, s->xi, s->V, s->vbar, (const float *)s->g->data,
+                                                                      lab, s->lambda, s->tau1, nullptr);
+    }
+    SCT_HIP(s->ctx, hipGetLastError());
+    return SCT_OK;
+}
+
+template <int K>
+int launch_labels(sct_potts_solver *s, float *res, uint32_t r0, uint32_t r1)
+{
+    const WorkGeom g = make_geom(s);
+    if (!g.lines || !g.len || r0 >= r1) return SCT_OK;
+    Labels<K> lab;
+    for (int i = 0; i < K; ++i) lab.c[i] = s->c[i];
+    potts_labels_kernel<K><<<grid_for(g), 256, 0, s->ctx->stream>>>(g, s->V, res, lab, r0, r1);
+    SCT_HIP(s->ctx, hipGetLastError());
+    return SCT_OK;
+}
+
+#define SCT_DISPATCH_K(K, fn, ...)              \
+    switch (K) {                                \
+    case 2: return fn<2>(__VA_ARGS__);          \
+    case 3: return fn<3>(__VA_ARGS__);          \
+    case 4: return fn<4>(__VA_ARGS__);          \
+    case 5: return fn<5>(__VA_ARGS__);          \
+    case 6: return fn<6>(__VA_ARGS__);          \
+    default: return SCT_ERR_INVALID;            \
+    }
+
+int do_init(sct_potts_solver *s) { SCT_DISPATCH_K(s->K, launch_init, s); }
+int do_iterate(sct_potts_solver *s, int iters, float *dq) { SCT_DISPATCH_K(s->K, launch_iterate, s, iters, dq); }
+int do_labels(sct_potts_solver *s, float *res, uint32_t r0, uint32_t r1) { SCT_DISPATCH_K(s->K, launch_labels, s, res, r0, r1); }
+
+// Words of solver state per voxel of a slab.
+uint64_t words_per_voxel(int K) { return uint64_t(5) * (K - 1); }
+
+void release_buffers(sct_potts_solver *s)
+{
+    sct_free(s->ctx, s->xi);
+    sct_free(s->ctx, s->V);
+    sct_free(s->ctx, s->vbar);
+    s->xi = s->V = s->vbar = nullptr;
+}
+
+int alloc_buffers(sct_potts_solver *s, uint32_t slab_len)
+{
+    const Geometry &geo = s->g->geo;
+    s->wstride = sct_get_stride(slab_len);
+    s->wspan = uint64_t(geo.lines) * s->wstride;
+    const size_t comp = sizeof(float) * std::max<uint64_t>(1, s->wspan);
+    int rc;
+    if ((rc = sct_alloc(s->ctx, (void **)&s->xi, comp * 3 * (s->K - 1)))) return rc;
+    if ((rc = sct_alloc(s->ctx, (void **)&s->V, comp * (s->K - 1)))) return rc;
+    if ((rc = sct_alloc(s->ctx, (void **)&s->vbar, comp * (s->K - 1)))) return rc;
+    return SCT_OK;
+}
+
+int validate(sct_ctx *ctx, const sct_field *g, int K, const float *c, int iters)
+{
+    if (!ctx || !g || !c) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
+    if (g->dtype != SCT_F32) return sct_fail(ctx, SCT_ERR_INVALID, "Potts needs a float field");
+    if (K < 2 || K > 6) return sct_fail(ctx, SCT_ERR_INVALID, "K must be in [2, 6]");
+    if (iters < 0) return sct_fail(ctx, SCT_ERR_INVALID, "iters must be >= 0");
+    return SCT_OK;
+}
+
+}  // namespace
+
+extern "C" void sct_potts_opts_init(sct_potts_opts *o)
+{
+    std::memset(o, 0, sizeof(*o));
+    o->size = sizeof(*o);
+    o->mem_available = -1;
+}
+
+// Slab plan, potts-min-partition.cpp:560-577 and :627-630.
+int sct::potts_plan(sct_ctx *ctx, const sct_field *g, int K, int iters, const sct_potts_opts *o,
+                    PottsPlan *plan)
+{
+    const Geometry &geo = g->geo;
+    const int n = int(geo.n_images);
+    int slab_w;
+    if (o && o->slab_w > 0) {
+        slab_w = o->slab_w;
+    } else {
+        size_t avail = 0, total = 0;
+        if (o && o->mem_available >= 0) {
+            avail = size_t(o->mem_available);
+        } else {
+            int rc = sct_device_memory(ctx, &avail, &total);
+            if (rc) return rc;
+        }
+        const uint64_t per_slice = sizeof(float) * uint64_t(std::max<uint32_t>(1, geo.lines)) * words_per_voxel(K);
+        const uint64_t w = avail / per_slice;
+        slab_w = int(sct_prev_stride(uint32_t(std::min<uint64_t>(w, 0x7fffffffu))));
+    }
+    plan->halo = iters;
+    if (n <= slab_w) {
+        plan->slab_w = n;
+        plan->res_w = n;
+        plan->n_slabs = 1;
+        return SCT_OK;
+    }
+    plan->slab_w = slab_w;
+    plan->res_w = slab_w - 2 * iters;
+    if (plan->res_w <= 0)
+        return sct_fail(ctx, SCT_ERR_NOMEM, "Not enough memory. Reduce the mask radius or the number of iterations.");
+    plan->n_slabs = 1 + (n - slab_w + plan->res_w - 1) / plan->res_w;
+    return SCT_OK;
+}
+
+extern "C" int sct_potts_solver_create(sct_ctx *ctx, const sct_field *g, int K, const float *c, float lambda,
+                                       float tau1, float tau2, const sct_potts_opts *opts,
+                                       sct_potts_solver **out)
+{
+    if (!out) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    int rc = validate(ctx, g, K, c, 0);
+    if (rc) return rc;
+    auto *s = new sct_potts_solver;
+    s->ctx = ctx;
+    s->g = g;
+    s->K = K;
+    std::copy(c, c + K, s->c);
+    s->lambda = lambda;
+    s->tau1 = tau1;
+    s->tau2 = tau2;
+    if (opts) std::memcpy(&s->opts, opts, std::min<size_t>(sizeof(s->opts), opts->size ? opts->size : sizeof(s->opts)));
+    else sct_potts_opts_init(&s->opts);
+    s->s0 = 0;
+    s->s1 = g->geo.n_images;
+    rc = alloc_buffers(s, s->s1);
+    if (!rc) rc = do_init(s);
+    if (rc) {
+        release_buffers(s);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return SCT_OK;
+}
+
+extern "C" int sct_potts_solver_iterate(sct_potts_solver *s, int iters)
+{
+    if (!s || iters < 0) return SCT_ERR_INVALID;
+    float *dq = nullptr;
+    if (s->opts.dq_trace) {
+        const int need = s->it + iters;
+        if (need > s->dq_cap) {
+            float *nd = nullptr;
+            int rc = sct_alloc(s->ctx, (void **)&nd, sizeof(float) * size_t(std::max(need, 1)));
+            if (rc) return rc;
+            SCT_HIP(s->ctx, hipMemsetAsync(nd, 0, sizeof(float) * size_t(std::max(need, 1)), s->ctx->stream));
+            if (s->d_dq && s->it)
+                SCT_HIP(s->ctx, hipMemcpyAsync(nd, s->d_dq, sizeof(float) * s->it, hipMemcpyDeviceToDevice, s->ctx->stream));
+            SCT_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
+            sct_free(s->ctx, s->d_dq);
+            s->d_dq = nd;
+            s->dq_cap = need;
+        }
+        dq = s->d_dq + s->it;
+    }
+    int rc = do_iterate(s, iters, dq);
+    if (!rc) s->it += iters;
+    return rc;
+}
+
+extern "C" int sct_potts_solver_labels(sct_potts_solver *s, sct_field *res)
+{
+    if (!s || !res) return SCT_ERR_INVALID;
+    if (res->geo.span() != s->g->geo.span() || res->geo.lines != s->g->geo.lines || res->dtype != SCT_F32)
+        return sct_fail(s->ctx, SCT_ERR_INVALID, "result field layout differs from the input");
+    return do_labels(s, (float *)res->data, s->s0, s->s1);
+}
+
+extern "C" int sct_potts_solver_state(sct_potts_solver *s, float *V, float *xi)
+{
+    if (!s) return SCT_ERR_INVALID;
+    const Geometry &geo = s->g->geo;
+    const size_t span = geo.span();
+    SCT_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    std::vector<float> tmp(s->wspan);
+    auto export_comp = [&](const float *dev, float *host) -> int {
+        SCT_HIP(s->ctx, hipMemcpy(tmp.data(), dev, sizeof(float) * s->wspan, hipMemcpyDeviceToHost));
+        std::fill(host, host + span, 0.0f);
+        const uint32_t len = s->s1 - s->s0;
+        for (uint32_t L = 0; L < geo.lines; ++L)
+            std::memcpy(host + size_t(L) * geo.nstride + s->s0, tmp.data() + size_t(L) * s->wstride, sizeof(float) * len);
+        return SCT_OK;
+    };
+    int rc;
+    if (V)
+        for (int i = 0; i < s->K - 1; ++i)
+            if ((rc = export_comp(s->V + i * s->wspan, V + i * span))) return rc;
+    if (xi)
+        for (int j = 0; j < 3 * (s->K - 1); ++j)
+            if ((rc = export_comp(s->xi + j * s->wspan, xi + j * span))) return rc;
+    return SCT_OK;
+}
+
+extern "C" int sct_potts_solver_dq(sct_potts_solver *s, float *dq, int n)
+{
+    if (!s || !dq || n < 0 || n > s->it || !s->d_dq) return SCT_ERR_INVALID;
+    SCT_HIP(s->ctx, hipMemcpyAsync(dq, s->d_dq, sizeof(float) * size_t(n), hipMemcpyDeviceToHost, s->ctx->stream));
+    SCT_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    return SCT_OK;
+}
+
+extern "C" void sct_potts_solver_destroy(sct_potts_solver *s)
+{
+    if (!s) return;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    release_buffers(s);
+    sct_free(s->ctx, s->d_dq);
+    delete s;
+}
+
+// potts_min_partition<K>, potts-min-partition.cpp:540-679.
+extern "C" int sct_potts_min_partition(sct_ctx *ctx, const sct_field *g, int K, const float *c, float lambda,
+                                       float tau1, float tau2, int iters, const sct_potts_opts *opts,
+                                       sct_field **out)
+{
+    if (!out) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    int rc = validate(ctx, g, K, c, iters);
+    if (rc) return rc;
+    sct_potts_opts o;
+    sct_potts_opts_init(&o);
+    if (opts) std::memcpy(&o, opts, std::min<size_t>(sizeof(o), opts->size ? opts->size : sizeof(o)));
+    PottsPlan plan;
+    if ((rc = potts_plan(ctx, g, K, iters, &o, &plan))) return rc;
+
+    sct_field *res = nullptr;
+    sct_layout l{g->geo.n_images, g->geo.height, g->geo.width, g->geo.nstride, g->geo.col_begin, g->geo.col_end,
+                 g->geo.start_row.data(), g->geo.base.data()};
+    if ((rc = sct_field_create(ctx, &l, SCT_F32, &res))) return rc;
+    SCT_HIP(ctx, hipMemsetAsync(res->data, 0, sizeof(float) * res->geo.span(), ctx->stream));
+
+    sct_potts_solver s;
+    s.ctx = ctx;
+    s.g = g;
+    s.K = K;
+    std::copy(c, c + K, s.c);
+    s.lambda = lambda;
+    s.tau1 = tau1;
+    s.tau2 = tau2;
+    s.opts = o;
+    const int n = int(g->geo.n_images);
+    rc = alloc_buffers(&s, uint32_t(plan.slab_w));
+    float *d_dq = nullptr;
+    if (!rc && o.dq_trace) {
+        rc = sct_alloc(ctx, (void **)&d_dq, sizeof(float) * size_t(std::max(iters, 1)));
+    }
+    for (int k = 0; !rc && k < plan.n_slabs; ++k) {
+        s.s0 = uint32_t(k * plan.res_w);
+        s.s1 = uint32_t(std::min(n, k * plan.res_w + plan.slab_w));
+        if (d_dq) {
+            hipError_t e = hipMemsetAsync(d_dq, 0, sizeof(float) * size_t(std::max(iters, 1)), ctx->stream);
+            if (e != hipSuccess) rc = sct_hip_fail(ctx, e, "hipMemsetAsync");
+        }
+        if (!rc) rc = do_init(&s);
+        if (!rc) rc = do_iterate(&s, iters, d_dq);
+        const uint32_t r0 = s.s0 == 0 ? s.s0 : s.s0 + uint32_t(plan.halo);
+        const uint32_t r1 = int(s.s1) == n ? s.s1 : s.s1 - uint32_t(plan.halo);
+        if (!rc) rc = do_labels(&s, (float *)res->data, r0, r1);
+    }
+    if (!rc && d_dq) {
+        hipError_t e = hipMemcpyAsync(o.dq_trace, d_dq, sizeof(float) * size_t(iters), hipMemcpyDeviceToHost, ctx->stream);
+        if (e != hipSuccess) rc = sct_hip_fail(ctx, e, "dq download");
+    }
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = sct_hip_fail(ctx, e, "potts");
+    }
+    release_buffers(&s);
+    sct_free(ctx, d_dq);
+    if (rc) {
+        sct_field_destroy(res);
+        return rc;
+    }
+    *out = res;
+    return SCT_OK;
+}

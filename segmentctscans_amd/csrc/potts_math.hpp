@@ -1,0 +1,271 @@
+// potts_math.hpp -- per-voxel arithmetic of the Potts primal-dual iteration.
+//
+// Operation order follows potts-model/potts-min-partition.cpp; fused
+// multiply-adds are explicit (__fmaf_rn) and everything else is unfused
+// (the library is compiled with -ffp-contract=off), so results are a
+// deterministic function of the inputs and match the FP32 host mirror bit
+// for bit.  Division and square root are IEEE correctly rounded.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace sct {
+
+// Thresholds that let the kernel skip square roots without changing results:
+//   sqrtf(ss) > 1          <=>  ss > t1      (t1: largest float with sqrtf <= 1)
+//   (double)sqrtf(m) < 1e-6 <=>  m < stol    (stol: smallest float with
+//                                             (double)sqrtf >= 1e-6)
+// Both follow from sqrtf being correctly rounded and monotone; the host
+// computes them once by bisection over float bit patterns.
+struct DykstraLimits {
+    float t1, stol;
+};
+
+template <int K>
+__device__ __forceinline__ void sort_desc(float (&u)[K])
+{
+    // Sorting networks of potts-min-partition.cpp:251-290, insertion sort for
+    // K = 6 (:318-323).  ce(a, b) puts the larger value at a.
+    auto ce = [&](int a, int b) {
+        const float x = u[a], y = u[b];
+        u[a] = fmaxf(x, y);
+        u[b] = fminf(x, y);
+    };
+    if constexpr (K == 2) {
+        ce(0, 1);
+    } else if constexpr (K == 3) {
+        ce(0, 2); ce(0, 1); ce(1, 2);
+    } else if constexpr (K == 4) {
+        ce(0, 2); ce(1, 3); ce(0, 1); ce(2, 3); ce(1, 2);
+    } else if constexpr (K == 5) {
+        ce(0, 3); ce(1, 4); ce(0, 2); ce(1, 3); ce(0, 1); ce(2, 4); ce(1, 2); ce(3, 4); ce(2, 3);
+    } else {
+#pragma unroll
+        for (int i = 1; i < K; ++i)
+#pragma unroll
+            for (int jj = 0; jj < i; ++jj) ce(i - 1 - jj, i - jj);
+    }
+}
+
+// Euclidean projection onto the probability simplex, potts-min-partition.cpp:292-347.
+template <int K>
+__device__ __forceinline__ void project_simplex(float (&q)[K])
+{
+    float u[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) u[i] = q[i];
+    sort_desc<K>(u);
+    float sum = 0.0f, theta = 0.0f;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        sum += u[i];
+        // (rho_sum - 1) / rho equals the candidate computed at index rho - 1,
+        // so the final division of :340 is the one already done here.
+        const float th = (sum - 1.0f) / float(i + 1);
+        if (u[i] > th) theta = th;
+    }
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        const float a = q[i] - theta;
+        q[i] = (a < 0.0f) ? 0.0f : a;    // cuda::std::max(a, 0.0f)
+    }
+}
+
+// Dykstra's alternating projection onto {|q_i1 - q_i2| <= 1 for all pairs},
+// potts-min-partition.cpp:202-238: at most 20 sweeps, stop when the largest
+// correction norm of a sweep is below 1e-6.  Returns the sweeps performed.
+template <int K>
+__device__ __forceinline__ int dykstra(float (&q)[K][3], DykstraLimits lim)
+{
+    constexpr int NP = K * (K - 1) / 2;
+    float r[NP][3];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) r[p][0] = r[p][1] = r[p][2] = 0.0f;
+    int sweep = 0;
+    while (sweep < 20) {
+        ++sweep;
+        float m2 = 0.0f;
+#pragma unroll
+        for (int i1 = 0; i1 < K - 1; ++i1) {
+#pragma unroll
+            for (int i2 = i1 + 1; i2 < K; ++i2) {
+                const int p = (2 * K - 1 - i1) * i1 / 2 + i2 - i1 - 1;
+                const float h0 = (q[i2][0] - q[i1][0]) + r[p][0];
+                const float h1 = (q[i2][1] - q[i1][1]) + r[p][1];
+                const float h2 = (q[i2][2] - q[i1][2]) + r[p][2];
+                const float ss = __fmaf_rn(h2, h2, __fmaf_rn(h1, h1, h0 * h0));
+                float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
+                if (ss > lim.t1) {
+                    const float l = sqrtf(ss);
+                    const float s = (l - 1.0f) / l;
+                    e0 = s * h0;
+                    e1 = s * h1;
+                    e2 = s * h2;
+                }
+                const float d0 = 0.5f * (e0 - r[p][0]);
+                const float d1 = 0.5f * (e1 - r[p][1]);
+                const float d2 = 0.5f * (e2 - r[p][2]);
+                const float dn2 = __fmaf_rn(d2, d2, __fmaf_rn(d1, d1, d0 * d0));
+                m2 = (dn2 < m2) ? m2 : dn2;
+                q[i1][0] += d0; q[i1][1] += d1; q[i1][2] += d2;
+                q[i2][0] -= d0; q[i2][1] -= d1; q[i2][2] -= d2;
+                r[p][0] = e0; r[p][1] = e1; r[p][2] = e2;
+            }
+        }
+        if (m2 < lim.stol) break;
+    }
+    return sweep;
+}
+
+// Xi update of one voxel, potts-min-partition.cpp:135-243.
+//   vb[i]      V-bar at x (explicit components)
+//   vbn[a][i]  V-bar at x + e_a, used when nx[a]
+//   xi[i][a]   in: Xi at x, out: projected Xi
+template <int K>
+__device__ __forceinline__ int xi_update(const float (&vb)[K - 1], const float (&vbn)[3][K - 1],
+                                         const bool (&nx)[3], float (&xi)[K - 1][3], float tau2,
+                                         DykstraLimits lim)
+{
+    float vf[K];
+    float vk = 1.0f;
+#pragma unroll
+    for (int i = 0; i < K - 1; ++i) {
+        vf[i] = vb[i];
+        vk = vk - vb[i];
+    }
+    vf[K - 1] = vk;
+    float q[K][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float pn[K];
+        if (nx[a]) {
+            float vka = 1.0f;
+#pragma unroll
+            for (int i = 0; i < K - 1; ++i) {
+                pn[i] = vbn[a][i];
+                vka = vka - vbn[a][i];
+            }
+            pn[K - 1] = vka;
+#pragma unroll
+            for (int i = 0; i < K; ++i) pn[i] = pn[i] - vf[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < K; ++i) pn[i] = 0.0f;
+        }
+        float xk = 0.0f;
+#pragma unroll
+        for (int i = 0; i < K - 1; ++i) {
+            xk = xk - xi[i][a];
+            q[i][a] = __fmaf_rn(pn[i], tau2, xi[i][a]);
+        }
+        q[K - 1][a] = __fmaf_rn(pn[K - 1], tau2, xk);
+    }
+    const int sweeps = dykstra<K>(q, lim);
+#pragma unroll
+    for (int i = 0; i < K - 1; ++i)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) xi[i][a] = q[i][a];
+    return sweeps;
+}
+
+// V update of one voxel, potts-min-partition.cpp:397-480.
+//   xo[i][a]   Xi at x;  xp[a][i] Xi at x - e_a (used when px[a])
+//   v[i]       in: V at x, out: projected V;  vbar[i] out: 2 V+ - V
+// Returns sum_k (V+_k - V_k)^2 over all K components.
+template <int K>
+__device__ __forceinline__ float v_update(const float (&xo)[K - 1][3], const float (&xp)[3][K - 1],
+                                          const bool (&px)[3], const bool (&nx)[3], float (&v)[K - 1],
+                                          float (&vbar)[K - 1], float u, const float (&c)[K],
+                                          float lambda, float tau1)
+{
+    float xof[K][3], xpf[3][K];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float xk = 0.0f, xkp = 0.0f;
+#pragma unroll
+        for (int i = 0; i < K - 1; ++i) {
+            xof[i][a] = xo[i][a];
+            xk = xk - xo[i][a];
+            xpf[a][i] = xp[a][i];
+            xkp = xkp - xp[a][i];
+        }
+        xof[K - 1][a] = xk;
+        xpf[a][K - 1] = xkp;
+    }
+    float vf[K];
+    float vk = 1.0f;
+#pragma unroll
+    for (int i = 0; i < K - 1; ++i) {
+        vf[i] = v[i];
+        vk = vk - v[i];
+    }
+    vf[K - 1] = vk;
+    float q[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        float div = 0.0f;
+        if (px[0]) div -= xpf[0][i];
+        if (nx[0]) div += xof[i][0];
+        if (px[1]) div -= xpf[1][i];
+        if (nx[1]) div += xof[i][1];
+        if (px[2]) div -= xpf[2][i];
+        if (nx[2]) div += xof[i][2];
+        const float d = u - c[i];
+        q[i] = __fmaf_rn(tau1, __fmaf_rn(-lambda, d * d, div), vf[i]);
+    }
+    project_simplex<K>(q);
+#pragma unroll
+    for (int i = 0; i < K - 1; ++i) {
+        vbar[i] = 2.0f * q[i] - vf[i];
+        v[i] = q[i];
+    }
+    float dq = 0.0f;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const float d = q[k] - vf[k];
+        dq = __fmaf_rn(d, d, dq);
+    }
+    return dq;
+}
+
+// Nearest label, strict < so the lowest index wins ties (potts-min-partition.cpp:81-87).
+template <int K>
+__device__ __forceinline__ int nearest_label(float u, const float (&c)[K])
+{
+    float best = 3.402823466e+38f;
+    int cl = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const float d = fabsf(c[k] - u);
+        if (d < best) {
+            best = d;
+            cl = k;
+        }
+    }
+    return cl;
+}
+
+// Argmax from 0 with strict >, lowest index wins ties (potts-min-partition.cpp:518-533).
+template <int K>
+__device__ __forceinline__ int argmax_label(const float (&v)[K - 1])
+{
+    float best = 0.0f, vk = 1.0f;
+    int lab = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        float vi;
+        if (i < K - 1) {
+            vi = v[i];
+            vk = vk - vi;
+        } else {
+            vi = vk;
+        }
+        if (vi > best) {
+            best = vi;
+            lab = i;
+        }
+    }
+    return lab;
+}
+
+}  // namespace sct

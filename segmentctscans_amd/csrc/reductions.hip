@@ -1,0 +1,417 @@
+// reductions.hip -- min_max, calc_hist, grad_magn, grad_min_max, uv_hist
+// (min_max/min_max.cpp, calc_hist/calc_hist.cpp, grad-magn/grad_magn.cpp,
+// uv-hist/uv-hist-kernel.cpp).
+//
+// All counts are integer adds and all extrema are order-independent, so the
+// results do not depend on scheduling.  Waves stream 64 consecutive n of one
+// line; work is distributed over (line, 64-chunk) pairs.
+#include "sct_internal.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+namespace {
+
+struct VoxGeom {
+    const LineNbr *nbr;
+    uint32_t lines, len, chunks, stride;
+};
+
+VoxGeom vox_geom(const sct_field *f)
+{
+    VoxGeom g;
+    g.nbr = f->geo.d_nbr;
+    g.lines = f->geo.lines;
+    g.len = f->geo.n_images;
+    g.chunks = (g.len + kWave - 1) / kWave;
+    g.stride = f->geo.nstride;
+    return g;
+}
+
+uint32_t stream_grid(const sct_ctx *ctx, const VoxGeom &g)
+{
+    const uint64_t waves = uint64_t(g.lines) * g.chunks;
+    const uint64_t cap = uint64_t(std::max<uint32_t>(ctx->cu_count, 1)) * 16;
+    return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, cap)));
+}
+
+// Block-wide reduction of (mn, mx) followed by one atomic per block on
+// order-preserving keys (min_max.cpp:45-53 uses one device atomic per block).
+__device__ void block_minmax_atomic(float mn, float mx, uint32_t *keys)
+{
+    __shared__ float smn[16], smx[16];
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    const int w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    if ((threadIdx.x % kWave) == 0) {
+        smn[w] = mn;
+        smx[w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < nw; ++i) {
+            mn = fminf(mn, smn[i]);
+            mx = fmaxf(mx, smx[i]);
+        }
+        atomicMin(&keys[0], float_key(mn));
+        atomicMax(&keys[1], float_key(mx));
+    }
+}
+
+// min_max_kernel (min_max.cpp:18-56); POSITIVE: grad_min_max_kernel
+// (uv-hist-kernel.cpp:31-71), which only considers v > 0.
+template <bool POSITIVE>
+__global__ __launch_bounds__(256) void minmax_kernel(VoxGeom g, const float *__restrict__ u, uint32_t *keys)
+{
+    float mn = FLT_MAX, mx = -FLT_MAX;
+    const uint64_t total = uint64_t(g.lines) * g.chunks;
+    const uint64_t nwaves = uint64_t(gridDim.x) * (blockDim.x / kWave);
+    const uint32_t lane = threadIdx.x % kWave;
+    for (uint64_t w = uint64_t(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; w < total; w += nwaves) {
+        const uint32_t line = uint32_t(w / g.chunks);
+        const uint32_t n = uint32_t(w % g.chunks) * kWave + lane;
+        if (n < g.len) {
+            const float x = u[uint64_t(line) * g.stride + n];
+            if (!POSITIVE || 0.0f < x) {
+                mn = (x < mn) ? x : mn;
+                mx = (mx < x) ? x : mx;
+            }
+        }
+    }
+    block_minmax_atomic(mn, mx, keys);
+}
+
+// Philox4x32-10 (Salmon et al., SC'11) for the optional histogram dither.
+__device__ __forceinline__ uint32_t philox_first(uint32_t c0, uint32_t c1, uint32_t k0, uint32_t k1)
+{
+    uint32_t c2 = 0, c3 = 0;
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c0;
+}
+
+struct HistArgs {
+    float rs, re, delta;
+    uint32_t nbins;
+    uint32_t k0, k1;        // dither key
+    uint32_t lines_per_block;
+};
+
+// bin = clamp(int(round(nbins * t - 0.5f)), 0, nbins - 1), t = (x - rs) / (re - rs)
+// (calc_hist.cpp:61-63); nbins * t - 0.5f is one fused multiply-add.
+__device__ __forceinline__ int32_t hist_bin(float x, float rs, float re, uint32_t nbins)
+{
+    const float t = (x - rs) / (re - rs);
+    const int32_t b = int32_t(roundf(__fmaf_rn(float(nbins), t, -0.5f)));
+    return max(0, min(int32_t(nbins) - 1, b));
+}
+
+// calc_hist_kernel (calc_hist.cpp:23-80).  MODE 0: u32 counters in LDS;
+// MODE 1: two u16 counters per LDS word, a block never counts more than
+// 65535 voxels so a half-word cannot carry into its neighbour; MODE 2:
+// device atomics only (bin counts too large for LDS).
+template <int MODE, bool DITHER>
+__global__ __launch_bounds__(256) void hist_kernel(VoxGeom g, const float *__restrict__ u, HistArgs a,
+                                                   uint32_t *__restrict__ counts)
+{
+    extern __shared__ uint32_t sh[];
+    const uint32_t nwords = MODE == 0 ? a.nbins : (MODE == 1 ? (a.nbins + 1) / 2 : 0);
+    if constexpr (MODE != 2) {
+        for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) sh[i] = 0;
+        __syncthreads();
+    }
+    const uint32_t l0 = blockIdx.x * a.lines_per_block;
+    const uint32_t l1 = min(g.lines, l0 + a.lines_per_block);
+    const uint32_t lane = threadIdx.x % kWave, nw = blockDim.x / kWave;
+    const uint32_t pairs = l0 < l1 ? (l1 - l0) * g.chunks : 0;
+    const float half = a.delta / 2;
+    for (uint32_t p = threadIdx.x / kWave; p < pairs; p += nw) {
+        const uint32_t line = l0 + p / g.chunks;
+        const uint32_t n = (p % g.chunks) * kWave + lane;
+        if (n >= g.len) continue;
+        float x = u[uint64_t(line) * g.stride + n];
+        if constexpr (DITHER) {
+            const uint32_t r = philox_first(line, n, a.k0, a.k1);
+            x = x + (-half + a.delta * (float(r >> 8) * (1.0f / 16777216.0f)));
+        }
+        if (a.rs - half <= x && x + half <= a.re) {
+            const int32_t b = hist_bin(x, a.rs, a.re, a.nbins);
+            if constexpr (MODE == 0) atomicAdd(&sh[b], 1u);
+            else if constexpr (MODE == 1) atomicAdd(&sh[b >> 1], 1u << (16 * (b & 1)));
+            else atomicAdd(&counts[b], 1u);
+        }
+    }
+    if constexpr (MODE != 2) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) {
+            const uint32_t c = sh[i];
+            if (c == 0) continue;
+            if constexpr (MODE == 0) {
+                atomicAdd(&counts[i], c);
+            } else {
+                if (c & 0xffffu) atomicAdd(&counts[2 * i], c & 0xffffu);
+                if (c >> 16) atomicAdd(&counts[2 * i + 1], c >> 16);
+            }
+        }
+    }
+}
+
+// grad_kernel of grad_magn.cpp:17-58.
+__global__ __launch_bounds__(256) void grad_magn_kernel(VoxGeom g, const float *__restrict__ u, float *__restrict__ v)
+{
+    const uint64_t w = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t line = uint32_t(w / g.chunks);
+    const uint32_t n = uint32_t(w % g.chunks) * kWave + threadIdx.x % kWave;
+    if (line >= g.lines || n >= g.len) return;
+    const LineNbr nb = g.nbr[line];
+    const uint64_t o = uint64_t(line) * g.stride + n;
+    if (!(n + 1 < g.len) || nb.nrow < 0 || nb.ncol < 0 || !(n > 0) || nb.prow < 0 || nb.pcol < 0) {
+        v[o] = -1.0f;
+        return;
+    }
+    const float q1 = u[o + 1] - u[o - 1];
+    const float q2 = u[uint64_t(nb.nrow) * g.stride + n] - u[uint64_t(nb.prow) * g.stride + n];
+    const float q3 = u[uint64_t(nb.ncol) * g.stride + n] - u[uint64_t(nb.pcol) * g.stride + n];
+    v[o] = 0.5f * sqrtf(__fmaf_rn(q3, q3, __fmaf_rn(q2, q2, q1 * q1)));
+}
+
+struct UvArgs {
+    float u0, u1, v0, v1;
+    uint32_t nbins, hstride;
+};
+
+// The bin-id functor and histogram of uv-hist-kernel.cpp:121-185.
+__global__ __launch_bounds__(256) void uv_hist_kernel(VoxGeom g, const float *__restrict__ u,
+                                                      const float *__restrict__ v, UvArgs a, uint32_t *counts)
+{
+    const uint64_t total = uint64_t(g.lines) * g.chunks;
+    const uint64_t nwaves = uint64_t(gridDim.x) * (blockDim.x / kWave);
+    const uint32_t lane = threadIdx.x % kWave;
+    for (uint64_t w = uint64_t(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; w < total; w += nwaves) {
+        const uint32_t line = uint32_t(w / g.chunks);
+        const uint32_t n = uint32_t(w % g.chunks) * kWave + lane;
+        if (n >= g.len) continue;
+        const uint64_t o = uint64_t(line) * g.stride + n;
+        const float x = u[o], y = v[o];
+        if (!(a.u0 <= x && x <= a.u1 && a.v0 <= y && y <= a.v1)) continue;
+        const int32_t xb = hist_bin(x, a.u0, a.u1, a.nbins);
+        const int32_t yb = hist_bin(y, a.v0, a.v1, a.nbins);
+        atomicAdd(&counts[uint64_t(yb) * a.hstride + uint64_t(xb)], 1u);
+    }
+}
+
+int run_minmax(sct_ctx *ctx, const sct_field *f, bool positive, float *mn, float *mx)
+{
+    if (!ctx || !f || !mn || !mx) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
+    if (f->dtype != SCT_F32) return sct_fail(ctx, SCT_ERR_INVALID, "float field required");
+    const VoxGeom g = vox_geom(f);
+    uint32_t *keys = nullptr;
+    int rc = sct_alloc(ctx, (void **)&keys, 2 * sizeof(uint32_t));
+    if (rc) return rc;
+    uint32_t h[2] = {0, 0};
+    {
+        // keys start at (FLT_MAX, lowest), the reference's initial values (min_max.cpp:61-65)
+        const float init[2] = {FLT_MAX, -FLT_MAX};
+        for (int i = 0; i < 2; ++i) {
+            uint32_t b;
+            std::memcpy(&b, &init[i], 4);
+            h[i] = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+        }
+    }
+    hipError_t e = hipMemcpyAsync(keys, h, sizeof(h), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && g.lines && g.len) {
+        if (positive)
+            minmax_kernel<true><<<stream_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const float *)f->data, keys);
+        else
+            minmax_kernel<false><<<stream_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const float *)f->data, keys);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(h, keys, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    sct_free(ctx, keys);
+    if (e != hipSuccess) return sct_hip_fail(ctx, e, "min_max");
+    *mn = key_float(h[0]);
+    *mx = key_float(h[1]);
+    return SCT_OK;
+}
+
+}  // namespace
+
+extern "C" int sct_min_max(sct_ctx *ctx, const sct_field *u, float *mn, float *mx)
+{
+    return run_minmax(ctx, u, false, mn, mx);
+}
+
+extern "C" int sct_grad_min_max(sct_ctx *ctx, const sct_field *v, float *mn, float *mx)
+{
+    return run_minmax(ctx, v, true, mn, mx);
+}
+
+// calc_hist(ctx, in, range_start, range_end, nbins, delta), calc_hist.cpp:94-138.
+extern "C" int sct_calc_hist(sct_ctx *ctx, const sct_field *f, float rs, float re, uint32_t nbins, float delta,
+                             uint64_t seed, uint32_t *counts, double *density)
+{
+    if (!ctx || !f || !counts) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
+    if (f->dtype != SCT_F32) return sct_fail(ctx, SCT_ERR_INVALID, "float field required");
+    if (nbins == 0) return sct_fail(ctx, SCT_ERR_INVALID, "nbins must be positive");
+    const VoxGeom g = vox_geom(f);
+    uint32_t *d = nullptr;
+    int rc = sct_alloc(ctx, (void **)&d, sizeof(uint32_t) * nbins);
+    if (rc) return rc;
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t) * nbins, ctx->stream);
+    HistArgs a;
+    a.rs = rs;
+    a.re = re;
+    a.delta = delta;
+    a.nbins = nbins;
+    a.k0 = uint32_t(seed);
+    a.k1 = uint32_t(seed >> 32);
+    const bool dither = delta != 0.0f;
+    int mode = nbins <= 32768 ? 0 : ((nbins <= 65536 && g.len <= 65535) ? 1 : 2);
+    size_t lds = mode == 0 ? sizeof(uint32_t) * nbins : (mode == 1 ? sizeof(uint32_t) * ((nbins + 1) / 2) : 0);
+    if (mode == 1) {
+        a.lines_per_block = std::max<uint32_t>(1, 65535u / std::max<uint32_t>(1, g.len));
+    } else {
+        const uint32_t blocks = std::max<uint32_t>(1, ctx->cu_count * 4);
+        a.lines_per_block = std::max<uint32_t>(1, (g.lines + blocks - 1) / blocks);
+    }
+    const uint32_t grid = std::max<uint32_t>(1, (g.lines + a.lines_per_block - 1) / a.lines_per_block);
+    auto launch = [&](auto kernel) -> hipError_t {
+        if (lds > 65536) {
+            hipError_t ea = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+            if (ea != hipSuccess) return ea;
+        }
+        kernel<<<grid, 256, lds, ctx->stream>>>(g, (const float *)f->data, a, d);
+        return hipGetLastError();
+    };
+    if (e == hipSuccess && g.lines && g.len) {
+        if (mode == 0) e = dither ? launch(hist_kernel<0, true>) : launch(hist_kernel<0, false>);
+        else if (mode == 1) e = dither ? launch(hist_kernel<1, true>) : launch(hist_kernel<1, false>);
+        else e = dither ? launch(hist_kernel<2, true>) : launch(hist_kernel<2, false>);
+        if (e != hipSuccess && mode != 2) {
+            // LDS request refused: count with device atomics instead.
+            (void)hipGetLastError();
+            mode = 2;
+            lds = 0;
+            const uint32_t blocks = std::max<uint32_t>(1, ctx->cu_count * 4);
+            a.lines_per_block = std::max<uint32_t>(1, (g.lines + blocks - 1) / blocks);
+            e = hipMemsetAsync(d, 0, sizeof(uint32_t) * nbins, ctx->stream);
+            if (e == hipSuccess) {
+                const uint32_t grid2 = std::max<uint32_t>(1, (g.lines + a.lines_per_block - 1) / a.lines_per_block);
+                if (dither) hist_kernel<2, true><<<grid2, 256, 0, ctx->stream>>>(g, (const float *)f->data, a, d);
+                else hist_kernel<2, false><<<grid2, 256, 0, ctx->stream>>>(g, (const float *)f->data, a, d);
+                e = hipGetLastError();
+            }
+        }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(counts, d, sizeof(uint32_t) * nbins, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    sct_free(ctx, d);
+    if (e != hipSuccess) return sct_hip_fail(ctx, e, "calc_hist");
+    if (density) {
+        // calc_hist.cpp:131-137
+        const double bin_width = (double(re) - double(rs)) / nbins;
+        const double scale = 1.0 / (double(f->geo.data_size) * bin_width);
+        for (uint32_t i = 0; i < nbins; ++i) density[i] = counts[i] * scale;
+    }
+    return SCT_OK;
+}
+
+// calc_hist(ctx, in, nbins, delta), calc_hist.cpp:83-92.
+extern "C" int sct_calc_hist_auto(sct_ctx *ctx, const sct_field *f, uint32_t nbins, float delta, uint64_t seed,
+                                  uint32_t *counts, double *density, float *range_start, float *range_end)
+{
+    float mn = 0, mx = 0;
+    int rc = sct_min_max(ctx, f, &mn, &mx);
+    if (rc) return rc;
+    if (range_start) *range_start = mn;
+    if (range_end) *range_end = mx;
+    return sct_calc_hist(ctx, f, mn, mx, nbins, delta, seed, counts, density);
+}
+
+// grad_magn(), grad_magn.cpp:60-79 (asynchronous, like the reference).
+extern "C" int sct_grad_magn(sct_ctx *ctx, const sct_field *u, sct_field **v)
+{
+    if (!ctx || !u || !v) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
+    *v = nullptr;
+    if (u->dtype != SCT_F32) return sct_fail(ctx, SCT_ERR_INVALID, "float field required");
+    sct_layout l{u->geo.n_images, u->geo.height, u->geo.width, u->geo.nstride, u->geo.col_begin, u->geo.col_end,
+                 u->geo.start_row.data(), u->geo.base.data()};
+    sct_field *out = nullptr;
+    int rc = sct_field_create(ctx, &l, SCT_F32, &out);
+    if (rc) return rc;
+    const VoxGeom g = vox_geom(u);
+    hipError_t e = hipMemsetAsync(out->data, 0, sizeof(float) * out->geo.span(), ctx->stream);
+    if (e == hipSuccess && g.lines && g.len) {
+        const uint32_t grid = uint32_t((uint64_t(g.lines) * g.chunks + 3) / 4);
+        grad_magn_kernel<<<grid, 256, 0, ctx->stream>>>(g, (const float *)u->data, (float *)out->data);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+        sct_field_destroy(out);
+        return sct_hip_fail(ctx, e, "grad_magn");
+    }
+    *v = out;
+    return SCT_OK;
+}
+
+// uv_hist(ctx, u, v, u0, u1, v0, v1, nbins), uv-hist-kernel.cpp:101-198.
+extern "C" int sct_uv_hist(sct_ctx *ctx, const sct_field *u, const sct_field *v, float u_start, float u_end,
+                           float v_start, float v_end, uint32_t nbins, uint32_t *counts, uint32_t *stride)
+{
+    if (!ctx || !u || !v || !counts) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
+    if (u->dtype != SCT_F32 || v->dtype != SCT_F32) return sct_fail(ctx, SCT_ERR_INVALID, "float fields required");
+    if (u->geo.span() != v->geo.span() || u->geo.lines != v->geo.lines)
+        return sct_fail(ctx, SCT_ERR_INVALID, "u and v layouts differ");
+    if (nbins == 0) return sct_fail(ctx, SCT_ERR_INVALID, "nbins must be positive");
+    UvArgs a;
+    a.u0 = u_start;
+    a.u1 = u_end;
+    a.v0 = v_start;
+    a.v1 = v_end;
+    a.nbins = nbins;
+    a.hstride = ((nbins + 31u) / 32u) * 32u;   // 128-byte rows, uv-hist-kernel.cpp:115-121
+    const size_t total = size_t(nbins - 1) * a.hstride + nbins;
+    if (stride) *stride = a.hstride;
+    uint32_t *d = nullptr;
+    int rc = sct_alloc(ctx, (void **)&d, sizeof(uint32_t) * total);
+    if (rc) return rc;
+    const VoxGeom g = vox_geom(u);
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t) * total, ctx->stream);
+    if (e == hipSuccess && g.lines && g.len) {
+        uv_hist_kernel<<<stream_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const float *)u->data, (const float *)v->data, a, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(counts, d, sizeof(uint32_t) * total, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    sct_free(ctx, d);
+    if (e != hipSuccess) return sct_hip_fail(ctx, e, "uv_hist");
+    return SCT_OK;
+}
+
+// uv_hist(ctx, u, v, nbins), uv-hist-kernel.cpp:73-99.
+extern "C" int sct_uv_hist_auto(sct_ctx *ctx, const sct_field *u, const sct_field *v, uint32_t nbins,
+                                uint32_t *counts, uint32_t *stride, float *ranges)
+{
+    float v0 = 0, v1 = 0, u0 = 0, u1 = 0;
+    int rc = sct_grad_min_max(ctx, v, &v0, &v1);
+    if (!rc) rc = sct_min_max(ctx, u, &u0, &u1);
+    if (rc) return rc;
+    if (ranges) {
+        ranges[0] = u0;
+        ranges[1] = u1;
+        ranges[2] = v0;
+        ranges[3] = v1;
+    }
+    return sct_uv_hist(ctx, u, v, u0, u1, v0, v1, nbins, counts, stride);
+}

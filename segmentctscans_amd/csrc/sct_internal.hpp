@@ -1,0 +1,110 @@
+// sct_internal.hpp -- shared host/device plumbing of the HIP library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sct.h"
+
+// ---------------------------------------------------------------------------
+// Line geometry.  The cylinder layout stores each live (row, col) pair as one
+// contiguous line of nstride elements, lines ordered by (col, row); since
+// base[col] is a multiple of nstride, offset(n, row, col) = line * nstride + n
+// (layout_cylinder.h:81-84).  Every stencil needs, per line, the line indices
+// of its four in-plane neighbours, or -1 where the reference's predicates say
+// the neighbour is absent (potts-min-partition.cpp:128,131,390,393).  The
+// table is built once per layout on the host and read once per wave.
+// ---------------------------------------------------------------------------
+struct LineNbr {
+    int32_t prow, nrow, pcol, ncol;
+};
+
+struct Geometry {
+    uint32_t n_images = 0, height = 0, width = 0, nstride = 0, col_begin = 0, col_end = 0;
+    std::vector<uint32_t> start_row, base;   // host copies of the mapping tables
+    uint32_t lines = 0;
+    uint64_t data_size = 0;
+    std::vector<LineNbr> h_nbr;
+    std::vector<uint32_t> line_col, line_row;
+    LineNbr *d_nbr = nullptr;                // device [lines]
+    uint32_t *d_start_row = nullptr, *d_base = nullptr;   // device mapping tables
+
+    uint32_t row_end(uint32_t col) const { return start_row[col] + (base[col + 1] - base[col]) / nstride; }
+    size_t span() const { return base[width]; }
+};
+
+struct sct_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipMemPool_t pool = nullptr;
+    std::string err;
+    uint32_t cu_count = 0;
+};
+
+struct sct_field {
+    sct_ctx *ctx = nullptr;
+    int dtype = SCT_F32;
+    void *data = nullptr;
+    Geometry geo;
+};
+
+// ---- error handling --------------------------------------------------------
+int sct_fail(sct_ctx *ctx, int code, const std::string &msg);
+int sct_hip_fail(sct_ctx *ctx, hipError_t e, const char *what);
+
+#define SCT_HIP(ctx, expr)                                        \
+    do {                                                          \
+        hipError_t e_ = (expr);                                   \
+        if (e_ != hipSuccess) return sct_hip_fail((ctx), e_, #expr); \
+    } while (0)
+
+// ---- memory (stream-ordered, from the device's default pool) --------------
+int sct_alloc(sct_ctx *ctx, void **p, size_t bytes);
+void sct_free(sct_ctx *ctx, void *p);
+
+// ---- geometry construction ------------------------------------------------
+int sct_geometry_build(sct_ctx *ctx, const sct_layout *l, Geometry &g);
+void sct_geometry_release(sct_ctx *ctx, Geometry &g);
+int sct_geometry_copy(sct_ctx *ctx, const Geometry &src, Geometry &dst);
+
+// ---- device helpers --------------------------------------------------------
+constexpr int kWave = 64;
+
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+__device__ __forceinline__ float wave_min(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Atomic max of a non-negative float through its ordered bit pattern.
+__device__ __forceinline__ void atomic_max_nonneg(float *addr, float v)
+{
+    atomicMax(reinterpret_cast<unsigned int *>(addr), __float_as_uint(v));
+}
+
+// Order-preserving float <-> uint key, for exact atomic min/max of any sign.
+__device__ __forceinline__ uint32_t float_key(float f)
+{
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+inline float key_float(uint32_t k)
+{
+    uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}

@@ -1,0 +1,73 @@
+"""Seeded synthetic micro-CT phantoms (see csrc/phantom.h for the construction).
+
+Input tooling for tests and the benchmark; no counterpart in the reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import Field, Mapping
+from . import _native as N
+
+
+class PhantomParams(C.Structure):
+    _fields_ = [
+        ("n_images", C.c_uint32), ("height", C.c_uint32), ("width", C.c_uint32),
+        ("xc", C.c_double), ("yc", C.c_double), ("r", C.c_double),
+        ("rmin", C.c_double), ("rmax", C.c_double), ("porosity", C.c_double),
+        ("c_matrix", C.c_double), ("c_phase1", C.c_double), ("c_phase2", C.c_double),
+        ("p_phase2", C.c_double), ("blur_sigma", C.c_double), ("blur_radius", C.c_int32),
+        ("noise_sd", C.c_double), ("lo", C.c_double), ("hi", C.c_double),
+        ("seed", C.c_uint64), ("max_attempts", C.c_int64),
+    ]
+
+
+class PhantomStats(C.Structure):
+    _fields_ = [("n_spheres", C.c_uint32), ("porosity", C.c_double)]
+
+
+@dataclass
+class Phantom:
+    field: Field            # normalised intensities u = (x - lo) / (hi - lo)
+    labels: np.ndarray      # uint8 [span]: 0 matrix, 1 brine, 2 CO2 (pre-blur truth)
+    params: PhantomParams
+    n_spheres: int
+    porosity: float
+
+    def normalized_labels(self) -> np.ndarray:
+        """The three phase intensities in the normalised units of `field`."""
+        p = self.params
+        return np.array([(c - p.lo) / (p.hi - p.lo) for c in (p.c_matrix, p.c_phase1, p.c_phase2)], np.float32)
+
+
+def default_params(n_images: int, side: int, rmin: float, rmax: float, seed: int) -> PhantomParams:
+    p = PhantomParams()
+    N.phantom_lib().sct_phantom_defaults(C.byref(p), C.c_uint32(n_images), C.c_uint32(side),
+                                         C.c_double(rmin), C.c_double(rmax), C.c_uint64(seed))
+    return p
+
+
+def make_phantom(n_images: int, side: int, rmin: float, rmax: float, seed: int = 0,
+                 nthreads: int = 0, **overrides) -> Phantom:
+    """Cylinder radius 0.45*side about ((side-1)/2, (side-1)/2); spheres U[rmin, rmax]."""
+    p = default_params(n_images, side, rmin, rmax, seed)
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    m = Mapping.cylinder(p.n_images, p.height, p.width, p.xc, p.yc, p.r)
+    u = np.zeros(m.span, np.float32)
+    lab = np.zeros(m.span, np.uint8)
+    st = PhantomStats()
+    sr = np.ascontiguousarray(m.start_row, np.uint32)
+    base = np.ascontiguousarray(m.base, np.uint32)
+    L = N.phantom_lib()
+    rc = L.sct_phantom_generate(C.byref(p), sr.ctypes.data_as(N.u32p), base.ctypes.data_as(N.u32p),
+                                C.c_uint32(m.nstride), C.c_uint32(m.col_begin), C.c_uint32(m.col_end),
+                                u.ctypes.data_as(N.f32p), lab.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                C.byref(st), C.c_int(nthreads))
+    if rc:
+        raise RuntimeError(f"sct_phantom_generate failed ({rc})")
+    f = Field(m, u, lo=p.lo, hi=p.hi)
+    return Phantom(f, lab, p, st.n_spheres, st.porosity)
