@@ -161,53 +161,7 @@ uint32_t grid_for(const WorkGeom &g)
     return uint32_t((waves + 3) / 4);
 }
 
-// sqrtf-free thresholds (see DykstraLimits).
-DykstraLimits dykstra_limits()
-{
-    static DykstraLimits lim = [] {
-        auto bits = [](float f) { uint32_t b; std::memcpy(&b, &f, 4); return b; };
-        auto flt = [](uint32_t b) { float f; std::memcpy(&f, &b, 4); return f; };
-        // t1: largest x >= 0 with sqrtf(x) <= 1.
-        uint32_t lo = 0, hi = bits(4.0f);
-        while (hi - lo > 1) {
-            uint32_t mid = lo + (hi - lo) / 2;
-            if (std::sqrt(flt(mid)) <= 1.0f) lo = mid; else hi = mid;
-        }
-        DykstraLimits l;
-        l.t1 = flt(lo);
-        // stol: smallest x >= 0 with (double)sqrtf(x) >= 1e-6.
-        lo = 0;
-        hi = bits(1.0f);
-        while (hi - lo > 1) {
-            uint32_t mid = lo + (hi - lo) / 2;
-            if (double(std::sqrt(flt(mid))) >= 1e-6) hi = mid; else lo = mid;
-        }
-        l.stol = flt(hi);
-        return l;
-    }();
-    return lim;
-}
-
 }  // namespace
-
-// ---------------------------------------------------------------------------
-// Solver object
-// ---------------------------------------------------------------------------
-struct sct_potts_solver {
-    sct_ctx *ctx = nullptr;
-    const sct_field *g = nullptr;
-    int K = 0;
-    float c[6] = {};
-    float lambda = 0, tau1 = 0, tau2 = 0;
-    sct_potts_opts opts{};
-    int kernel = SCT_KERNEL_TWOPASS;
-    uint32_t s0 = 0, s1 = 0, wstride = 0;
-    uint64_t wspan = 0;
-    float *xi = nullptr, *V = nullptr, *vbar = nullptr;
-    float *d_dq = nullptr;
-    int dq_cap = 0;
-    int it = 0;   // iterations done on this slab
-};
 
 namespace {
 
@@ -233,10 +187,11 @@ int launch_init(sct_potts_solver *s)
     Labels<K> lab;
     for (int i = 0; i < K; ++i) lab.c[i] = s->c[i];
     if (g.lines && g.len) {
-        potts_init_kernel<K><<<grid_for(g), 256, 0, s->ctx->stream>>>(g, (const float *)s->g->data, s->V, s->vbar, lab);
+        potts_init_kernel<K><<<grid_for(g), 256, 0, s->ctx->stream>>>(g, (const float *)s->g->data, s->V, s->vbar[0], lab);
         SCT_HIP(s->ctx, hipGetLastError());
     }
-    SCT_HIP(s->ctx, hipMemsetAsync(s->xi, 0, sizeof(float) * 3 * (K - 1) * s->wspan, s->ctx->stream));
+    s->cur = 0;
+    SCT_HIP(s->ctx, hipMemsetAsync(s->xi[0], 0, sizeof(float) * 3 * (K - 1) * s->wspan, s->ctx->stream));
     return SCT_OK;
 }
 
@@ -245,17 +200,18 @@ int launch_iterate(sct_potts_solver *s, int iters, float *dq_dev)
 {
     const WorkGeom g = make_geom(s);
     if (!g.lines || !g.len) return SCT_OK;
+    if (s->kernel == SCT_KERNEL_FUSED) return fused_iterate(s, iters, dq_dev);
     Labels<K> lab;
     for (int i = 0; i < K; ++i) lab.c[i] = s->c[i];
     const DykstraLimits lim = dykstra_limits();
     const uint32_t grid = grid_for(g);
     for (int i = 0; i < iters; ++i) {
-        potts_xi_kernel<K><<<grid, 256, 0, s->ctx->stream>>>(g, s->xi, s->vbar, s->tau2, lim);
+        potts_xi_kernel<K><<<grid, 256, 0, s->ctx->stream>>>(g, s->xi[0], s->vbar[0], s->tau2, lim);
         if (dq_dev)
-            potts_v_kernel<K, true><<<grid, 256, 0, s->ctx->stream>>>(g, s->xi, s->V, s->vbar, (const float *)s->g->data,
+            potts_v_kernel<K, true><<<grid, 256, 0, s->ctx->stream>>>(g, s->xi[0], s->V, s->vbar[0], (const float *)s->g->data,
                                                                      lab, s->lambda, s->tau1, dq_dev + i);
         else
-            potts_v_kernel<K, false><<<grid, 256, 0, s->ctx->stream>>>(g, s->xi, s->V, s->vbar, (const float *)s->g->data,
+            potts_v_kernel<K, false><<<grid, 256, 0, s->ctx->stream>>>(g, s->xi[0], s->V, s->vbar[0], (const float *)s->g->data,
                                                                       lab, s->lambda, s->tau1, nullptr);
     }
     SCT_HIP(s->ctx, hipGetLastError());
@@ -288,15 +244,28 @@ int do_init(sct_potts_solver *s) { SCT_DISPATCH_K(s->K, launch_init, s); }
 int do_iterate(sct_potts_solver *s, int iters, float *dq) { SCT_DISPATCH_K(s->K, launch_iterate, s, iters, dq); }
 int do_labels(sct_potts_solver *s, float *res, uint32_t r0, uint32_t r1) { SCT_DISPATCH_K(s->K, launch_labels, s, res, r0, r1); }
 
-// Words of solver state per voxel of a slab.
-uint64_t words_per_voxel(int K) { return uint64_t(5) * (K - 1); }
+// Words of solver state per voxel of a slab: Xi 3(K-1), V (K-1), V-bar (K-1),
+// with Xi and V-bar doubled for the fused kernel's ping-pong.
+uint64_t words_per_voxel(int K, int kernel)
+{
+    return uint64_t(kernel == SCT_KERNEL_FUSED ? 9 : 5) * (K - 1);
+}
+
+int pick_kernel(const sct_potts_opts &o)
+{
+    return o.kernel == SCT_KERNEL_TWOPASS ? SCT_KERNEL_TWOPASS : SCT_KERNEL_FUSED;
+}
 
 void release_buffers(sct_potts_solver *s)
 {
-    sct_free(s->ctx, s->xi);
+    for (int b = 0; b < 2; ++b) {
+        sct_free(s->ctx, s->xi[b]);
+        sct_free(s->ctx, s->vbar[b]);
+        s->xi[b] = s->vbar[b] = nullptr;
+    }
     sct_free(s->ctx, s->V);
-    sct_free(s->ctx, s->vbar);
-    s->xi = s->V = s->vbar = nullptr;
+    s->V = nullptr;
+    fused_release(s);
 }
 
 int alloc_buffers(sct_potts_solver *s, uint32_t slab_len)
@@ -305,10 +274,14 @@ int alloc_buffers(sct_potts_solver *s, uint32_t slab_len)
     s->wstride = sct_get_stride(slab_len);
     s->wspan = uint64_t(geo.lines) * s->wstride;
     const size_t comp = sizeof(float) * std::max<uint64_t>(1, s->wspan);
+    const int nb = s->kernel == SCT_KERNEL_FUSED ? 2 : 1;
     int rc;
-    if ((rc = sct_alloc(s->ctx, (void **)&s->xi, comp * 3 * (s->K - 1)))) return rc;
+    for (int b = 0; b < nb; ++b) {
+        if ((rc = sct_alloc(s->ctx, (void **)&s->xi[b], comp * 3 * (s->K - 1)))) return rc;
+        if ((rc = sct_alloc(s->ctx, (void **)&s->vbar[b], comp * (s->K - 1)))) return rc;
+    }
     if ((rc = sct_alloc(s->ctx, (void **)&s->V, comp * (s->K - 1)))) return rc;
-    if ((rc = sct_alloc(s->ctx, (void **)&s->vbar, comp * (s->K - 1)))) return rc;
+    if (s->kernel == SCT_KERNEL_FUSED && (rc = fused_prepare(s))) return rc;
     return SCT_OK;
 }
 
@@ -323,6 +296,34 @@ int validate(sct_ctx *ctx, const sct_field *g, int K, const float *c, int iters)
 
 }  // namespace
 
+// sqrtf-free thresholds (see DykstraLimits).
+DykstraLimits sct::dykstra_limits()
+{
+    static DykstraLimits lim = [] {
+        auto bits = [](float f) { uint32_t b; std::memcpy(&b, &f, 4); return b; };
+        auto flt = [](uint32_t b) { float f; std::memcpy(&f, &b, 4); return f; };
+        // t1: largest x >= 0 with sqrtf(x) <= 1.
+        uint32_t lo = 0, hi = bits(4.0f);
+        while (hi - lo > 1) {
+            uint32_t mid = lo + (hi - lo) / 2;
+            if (std::sqrt(flt(mid)) <= 1.0f) lo = mid; else hi = mid;
+        }
+        DykstraLimits l;
+        l.t1 = flt(lo);
+        // stol: smallest x >= 0 with (double)sqrtf(x) >= 1e-6.
+        lo = 0;
+        hi = bits(1.0f);
+        while (hi - lo > 1) {
+            uint32_t mid = lo + (hi - lo) / 2;
+            if (double(std::sqrt(flt(mid))) >= 1e-6) hi = mid; else lo = mid;
+        }
+        l.stol = flt(hi);
+        return l;
+    }();
+    return lim;
+}
+
+
 extern "C" void sct_potts_opts_init(sct_potts_opts *o)
 {
     std::memset(o, 0, sizeof(*o));
@@ -331,7 +332,7 @@ extern "C" void sct_potts_opts_init(sct_potts_opts *o)
 }
 
 // Slab plan, potts-min-partition.cpp:560-577 and :627-630.
-int sct::potts_plan(sct_ctx *ctx, const sct_field *g, int K, int iters, const sct_potts_opts *o,
+int sct::potts_plan(sct_ctx *ctx, const sct_field *g, int K, int kernel, int iters, const sct_potts_opts *o,
                     PottsPlan *plan)
 {
     const Geometry &geo = g->geo;
@@ -347,7 +348,7 @@ int sct::potts_plan(sct_ctx *ctx, const sct_field *g, int K, int iters, const sc
             int rc = sct_device_memory(ctx, &avail, &total);
             if (rc) return rc;
         }
-        const uint64_t per_slice = sizeof(float) * uint64_t(std::max<uint32_t>(1, geo.lines)) * words_per_voxel(K);
+        const uint64_t per_slice = sizeof(float) * uint64_t(std::max<uint32_t>(1, geo.lines)) * words_per_voxel(K, kernel);
         const uint64_t w = avail / per_slice;
         slab_w = int(sct_prev_stride(uint32_t(std::min<uint64_t>(w, 0x7fffffffu))));
     }
@@ -384,6 +385,7 @@ extern "C" int sct_potts_solver_create(sct_ctx *ctx, const sct_field *g, int K, 
     s->tau2 = tau2;
     if (opts) std::memcpy(&s->opts, opts, std::min<size_t>(sizeof(s->opts), opts->size ? opts->size : sizeof(s->opts)));
     else sct_potts_opts_init(&s->opts);
+    s->kernel = pick_kernel(s->opts);
     s->s0 = 0;
     s->s1 = g->geo.n_images;
     rc = alloc_buffers(s, s->s1);
@@ -451,7 +453,7 @@ extern "C" int sct_potts_solver_state(sct_potts_solver *s, float *V, float *xi)
             if ((rc = export_comp(s->V + i * s->wspan, V + i * span))) return rc;
     if (xi)
         for (int j = 0; j < 3 * (s->K - 1); ++j)
-            if ((rc = export_comp(s->xi + j * s->wspan, xi + j * span))) return rc;
+            if ((rc = export_comp(s->xi[s->cur] + j * s->wspan, xi + j * span))) return rc;
     return SCT_OK;
 }
 
@@ -485,7 +487,7 @@ extern "C" int sct_potts_min_partition(sct_ctx *ctx, const sct_field *g, int K, 
     sct_potts_opts_init(&o);
     if (opts) std::memcpy(&o, opts, std::min<size_t>(sizeof(o), opts->size ? opts->size : sizeof(o)));
     PottsPlan plan;
-    if ((rc = potts_plan(ctx, g, K, iters, &o, &plan))) return rc;
+    if ((rc = potts_plan(ctx, g, K, pick_kernel(o), iters, &o, &plan))) return rc;
 
     sct_field *res = nullptr;
     sct_layout l{g->geo.n_images, g->geo.height, g->geo.width, g->geo.nstride, g->geo.col_begin, g->geo.col_end,
@@ -502,6 +504,7 @@ extern "C" int sct_potts_min_partition(sct_ctx *ctx, const sct_field *g, int K, 
     s.tau1 = tau1;
     s.tau2 = tau2;
     s.opts = o;
+    s.kernel = pick_kernel(o);
     const int n = int(g->geo.n_images);
     rc = alloc_buffers(&s, uint32_t(plan.slab_w));
     float *d_dq = nullptr;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "sct_internal.hpp"
+#include "potts_math.hpp"
 
 namespace sct {
 
@@ -13,6 +14,57 @@ struct PottsPlan {
     int n_slabs = 0;
 };
 
-int potts_plan(sct_ctx *ctx, const sct_field *g, int K, int iters, const sct_potts_opts *o, PottsPlan *plan);
+// Per-column row range and first line, for kernels that march along col.
+struct ColInfo {
+    uint32_t line0;   // line index of (start_row, col)
+    uint32_t rb, re;  // live rows [rb, re); rb == re for dead columns
+    uint32_t pad;
+};
+
+// A band of rows marched over the columns [cs, ce) where any of its rows is live.
+struct BandInfo {
+    uint32_t r0;      // first row of the band
+    uint32_t cs, ce;  // column range
+    uint32_t pad;
+};
+
+DykstraLimits dykstra_limits();
+
+}  // namespace sct
+
+struct sct_potts_solver {
+    sct_ctx *ctx = nullptr;
+    const sct_field *g = nullptr;
+    int K = 0;
+    float c[6] = {};
+    float lambda = 0, tau1 = 0, tau2 = 0;
+    sct_potts_opts opts{};
+    int kernel = SCT_KERNEL_FUSED;
+    uint32_t s0 = 0, s1 = 0, wstride = 0;
+    uint64_t wspan = 0;
+    // Xi and V-bar are double-buffered for the fused kernel (read [cur], write
+    // [cur ^ 1]); the two-pass kernels update [0] in place.
+    float *xi[2] = {nullptr, nullptr};
+    float *vbar[2] = {nullptr, nullptr};
+    float *V = nullptr;
+    int cur = 0;
+    float *d_dq = nullptr;
+    int dq_cap = 0;
+    int it = 0;   // iterations done on this slab
+    // fused-kernel tables (device)
+    sct::ColInfo *d_cols = nullptr;
+    sct::BandInfo *d_bands = nullptr;
+    uint32_t n_bands = 0;
+};
+
+namespace sct {
+
+int potts_plan(sct_ctx *ctx, const sct_field *g, int K, int kernel, int iters, const sct_potts_opts *o,
+               PottsPlan *plan);
+
+// Fused one-pass iteration (potts_fused.hip).
+int fused_prepare(sct_potts_solver *s);
+int fused_iterate(sct_potts_solver *s, int iters, float *dq_dev);
+void fused_release(sct_potts_solver *s);
 
 }  // namespace sct

@@ -71,3 +71,23 @@ def make_phantom(n_images: int, side: int, rmin: float, rmax: float, seed: int =
         raise RuntimeError(f"sct_phantom_generate failed ({rc})")
     f = Field(m, u, lo=p.lo, hi=p.hi)
     return Phantom(f, lab, p, st.n_spheres, st.porosity)
+
+
+def crop_box(ph: Phantom, n0: int, r0: int, c0: int, size: int) -> Field:
+    """A size^3 box cut out of a phantom (rows/cols/n starting at r0/c0/n0),
+    stored in a box layout (cylinder radius larger than the box diagonal).
+    Every (row, col) of the box must lie inside the phantom's mask."""
+    m = ph.field.mapping
+    box = Mapping.cylinder(size, size, size, (size - 1) / 2, (size - 1) / 2, 4.0 * size)
+    assert box.lines == size * size
+    out = np.zeros(box.span, np.float32)
+    src = ph.field.data
+    for col in range(size):
+        cc = c0 + col
+        rb, re = m.row_begin(cc), m.row_end(cc)
+        if not (rb <= r0 and r0 + size <= re):
+            raise ValueError("crop leaves the mask")
+        line0 = int(m.base[cc]) // m.nstride + (r0 - rb)
+        blk = src[line0 * m.nstride:(line0 + size) * m.nstride].reshape(size, m.nstride)[:, n0:n0 + size]
+        out[col * size * box.nstride:(col + 1) * size * box.nstride].reshape(size, box.nstride)[:, :size] = blk
+    return Field(box, out, lo=ph.field.lo, hi=ph.field.hi)
