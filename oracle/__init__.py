@@ -47,9 +47,13 @@ def lib() -> C.CDLL:
         L.or_dykstra_d.argtypes = [C.c_int, f64p, C.c_int, C.c_double]
         L.or_dykstra_f.argtypes = [C.c_int, f32p, C.c_int, C.c_double]
         L.or_potts_d.argtypes = [C.POINTER(OrLayout), f32p, C.c_int, f32p, C.c_float, C.c_float, C.c_float,
-                                 C.c_int, C.c_int, f32p, f64p, f64p, f64p, C.c_int]
+                                 C.c_int, C.c_int, f32p, f64p, f64p, f64p, C.c_int, C.c_int]
         L.or_potts_f.argtypes = [C.POINTER(OrLayout), f32p, C.c_int, f32p, C.c_float, C.c_float, C.c_float,
-                                 C.c_int, C.c_int, f32p, f32p, f32p, f32p, C.c_int]
+                                 C.c_int, C.c_int, f32p, f32p, f32p, f32p, C.c_int, C.c_int]
+        L.or_bf16_round.argtypes = [C.c_float]
+        L.or_bf16_round.restype = C.c_float
+        L.or_f16_round.argtypes = [C.c_float]
+        L.or_f16_round.restype = C.c_float
         L.or_chambolle_d.argtypes = [C.POINTER(OrLayout), f64p, C.c_float, C.c_float, C.c_int, f64p, f64p, C.c_int]
         L.or_chambolle_f.argtypes = [C.POINTER(OrLayout), f32p, C.c_float, C.c_float, C.c_int, f32p, f32p, C.c_int]
         L.or_calc_hist.argtypes = [C.POINTER(OrLayout), f32p, C.c_float, C.c_float, C.c_uint32, u32p]
@@ -100,8 +104,11 @@ class Layout:
                         _ptr(self.start_row, u32p), _ptr(self.base, u32p))
 
 
+STORAGE = {"fp32": 0, "bf16": 1, "fp16": 2}
+
+
 def potts(layout: Layout, u: np.ndarray, c, lam, tau1, tau2, iters, *, precision="f64", slab_w=0,
-          want_state=False, nthreads=0):
+          want_state=False, nthreads=0, storage="fp32"):
     """Returns dict(res, V, xi, dq, rc)."""
     c = np.ascontiguousarray(c, np.float32)
     K = c.size
@@ -115,12 +122,19 @@ def potts(layout: Layout, u: np.ndarray, c, lam, tau1, tau2, iters, *, precision
     lay = layout.c()
     u = np.ascontiguousarray(u, np.float32)
     rc = fn(C.byref(lay), _ptr(u, f32p), K, _ptr(c, f32p), lam, tau1, tau2, iters, slab_w, _ptr(res, f32p),
-            _ptr(V, tp), _ptr(xi, tp), _ptr(dq, tp), nthreads)
+            _ptr(V, tp), _ptr(xi, tp), _ptr(dq, tp), STORAGE[storage], nthreads)
     out = dict(rc=rc, res=res, dq=dq[:iters])
     if want_state:
         out["V"] = V.reshape(K - 1, layout.span)
         out["xi"] = xi.reshape(3 * (K - 1), layout.span)
     return out
+
+
+def store_round(x, storage: str) -> np.ndarray:
+    """Round float32 values through bfloat16 / half storage (oracle's rounding)."""
+    fn = {"bf16": lib().or_bf16_round, "fp16": lib().or_f16_round}[storage]
+    x = np.asarray(x, np.float32)
+    return np.array([fn(float(v)) for v in x.reshape(-1)], np.float32).reshape(x.shape)
 
 
 def chambolle(layout: Layout, g: np.ndarray, lam, tau, iters, *, precision="f32", nthreads=0):

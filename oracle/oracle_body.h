@@ -166,7 +166,7 @@ static inline void FN(potts_xi_voxel)(const FN(slab_t) *S, int K, uint32_t n, ui
 /* potts-min-partition.cpp:349-488 for one voxel; returns sum_k (q_k - v_k)^2. */
 static inline REAL FN(potts_v_voxel)(const FN(slab_t) *S, int K, uint32_t n, uint32_t row,
                                      uint32_t col, const float *u, const REAL *cc, REAL lambda,
-                                     REAL tau1, const REAL *xi, REAL *V, REAL *vbar)
+                                     REAL tau1, const REAL *xi, REAL *V, REAL *vbar, int storage)
 {
     const or_layout *m = S->m;
     size_t L = line_of(m, row, col);
@@ -213,7 +213,7 @@ static inline REAL FN(potts_v_voxel)(const FN(slab_t) *S, int K, uint32_t n, uin
     FN(proj_simplex_)(K, q);
     REAL dq = 0;
     for (int i = 0; i < K - 1; ++i) {
-        vbar[i * ws + w] = (REAL)2 * q[i] - v[i];       /* :471-472 */
+        vbar[i * ws + w] = STORE_ROUND((REAL)2 * q[i] - v[i], storage);   /* :471-472 */
         V[i * ws + w] = q[i];
     }
     for (int k = 0; k < K; ++k) { REAL d = q[k] - v[k]; dq = FMA(d, d, dq); }  /* :474-480 */
@@ -222,9 +222,9 @@ static inline REAL FN(potts_v_voxel)(const FN(slab_t) *S, int K, uint32_t n, uin
 
 int FN(or_potts)(const or_layout *m, const float *u, int K, const float *c,
                  float lambda, float tau1, float tau2, int iters, int slab_w,
-                 float *res, REAL *V_out, REAL *xi_out, REAL *dq, int nthreads)
+                 float *res, REAL *V_out, REAL *xi_out, REAL *dq, int storage, int nthreads)
 {
-    if (K < 2 || K > 6 || iters < 0) return 2;
+    if (K < 2 || K > 6 || iters < 0 || storage < 0 || storage > 2) return 2;
     const uint32_t n = m->n_images;
     const int halo = iters;
     int single = (slab_w <= 0) || ((uint32_t)slab_w >= n);
@@ -282,10 +282,16 @@ int FN(or_potts)(const or_layout *m, const float *u, int K, const float *c,
                 for (uint32_t row = row_begin(m, col); row < row_end(m, col); ++row)
                     for (uint32_t nn = S.s0; nn < S.s1; ++nn) {
                         REAL d = FN(potts_v_voxel)(&S, K, nn, row, col, u, cc, (REAL)lambda,
-                                                   (REAL)tau1, xi, V, vbar);
+                                                   (REAL)tau1, xi, V, vbar, storage);
                         if (d > mx) mx = d;
                     }
             if (dq) dq[it] = mx;
+            /* Reduced-precision storage: the V step above used Xi+ as computed;
+             * the stored Xi (input of the next iteration) is rounded. */
+            if (storage) {
+#pragma omp parallel for schedule(static)
+                for (size_t k = 0; k < (size_t)3 * (K - 1) * wspan; ++k) xi[k] = STORE_ROUND(xi[k], storage);
+            }
         }
         /* update_kernel over the slab interior, :491-538 and :663-674 */
         uint32_t r0 = (S.s0 == 0) ? S.s0 : S.s0 + (uint32_t)halo;

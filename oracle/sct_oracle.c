@@ -117,9 +117,82 @@ void or_div_d(const or_layout *m, const double *p0, const double *p1, const doub
             }
 }
 
+/* ---- reduced-precision storage --------------------------------------------
+ * Round-to-nearest-even float -> bfloat16 / IEEE half -> float, written on
+ * the bit patterns (the HIP kernels use the same rounding).  Only the FP32
+ * mirror rounds; the FP64 oracle ignores the storage mode. */
+static inline float bf16_round(float f)
+{
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    if ((x & 0x7fffffffu) > 0x7f800000u) {
+        x |= 0x00400000u;                      /* quiet NaN */
+    } else {
+        x += 0x7fffu + ((x >> 16) & 1u);
+    }
+    x &= 0xffff0000u;
+    memcpy(&f, &x, 4);
+    return f;
+}
+
+static inline uint16_t f32_to_f16_bits(float f)
+{
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) return (uint16_t)(sign | (ax > 0x7f800000u ? 0x7e00u : 0x7c00u));
+    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);          /* rounds to infinity */
+    if (ax < 0x38800000u) {                                             /* half subnormal or zero */
+        const int e = (int)(ax >> 23);
+        const int shift = 126 - e;
+        if (e == 0 || shift > 24) return (uint16_t)sign;
+        const uint32_t mant = (ax & 0x7fffffu) | 0x800000u;
+        uint32_t q = mant >> shift;
+        const uint32_t rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (q & 1u))) ++q;
+        return (uint16_t)(sign | q);
+    }
+    uint32_t r = ax - 0x38000000u;                                       /* rebias 127 -> 15 */
+    r += 0xfffu + ((r >> 13) & 1u);
+    return (uint16_t)(sign | (r >> 13));
+}
+
+static inline float f16_bits_to_f32(uint16_t h)
+{
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    uint32_t e = (h >> 10) & 0x1fu, mant = h & 0x3ffu, x;
+    if (e == 0x1f) {
+        x = sign | 0x7f800000u | (mant << 13);
+    } else if (e == 0) {
+        if (mant == 0) {
+            x = sign;
+        } else {                                                        /* normalise the subnormal */
+            int k = 0;
+            while (!(mant & 0x400u)) { mant <<= 1; ++k; }
+            x = sign | ((uint32_t)(113 - k) << 23) | ((mant & 0x3ffu) << 13);
+        }
+    } else {
+        x = sign | ((e + 112u) << 23) | (mant << 13);
+    }
+    float f;
+    memcpy(&f, &x, 4);
+    return f;
+}
+
+static inline float f16_round(float f) { return f16_bits_to_f32(f32_to_f16_bits(f)); }
+
+static inline float store_round_f(float f, int storage)
+{
+    return storage == 1 ? bf16_round(f) : (storage == 2 ? f16_round(f) : f);
+}
+
+float or_bf16_round(float f) { return bf16_round(f); }
+float or_f16_round(float f) { return f16_round(f); }
+
 /* ---- precision-generic solvers ----------------------------------------- */
 #define REAL double
 #define SFX _d
+#define STORE_ROUND(x, storage) ((void)(storage), (x))
 #define REAL_MAX DBL_MAX
 #define FMA fma
 #define SQRT sqrt
@@ -135,8 +208,10 @@ void or_div_d(const or_layout *m, const double *p0, const double *p1, const doub
 #undef FABS
 #undef FMAX
 #undef FMIN
+#undef STORE_ROUND
 
 #define REAL float
+#define STORE_ROUND(x, storage) store_round_f((x), (storage))
 #define SFX _f
 #define REAL_MAX FLT_MAX
 #define FMA fmaf

@@ -77,10 +77,14 @@ int or_dykstra_f(int K, float *q, int max_sweeps, double tol);
  * out-of-memory return), or 2 on invalid arguments. */
 int or_potts_d(const or_layout *m, const float *u, int K, const float *c,
                float lambda, float tau1, float tau2, int iters, int slab_w,
-               float *res, double *V_out, double *xi_out, double *dq, int nthreads);
+               float *res, double *V_out, double *xi_out, double *dq, int storage, int nthreads);
+/* storage: 0 FP32; 1 Xi and V-bar stored as bfloat16; 2 as IEEE half (the
+ * V step of an iteration uses that iteration's Xi+ before rounding). */
 int or_potts_f(const or_layout *m, const float *u, int K, const float *c,
                float lambda, float tau1, float tau2, int iters, int slab_w,
-               float *res, float *V_out, float *xi_out, float *dq, int nthreads);
+               float *res, float *V_out, float *xi_out, float *dq, int storage, int nthreads);
+float or_bf16_round(float f);
+float or_f16_round(float f);
 
 /* Chambolle ROF, chambolle.cpp:225-276.  g is overwritten with the result.
  * xi_out optional (3 x span), dq optional [iters]. */

@@ -81,11 +81,22 @@ def build_oracle() -> Path:
     return ROOT / "oracle" / "_build" / "libsct_oracle.so"
 
 
+def build_cpp_example() -> Path:
+    """tests/cpp/api_example: the C++ API (include/segmentctscans.hpp) over libsct.so."""
+    out_dir = ROOT / "tests" / "cpp" / "_build"
+    out_dir.mkdir(exist_ok=True)
+    target = out_dir / "api_example"
+    _run(["g++", "-O2", "-std=c++17", "-I", str(ROOT / "include"), str(ROOT / "tests" / "cpp" / "api_example.cpp"),
+          "-L", str(LIB), "-lsct", "-Wl,-rpath,$ORIGIN/../../../segmentctscans_amd/_lib", "-o", str(target)])
+    return target
+
+
 def build_all() -> None:
     with cf.ThreadPoolExecutor(max_workers=3) as ex:
         futs = [ex.submit(build_hip), ex.submit(build_phantom), ex.submit(build_oracle)]
         for f in futs:
             f.result()
+    build_cpp_example()
 
 
 if __name__ == "__main__":

@@ -18,6 +18,23 @@ using namespace sct;
 
 namespace {
 
+size_t storage_bytes(int storage) { return storage == SCT_STORAGE_F32 ? 4 : 2; }
+
+float bf16_to_float(uint16_t h)
+{
+    const uint32_t x = uint32_t(h) << 16;
+    float f;
+    std::memcpy(&f, &x, 4);
+    return f;
+}
+
+float half_to_float(uint16_t h)
+{
+    _Float16 v;
+    std::memcpy(&v, &h, 2);
+    return float(v);
+}
+
 struct WorkGeom {
     const LineNbr *nbr;
     uint32_t lines;      // number of lines
@@ -44,9 +61,9 @@ struct Labels {
 };
 
 // init_v_kernel, potts-min-partition.cpp:56-94.
-template <int K>
+template <int K, typename St>
 __global__ __launch_bounds__(256) void potts_init_kernel(WorkGeom g, const float *__restrict__ u,
-                                                         float *__restrict__ V, float *__restrict__ vbar,
+                                                         float *__restrict__ V, typename St::T *__restrict__ vbar,
                                                          Labels<K> lab)
 {
     uint32_t line, n;
@@ -58,7 +75,7 @@ __global__ __launch_bounds__(256) void potts_init_kernel(WorkGeom g, const float
     for (int i = 0; i < K - 1; ++i) {
         const float o = (i == cl) ? 1.0f : 0.0f;
         V[i * g.wspan + w] = o;
-        vbar[i * g.wspan + w] = o;
+        vbar[i * g.wspan + w] = St::st(o);
     }
 }
 
@@ -187,11 +204,17 @@ int launch_init(sct_potts_solver *s)
     Labels<K> lab;
     for (int i = 0; i < K; ++i) lab.c[i] = s->c[i];
     if (g.lines && g.len) {
-        potts_init_kernel<K><<<grid_for(g), 256, 0, s->ctx->stream>>>(g, (const float *)s->g->data, s->V, s->vbar[0], lab);
+        const float *u = (const float *)s->g->data;
+        if (s->storage == SCT_STORAGE_BF16)
+            potts_init_kernel<K, StorageBF16><<<grid_for(g), 256, 0, s->ctx->stream>>>(g, u, s->V, (uint16_t *)s->vbar[0], lab);
+        else if (s->storage == SCT_STORAGE_F16)
+            potts_init_kernel<K, StorageF16><<<grid_for(g), 256, 0, s->ctx->stream>>>(g, u, s->V, (_Float16 *)s->vbar[0], lab);
+        else
+            potts_init_kernel<K, StorageF32><<<grid_for(g), 256, 0, s->ctx->stream>>>(g, u, s->V, (float *)s->vbar[0], lab);
         SCT_HIP(s->ctx, hipGetLastError());
     }
     s->cur = 0;
-    SCT_HIP(s->ctx, hipMemsetAsync(s->xi[0], 0, sizeof(float) * 3 * (K - 1) * s->wspan, s->ctx->stream));
+    SCT_HIP(s->ctx, hipMemsetAsync(s->xi[0], 0, storage_bytes(s->storage) * 3 * (K - 1) * s->wspan, s->ctx->stream));
     return SCT_OK;
 }
 
@@ -206,12 +229,13 @@ int launch_iterate(sct_potts_solver *s, int iters, float *dq_dev)
     const DykstraLimits lim = dykstra_limits();
     const uint32_t grid = grid_for(g);
     for (int i = 0; i < iters; ++i) {
-        potts_xi_kernel<K><<<grid, 256, 0, s->ctx->stream>>>(g, s->xi[0], s->vbar[0], s->tau2, lim);
+        float *xi = (float *)s->xi[0], *vbar = (float *)s->vbar[0];
+        potts_xi_kernel<K><<<grid, 256, 0, s->ctx->stream>>>(g, xi, vbar, s->tau2, lim);
         if (dq_dev)
-            potts_v_kernel<K, true><<<grid, 256, 0, s->ctx->stream>>>(g, s->xi[0], s->V, s->vbar[0], (const float *)s->g->data,
+            potts_v_kernel<K, true><<<grid, 256, 0, s->ctx->stream>>>(g, xi, s->V, vbar, (const float *)s->g->data,
                                                                      lab, s->lambda, s->tau1, dq_dev + i);
         else
-            potts_v_kernel<K, false><<<grid, 256, 0, s->ctx->stream>>>(g, s->xi[0], s->V, s->vbar[0], (const float *)s->g->data,
+            potts_v_kernel<K, false><<<grid, 256, 0, s->ctx->stream>>>(g, xi, s->V, vbar, (const float *)s->g->data,
                                                                       lab, s->lambda, s->tau1, nullptr);
     }
     SCT_HIP(s->ctx, hipGetLastError());
@@ -244,15 +268,19 @@ int do_init(sct_potts_solver *s) { SCT_DISPATCH_K(s->K, launch_init, s); }
 int do_iterate(sct_potts_solver *s, int iters, float *dq) { SCT_DISPATCH_K(s->K, launch_iterate, s, iters, dq); }
 int do_labels(sct_potts_solver *s, float *res, uint32_t r0, uint32_t r1) { SCT_DISPATCH_K(s->K, launch_labels, s, res, r0, r1); }
 
-// Words of solver state per voxel of a slab: Xi 3(K-1), V (K-1), V-bar (K-1),
-// with Xi and V-bar doubled for the fused kernel's ping-pong.
-uint64_t words_per_voxel(int K, int kernel)
+// Bytes of solver state per voxel of a slab: Xi 3(K-1) and V-bar (K-1) in the
+// storage type (doubled for the fused kernel's ping-pong), V (K-1) in FP32.
+uint64_t bytes_per_voxel(int K, int kernel, int storage)
 {
-    return uint64_t(kernel == SCT_KERNEL_FUSED ? 9 : 5) * (K - 1);
+    const uint64_t copies = kernel == SCT_KERNEL_FUSED ? 2 : 1;
+    return (copies * 4 * storage_bytes(storage) + 4) * uint64_t(K - 1);
 }
 
 int pick_kernel(const sct_potts_opts &o)
 {
+    // Reduced-precision storage needs the fused kernel: its V step consumes
+    // Xi+ before Xi+ is rounded for storage.
+    if (o.storage != SCT_STORAGE_F32) return SCT_KERNEL_FUSED;
     return o.kernel == SCT_KERNEL_TWOPASS ? SCT_KERNEL_TWOPASS : SCT_KERNEL_FUSED;
 }
 
@@ -273,23 +301,26 @@ int alloc_buffers(sct_potts_solver *s, uint32_t slab_len)
     const Geometry &geo = s->g->geo;
     s->wstride = sct_get_stride(slab_len);
     s->wspan = uint64_t(geo.lines) * s->wstride;
-    const size_t comp = sizeof(float) * std::max<uint64_t>(1, s->wspan);
+    const size_t comp = std::max<uint64_t>(1, s->wspan);
+    const size_t sb = storage_bytes(s->storage);
     const int nb = s->kernel == SCT_KERNEL_FUSED ? 2 : 1;
     int rc;
     for (int b = 0; b < nb; ++b) {
-        if ((rc = sct_alloc(s->ctx, (void **)&s->xi[b], comp * 3 * (s->K - 1)))) return rc;
-        if ((rc = sct_alloc(s->ctx, (void **)&s->vbar[b], comp * (s->K - 1)))) return rc;
+        if ((rc = sct_alloc(s->ctx, &s->xi[b], sb * comp * 3 * (s->K - 1)))) return rc;
+        if ((rc = sct_alloc(s->ctx, &s->vbar[b], sb * comp * (s->K - 1)))) return rc;
     }
-    if ((rc = sct_alloc(s->ctx, (void **)&s->V, comp * (s->K - 1)))) return rc;
+    if ((rc = sct_alloc(s->ctx, (void **)&s->V, sizeof(float) * comp * (s->K - 1)))) return rc;
     if (s->kernel == SCT_KERNEL_FUSED && (rc = fused_prepare(s))) return rc;
     return SCT_OK;
 }
 
-int validate(sct_ctx *ctx, const sct_field *g, int K, const float *c, int iters)
+int validate(sct_ctx *ctx, const sct_field *g, int K, const float *c, int iters, const sct_potts_opts *opts)
 {
     if (!ctx || !g || !c) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
     if (g->dtype != SCT_F32) return sct_fail(ctx, SCT_ERR_INVALID, "Potts needs a float field");
     if (K < 2 || K > 6) return sct_fail(ctx, SCT_ERR_INVALID, "K must be in [2, 6]");
+    if (opts && (opts->storage < SCT_STORAGE_F32 || opts->storage > SCT_STORAGE_F16))
+        return sct_fail(ctx, SCT_ERR_INVALID, "unknown storage type");
     if (iters < 0) return sct_fail(ctx, SCT_ERR_INVALID, "iters must be >= 0");
     return SCT_OK;
 }
@@ -348,7 +379,7 @@ int sct::potts_plan(sct_ctx *ctx, const sct_field *g, int K, int kernel, int ite
             int rc = sct_device_memory(ctx, &avail, &total);
             if (rc) return rc;
         }
-        const uint64_t per_slice = sizeof(float) * uint64_t(std::max<uint32_t>(1, geo.lines)) * words_per_voxel(K, kernel);
+        const uint64_t per_slice = sizeof(float) * uint64_t(std::max<uint32_t>(1, geo.lines)) * bytes_per_voxel(K, kernel, o ? o->storage : 0) / sizeof(float);
         const uint64_t w = avail / per_slice;
         slab_w = int(sct_prev_stride(uint32_t(std::min<uint64_t>(w, 0x7fffffffu))));
     }
@@ -373,7 +404,7 @@ extern "C" int sct_potts_solver_create(sct_ctx *ctx, const sct_field *g, int K, 
 {
     if (!out) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
     *out = nullptr;
-    int rc = validate(ctx, g, K, c, 0);
+    int rc = validate(ctx, g, K, c, 0, opts);
     if (rc) return rc;
     auto *s = new sct_potts_solver;
     s->ctx = ctx;
@@ -386,6 +417,7 @@ extern "C" int sct_potts_solver_create(sct_ctx *ctx, const sct_field *g, int K, 
     if (opts) std::memcpy(&s->opts, opts, std::min<size_t>(sizeof(s->opts), opts->size ? opts->size : sizeof(s->opts)));
     else sct_potts_opts_init(&s->opts);
     s->kernel = pick_kernel(s->opts);
+    s->storage = s->opts.storage;
     s->s0 = 0;
     s->s1 = g->geo.n_images;
     rc = alloc_buffers(s, s->s1);
@@ -438,22 +470,37 @@ extern "C" int sct_potts_solver_state(sct_potts_solver *s, float *V, float *xi)
     const Geometry &geo = s->g->geo;
     const size_t span = geo.span();
     SCT_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
-    std::vector<float> tmp(s->wspan);
-    auto export_comp = [&](const float *dev, float *host) -> int {
-        SCT_HIP(s->ctx, hipMemcpy(tmp.data(), dev, sizeof(float) * s->wspan, hipMemcpyDeviceToHost));
+    // Copy one component (elem bytes per value, storage type st) to the host
+    // layout, widening bfloat16 / half to float.
+    std::vector<uint8_t> tmp(s->wspan * sizeof(float));
+    auto export_comp = [&](const void *dev, size_t elem, int st, float *host) -> int {
+        SCT_HIP(s->ctx, hipMemcpy(tmp.data(), dev, elem * s->wspan, hipMemcpyDeviceToHost));
         std::fill(host, host + span, 0.0f);
         const uint32_t len = s->s1 - s->s0;
-        for (uint32_t L = 0; L < geo.lines; ++L)
-            std::memcpy(host + size_t(L) * geo.nstride + s->s0, tmp.data() + size_t(L) * s->wstride, sizeof(float) * len);
+        for (uint32_t L = 0; L < geo.lines; ++L) {
+            float *dst = host + size_t(L) * geo.nstride + s->s0;
+            const uint8_t *src = tmp.data() + size_t(L) * s->wstride * elem;
+            for (uint32_t k = 0; k < len; ++k) {
+                if (st == SCT_STORAGE_F32) {
+                    std::memcpy(dst + k, src + 4 * k, 4);
+                } else {
+                    uint16_t h;
+                    std::memcpy(&h, src + 2 * k, 2);
+                    dst[k] = st == SCT_STORAGE_BF16 ? bf16_to_float(h) : half_to_float(h);
+                }
+            }
+        }
         return SCT_OK;
     };
     int rc;
+    const size_t sb = storage_bytes(s->storage);
     if (V)
         for (int i = 0; i < s->K - 1; ++i)
-            if ((rc = export_comp(s->V + i * s->wspan, V + i * span))) return rc;
+            if ((rc = export_comp(s->V + i * s->wspan, 4, SCT_STORAGE_F32, V + i * span))) return rc;
     if (xi)
         for (int j = 0; j < 3 * (s->K - 1); ++j)
-            if ((rc = export_comp(s->xi[s->cur] + j * s->wspan, xi + j * span))) return rc;
+            if ((rc = export_comp((const uint8_t *)s->xi[s->cur] + j * s->wspan * sb, sb, s->storage, xi + j * span)))
+                return rc;
     return SCT_OK;
 }
 
@@ -481,7 +528,7 @@ extern "C" int sct_potts_min_partition(sct_ctx *ctx, const sct_field *g, int K, 
 {
     if (!out) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
     *out = nullptr;
-    int rc = validate(ctx, g, K, c, iters);
+    int rc = validate(ctx, g, K, c, iters, opts);
     if (rc) return rc;
     sct_potts_opts o;
     sct_potts_opts_init(&o);
@@ -505,6 +552,7 @@ extern "C" int sct_potts_min_partition(sct_ctx *ctx, const sct_field *g, int K, 
     s.tau2 = tau2;
     s.opts = o;
     s.kernel = pick_kernel(o);
+    s.storage = o.storage;
     const int n = int(g->geo.n_images);
     rc = alloc_buffers(&s, uint32_t(plan.slab_w));
     float *d_dq = nullptr;

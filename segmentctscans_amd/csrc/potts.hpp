@@ -44,8 +44,10 @@ struct sct_potts_solver {
     uint64_t wspan = 0;
     // Xi and V-bar are double-buffered for the fused kernel (read [cur], write
     // [cur ^ 1]); the two-pass kernels update [0] in place.
-    float *xi[2] = {nullptr, nullptr};
-    float *vbar[2] = {nullptr, nullptr};
+    // Element type of Xi and V-bar follows opts.storage (float, bfloat16 or half).
+    void *xi[2] = {nullptr, nullptr};
+    void *vbar[2] = {nullptr, nullptr};
+    int storage = SCT_STORAGE_F32;
     float *V = nullptr;
     int cur = 0;
     float *d_dq = nullptr;

@@ -9,18 +9,23 @@
 //       Xi+ is computed here (halo points are recomputed, not exchanged);
 //   T = the tile itself: V+ and V-bar+ are computed and stored here.
 //
-// Wave w owns P rows 4w..4w+3; lane 0 is the n-halo, lanes 1..63 are tile
-// voxels, so the halo costs no extra pass.  Xi+ of the current column goes to
-// LDS for the -n / -row neighbours and stays in registers as the -col
-// neighbour of the next column.  V-bar of columns c and c+1 sit in two LDS
-// boxes that alternate.  Xi and V-bar are double-buffered in HBM (read the
-// previous iteration, write the next), so no tile ever sees a neighbour's
-// half-updated state; V and u are only touched at a tile's own voxels.
-//
-// Per voxel and iteration the kernel moves the algorithmic 4(10(K-1)+1) bytes
-// plus halo re-reads, which mostly hit L2 (neighbour tiles of one band are
-// placed on one XCD).  Arithmetic per voxel is exactly that of the two-pass
-// kernels (potts_math.hpp), so results are bitwise identical.
+// Lanes run along n (lane 0 is the n-halo, lanes 1..63 tile voxels); each of
+// the 8 waves owns two P rows.  Per column:
+//   A  every owner lane forms the Dykstra input q = Xi + tau2 grad V-bar of its
+//      points.  Most points are already feasible, so Dykstra stops after its
+//      first sweep; that case is finished in place.  The others are appended
+//      to a work list in LDS.
+//   B  all 512 lanes drain the work list, one Dykstra sweep per loop trip and a
+//      new item as soon as one converges, so a wave's time follows the total
+//      number of sweeps rather than the slowest of its 64 points.
+//   C  owner lanes store Xi+ and compute V+, V-bar+ of their tile points, with
+//      Xi+ of the -n / -row neighbours from LDS and of the -col neighbour from
+//      their registers (kept from the previous column).
+// The next column's HBM loads are issued before B and consumed after it.
+// Xi and V-bar are double-buffered in HBM (read the previous iteration, write
+// the next), so no tile ever sees a neighbour's half-updated state; V and u
+// are only touched at a tile's own voxels.  Arithmetic per voxel is that of
+// the two-pass kernels (potts_math.hpp), so results are bitwise identical.
 #include "potts.hpp"
 
 #include <algorithm>
@@ -36,20 +41,21 @@ constexpr int kTRP = 16;             // P rows (row 0 = halo)
 constexpr int kTR = kTRP - 1;        // tile rows
 constexpr int kBoxR = kTRP + 1;      // V-bar box rows (P rows + the +row neighbour)
 constexpr int kBoxN = kTNP + 1;      // V-bar box width (P + the +n neighbour)
-constexpr int kWaves = 4;
+constexpr int kWaves = 8;
+constexpr int kThreads = kWaves * 64;
 constexpr int kRowsPerWave = kTRP / kWaves;
 constexpr int kXcds = 8;             // MI355X: workgroups are dealt round-robin to 8 XCDs
 
-template <int K>
+template <int K, typename St>
 struct FusedArgs {
     const ColInfo *cols;
     const BandInfo *bands;
     uint32_t nbands, ntn, len, wstride, in_stride, s0, col_begin, col_end;
     uint64_t wspan;
-    const float *xi_in;
-    float *xi_out;
-    const float *vb_in;
-    float *vb_out;
+    const typename St::T *xi_in;
+    typename St::T *xi_out;
+    const typename St::T *vb_in;
+    typename St::T *vb_out;
     float *V;
     const float *u;
     float c[K];
@@ -59,24 +65,28 @@ struct FusedArgs {
 };
 
 template <int K>
-constexpr size_t fused_lds_bytes()
-{
-    return sizeof(float) * (size_t(2) * (K - 1) * kBoxR * kBoxN + size_t(kTRP) * 3 * (K - 1) * kTNP);
-}
+struct FusedLds {
+    float box[2][K - 1][kBoxR][kBoxN];   // V-bar of columns c and c + 1 (slot = col & 1)
+    float xl[kTRP][3 * (K - 1)][kTNP];    // Dykstra input, then Xi+, of the P points
+    float qk[kTRP][3][kTNP];              // implicit K-th component of queued inputs
+    uint16_t wl[kTRP * kTNP];             // work list: point = row * 64 + lane
+    int wl_n, wl_next;
+    float red[kWaves];
+};
 
 __device__ __forceinline__ bool row_live(const ColInfo &ci, int r)
 {
     return r >= int(ci.rb) && r < int(ci.re);
 }
 
-template <int K, bool TRACE>
-__global__ __launch_bounds__(256) void potts_fused_kernel(FusedArgs<K> a)
+template <int K, typename St, bool TRACE>
+__global__ __launch_bounds__(kThreads, (K <= 3 ? 4 : 2)) void potts_fused_kernel(FusedArgs<K, St> a)
 {
     constexpr int KM = K - 1;
-    constexpr int NX = 3 * KM;
-    extern __shared__ float smem[];
-    float(*box)[KM][kBoxR][kBoxN] = reinterpret_cast<float(*)[KM][kBoxR][kBoxN]>(smem);
-    float(*xl)[NX][kTNP] = reinterpret_cast<float(*)[NX][kTNP]>(smem + 2 * KM * kBoxR * kBoxN);
+    constexpr int NP = K * (K - 1) / 2;
+    constexpr int BOXP = (KM * kBoxR + kWaves - 1) / kWaves;   // box rows loaded per wave
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    FusedLds<K> &L = *reinterpret_cast<FusedLds<K> *>(smem_raw);
 
     // Workgroup -> (band, n-tile): all n-tiles of a band on one XCD, so the
     // n-halo reads of one tile hit the L2 lines its neighbours just fetched.
@@ -91,9 +101,73 @@ __global__ __launch_bounds__(256) void potts_fused_kernel(FusedArgs<K> a)
     const int n = n0 - 1 + lane;
     const bool n_ok = n >= 0 && n < len;
     const bool t_lane = lane >= 1;
+    const int n2 = n0 - 1 + kTNP;        // the box's extra +n column (lane 0 loads it)
     const ColInfo dead{0, 0, 0, 0};
 
-    float cur[kRowsPerWave][KM][3];
+    // Registers holding one column's HBM data (filled one column ahead).
+    struct Col {
+        float xin[kRowsPerWave][KM][3];
+        float vv[kRowsPerWave][KM];
+        float uu[kRowsPerWave];
+        float bx[BOXP], bx2[BOXP];       // V-bar of the next column for the box
+    };
+
+    auto fetch = [&](int c, Col &f) {
+        const ColInfo cc = a.cols[c];
+        const ColInfo cn = (c + 1 < int(a.col_end)) ? a.cols[c + 1] : dead;
+#pragma unroll
+        for (int p = 0; p < kRowsPerWave; ++p) {
+            const int pr = w * kRowsPerWave + p;
+            const int r = r0 - 1 + pr;
+            f.uu[p] = 0.0f;
+#pragma unroll
+            for (int i = 0; i < KM; ++i) {
+                f.vv[p][i] = 0.0f;
+                f.xin[p][i][0] = f.xin[p][i][1] = f.xin[p][i][2] = 0.0f;
+            }
+            if (n_ok && row_live(cc, r)) {
+                const uint64_t line = uint64_t(cc.line0 + uint32_t(r - int(cc.rb)));
+                const uint64_t off = line * a.wstride + uint64_t(n);
+#pragma unroll
+                for (int i = 0; i < KM; ++i)
+#pragma unroll
+                    for (int ax = 0; ax < 3; ++ax) f.xin[p][i][ax] = St::ld(a.xi_in[(3 * i + ax) * a.wspan + off]);
+                if (pr >= 1 && t_lane) {
+#pragma unroll
+                    for (int i = 0; i < KM; ++i) f.vv[p][i] = a.V[i * a.wspan + off];
+                    f.uu[p] = a.u[line * a.in_stride + a.s0 + uint32_t(n)];
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < BOXP; ++t) {
+            const int pair = w + t * kWaves;
+            f.bx[t] = f.bx2[t] = 0.0f;
+            if (pair < KM * kBoxR) {
+                const int i = pair / kBoxR, q = pair % kBoxR;
+                const int r = r0 - 1 + q;
+                if (row_live(cn, r)) {
+                    const typename St::T *src =
+                        a.vb_in + i * a.wspan + uint64_t(cn.line0 + uint32_t(r - int(cn.rb))) * a.wstride;
+                    if (n_ok) f.bx[t] = St::ld(src[n]);
+                    if (lane == 0 && n2 < len) f.bx2[t] = St::ld(src[n2]);
+                }
+            }
+        }
+    };
+
+    auto commit_box = [&](const Col &f, int slot) {
+#pragma unroll
+        for (int t = 0; t < BOXP; ++t) {
+            const int pair = w + t * kWaves;
+            if (pair < KM * kBoxR) {
+                const int i = pair / kBoxR, q = pair % kBoxR;
+                L.box[slot][i][q][lane] = f.bx[t];
+                if (lane == 0) L.box[slot][i][q][kTNP] = f.bx2[t];
+            }
+        }
+    };
+
     float prev[kRowsPerWave][KM][3];
 #pragma unroll
     for (int p = 0; p < kRowsPerWave; ++p)
@@ -101,150 +175,195 @@ __global__ __launch_bounds__(256) void potts_fused_kernel(FusedArgs<K> a)
         for (int i = 0; i < KM; ++i) prev[p][i][0] = prev[p][i][1] = prev[p][i][2] = 0.0f;
     float dqmax = 0.0f;
 
-    auto load_box = [&](int col, int slot) {
-        const ColInfo ci = (col < int(a.col_end)) ? a.cols[col] : dead;
-        for (int pair = w; pair < KM * kBoxR; pair += kWaves) {
-            const int i = pair / kBoxR, q = pair % kBoxR;
-            const int r = r0 - 1 + q;
-            float v0 = 0.0f, v1 = 0.0f;
-            if (row_live(ci, r)) {
-                const float *src = a.vb_in + i * a.wspan + uint64_t(ci.line0 + uint32_t(r - int(ci.rb))) * a.wstride;
-                if (n_ok) v0 = src[n];
-                const int n2 = n0 - 1 + kTNP;
-                if (lane == 0 && n2 < len) v1 = src[n2];
+    // Prologue: V-bar of the first column into its box, HBM data of the first column.
+    Col cur, nxt;
+    fetch(int(band.cs) - 1 >= int(a.col_begin) ? int(band.cs) - 1 : int(band.cs), nxt);
+    if (int(band.cs) - 1 >= int(a.col_begin)) {
+        commit_box(nxt, int(band.cs) & 1);
+    } else {
+        // no previous column to fetch from: load the first column's V-bar box directly
+        const ColInfo ci = a.cols[band.cs];
+#pragma unroll
+        for (int t = 0; t < BOXP; ++t) {
+            const int pair = w + t * kWaves;
+            if (pair < KM * kBoxR) {
+                const int i = pair / kBoxR, q = pair % kBoxR;
+                const int r = r0 - 1 + q;
+                float v0 = 0.0f, v1 = 0.0f;
+                if (row_live(ci, r)) {
+                    const typename St::T *src =
+                        a.vb_in + i * a.wspan + uint64_t(ci.line0 + uint32_t(r - int(ci.rb))) * a.wstride;
+                    if (n_ok) v0 = St::ld(src[n]);
+                    if (lane == 0 && n2 < len) v1 = St::ld(src[n2]);
+                }
+                L.box[int(band.cs) & 1][i][q][lane] = v0;
+                if (lane == 0) L.box[int(band.cs) & 1][i][q][kTNP] = v1;
             }
-            box[slot][i][q][lane] = v0;
-            if (lane == 0) box[slot][i][q][kTNP] = v1;
         }
-    };
+    }
+    fetch(int(band.cs), cur);
 
     for (int c = int(band.cs); c < int(band.ce); ++c) {
         const ColInfo cc = a.cols[c];
         const ColInfo cn = (c + 1 < int(a.col_end)) ? a.cols[c + 1] : dead;
         const ColInfo cp = (c > int(a.col_begin)) ? a.cols[c - 1] : dead;
-        if (c == int(band.cs)) load_box(c, c & 1);
-        load_box(c + 1, (c + 1) & 1);
+        commit_box(cur, (c + 1) & 1);
+        if (threadIdx.x == 0) {
+            L.wl_n = 0;
+            L.wl_next = 0;
+        }
+        __syncthreads();   // S1: boxes of c and c+1 in LDS; counters reset
+        if (c + 1 < int(band.ce)) fetch(c + 1, nxt);
 
+        // ---- A: Dykstra inputs; finish the points feasible after one sweep ----
+        const int sc = c & 1, sn = (c + 1) & 1;
         bool pval[kRowsPerWave];
-        float vv[kRowsPerWave][KM], uu[kRowsPerWave];
 #pragma unroll
         for (int p = 0; p < kRowsPerWave; ++p) {
             const int pr = w * kRowsPerWave + p;
             const int r = r0 - 1 + pr;
             pval[p] = n_ok && row_live(cc, r);
-            uu[p] = 0.0f;
-#pragma unroll
-            for (int i = 0; i < KM; ++i) {
-                vv[p][i] = 0.0f;
-                cur[p][i][0] = cur[p][i][1] = cur[p][i][2] = 0.0f;
-            }
-            if (pval[p]) {
-                const uint64_t line = uint64_t(cc.line0 + uint32_t(r - int(cc.rb)));
-                const uint64_t off = line * a.wstride + uint64_t(n);
-#pragma unroll
-                for (int i = 0; i < KM; ++i)
-#pragma unroll
-                    for (int ax = 0; ax < 3; ++ax) cur[p][i][ax] = a.xi_in[(3 * i + ax) * a.wspan + off];
-                if (pr >= 1 && t_lane) {
-#pragma unroll
-                    for (int i = 0; i < KM; ++i) vv[p][i] = a.V[i * a.wspan + off];
-                    uu[p] = a.u[line * a.in_stride + a.s0 + uint32_t(n)];
-                }
-            }
-        }
-        __syncthreads();   // boxes visible; the previous column's xl reads are done
-
-        // ---- Xi+ on P ----
-        const int sc = c & 1, sn = (c + 1) & 1;
-#pragma unroll
-        for (int p = 0; p < kRowsPerWave; ++p) {
-            const int pr = w * kRowsPerWave + p;
-            const int r = r0 - 1 + pr;
+            bool queued = false;
             if (pval[p]) {
                 const bool nx[3] = {n + 1 < len, r + 1 < int(cc.re), row_live(cn, r)};
                 float vb[KM], vbn[3][KM];
 #pragma unroll
                 for (int i = 0; i < KM; ++i) {
-                    vb[i] = box[sc][i][pr][lane];
-                    vbn[0][i] = box[sc][i][pr][lane + 1];
-                    vbn[1][i] = box[sc][i][pr + 1][lane];
-                    vbn[2][i] = box[sn][i][pr][lane];
+                    vb[i] = L.box[sc][i][pr][lane];
+                    vbn[0][i] = L.box[sc][i][pr][lane + 1];
+                    vbn[1][i] = L.box[sc][i][pr + 1][lane];
+                    vbn[2][i] = L.box[sn][i][pr][lane];
                 }
-                xi_update<K>(vb, vbn, nx, cur[p], a.tau2, a.lim);
-                if (pr >= 1 && t_lane) {
-                    const uint64_t off = uint64_t(cc.line0 + uint32_t(r - int(cc.rb))) * a.wstride + uint64_t(n);
+                float q[K][3];
+                xi_input<K>(vb, vbn, nx, cur.xin[p], a.tau2, q);
+                queued = !dykstra_first_sweep_done<K>(q, a.lim);
 #pragma unroll
-                    for (int i = 0; i < KM; ++i)
+                for (int i = 0; i < KM; ++i)
 #pragma unroll
-                        for (int ax = 0; ax < 3; ++ax) a.xi_out[(3 * i + ax) * a.wspan + off] = cur[p][i][ax];
+                    for (int ax = 0; ax < 3; ++ax) L.xl[pr][3 * i + ax][lane] = q[i][ax];
+                if (queued) {
+#pragma unroll
+                    for (int ax = 0; ax < 3; ++ax) L.qk[pr][ax][lane] = q[KM][ax];
                 }
             }
-#pragma unroll
-            for (int i = 0; i < KM; ++i)
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) xl[pr][3 * i + ax][lane] = cur[p][i][ax];
+            // wave-aggregated append to the work list
+            const uint64_t mask = __ballot(queued);
+            if (mask) {
+                int base = 0;
+                const int leader = __ffsll((unsigned long long)mask) - 1;
+                if (lane == leader) base = atomicAdd(&L.wl_n, __popcll(mask));
+                base = __shfl(base, leader, kWave);
+                if (queued) {
+                    const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+                    L.wl[pos] = uint16_t(pr * kTNP + lane);
+                }
+            }
         }
-        __syncthreads();   // Xi+ of column c visible
+        __syncthreads();   // S2: work list complete
 
-        // ---- V+ on T ----
+        // ---- B: drain the work list, one sweep per trip ----
+        {
+            const int nwl = L.wl_n;
+            float q[K][3], r[NP][3];
+            int sw = 0, pt = 0;
+            bool have = false, drained = nwl == 0;
+            while (true) {
+                if (!have && !drained) {
+                    const int t = atomicAdd(&L.wl_next, 1);
+                    if (t < nwl) {
+                        pt = L.wl[t];
+                        const int pr = pt / kTNP, ln = pt % kTNP;
+#pragma unroll
+                        for (int i = 0; i < KM; ++i)
+#pragma unroll
+                            for (int ax = 0; ax < 3; ++ax) q[i][ax] = L.xl[pr][3 * i + ax][ln];
+#pragma unroll
+                        for (int ax = 0; ax < 3; ++ax) q[KM][ax] = L.qk[pr][ax][ln];
+#pragma unroll
+                        for (int pp = 0; pp < NP; ++pp) r[pp][0] = r[pp][1] = r[pp][2] = 0.0f;
+                        sw = 0;
+                        have = true;
+                    } else {
+                        drained = true;
+                    }
+                }
+                if (!__any(have)) break;
+                if (have) {
+                    const float m2 = dykstra_sweep<K>(q, r, a.lim);
+                    ++sw;
+                    if (m2 < a.lim.stol || sw == 20) {
+                        const int pr = pt / kTNP, ln = pt % kTNP;
+#pragma unroll
+                        for (int i = 0; i < KM; ++i)
+#pragma unroll
+                            for (int ax = 0; ax < 3; ++ax) L.xl[pr][3 * i + ax][ln] = q[i][ax];
+                        have = false;
+                    }
+                }
+            }
+        }
+        __syncthreads();   // S3: Xi+ of every P point in LDS
+
+        // ---- C: store Xi+, V step on the tile points ----
 #pragma unroll
         for (int p = 0; p < kRowsPerWave; ++p) {
             const int pr = w * kRowsPerWave + p;
             const int r = r0 - 1 + pr;
+            float own[KM][3];
+#pragma unroll
+            for (int i = 0; i < KM; ++i)
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) own[i][ax] = pval[p] ? L.xl[pr][3 * i + ax][lane] : 0.0f;
             if (pval[p] && pr >= 1 && t_lane) {
+                const uint64_t off = uint64_t(cc.line0 + uint32_t(r - int(cc.rb))) * a.wstride + uint64_t(n);
+#pragma unroll
+                for (int i = 0; i < KM; ++i)
+#pragma unroll
+                    for (int ax = 0; ax < 3; ++ax) a.xi_out[(3 * i + ax) * a.wspan + off] = St::st(own[i][ax]);
                 const bool px[3] = {n > 0, r > int(cc.rb), row_live(cp, r)};
                 const bool nx[3] = {n + 1 < len, r + 1 < int(cc.re), row_live(cn, r)};
                 float xp[3][KM];
 #pragma unroll
                 for (int i = 0; i < KM; ++i) {
-#pragma unroll
-                    for (int ax = 0; ax < 3; ++ax) {
-                        // backward neighbour along axis ax, component ax of label i
-                        const float fn = xl[pr][3 * i + ax][lane - 1];
-                        const float fr = xl[pr - 1][3 * i + ax][lane];
-                        const float fc = prev[p][i][ax];
-                        if (ax == 0) xp[0][i] = fn;
-                        if (ax == 1) xp[1][i] = fr;
-                        if (ax == 2) xp[2][i] = fc;
-                    }
+                    xp[0][i] = L.xl[pr][3 * i + 0][lane - 1];
+                    xp[1][i] = L.xl[pr - 1][3 * i + 1][lane];
+                    xp[2][i] = prev[p][i][2];
                 }
                 float vbo[KM];
-                const float d = v_update<K>(cur[p], xp, px, nx, vv[p], vbo, uu[p], a.c, a.lambda, a.tau1);
+                const float d = v_update<K>(own, xp, px, nx, cur.vv[p], vbo, cur.uu[p], a.c, a.lambda, a.tau1);
                 if constexpr (TRACE) dqmax = fmaxf(dqmax, d);
-                const uint64_t off = uint64_t(cc.line0 + uint32_t(r - int(cc.rb))) * a.wstride + uint64_t(n);
 #pragma unroll
                 for (int i = 0; i < KM; ++i) {
-                    a.V[i * a.wspan + off] = vv[p][i];
-                    a.vb_out[i * a.wspan + off] = vbo[i];
+                    a.V[i * a.wspan + off] = cur.vv[p][i];
+                    a.vb_out[i * a.wspan + off] = St::st(vbo[i]);
                 }
             }
-        }
-#pragma unroll
-        for (int p = 0; p < kRowsPerWave; ++p)
 #pragma unroll
             for (int i = 0; i < KM; ++i)
 #pragma unroll
-                for (int ax = 0; ax < 3; ++ax) prev[p][i][ax] = cur[p][i][ax];
+                for (int ax = 0; ax < 3; ++ax) prev[p][i][ax] = own[i][ax];
+        }
+        cur = nxt;
     }
 
     if constexpr (TRACE) {
-        __shared__ float red[kWaves];
         dqmax = wave_max(dqmax);
-        if (lane == 0) red[w] = dqmax;
+        if (lane == 0) L.red[w] = dqmax;
         __syncthreads();
         if (threadIdx.x == 0) {
-            float m = red[0];
-            for (int i = 1; i < kWaves; ++i) m = fmaxf(m, red[i]);
+            float m = L.red[0];
+            for (int i = 1; i < kWaves; ++i) m = fmaxf(m, L.red[i]);
             atomic_max_nonneg(a.dq, m);
         }
     }
 }
 
-template <int K>
+template <int K, typename St>
 int launch_fused(sct_potts_solver *s, int iters, float *dq_dev)
 {
+    using T = typename St::T;
     const Geometry &geo = s->g->geo;
-    FusedArgs<K> a;
+    FusedArgs<K, St> a;
     a.cols = s->d_cols;
     a.bands = s->d_bands;
     a.nbands = s->n_bands;
@@ -264,28 +383,28 @@ int launch_fused(sct_potts_solver *s, int iters, float *dq_dev)
     a.lambda = s->lambda;
     a.lim = dykstra_limits();
     if (!a.nbands || !a.len) return SCT_OK;
-    const size_t lds = fused_lds_bytes<K>();
+    const size_t lds = sizeof(FusedLds<K>);
     static bool attr_set[2] = {false, false};
     if (lds > 65536) {
         for (int t = 0; t < 2; ++t) {
             if (attr_set[t]) continue;
-            const void *fn = t ? (const void *)potts_fused_kernel<K, true> : (const void *)potts_fused_kernel<K, false>;
+            const void *fn = t ? (const void *)potts_fused_kernel<K, St, true> : (const void *)potts_fused_kernel<K, St, false>;
             SCT_HIP(s->ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
             attr_set[t] = true;
         }
     }
     const uint32_t grid = ((a.nbands + kXcds - 1) / kXcds) * kXcds * a.ntn;
     for (int it = 0; it < iters; ++it) {
-        a.xi_in = s->xi[s->cur];
-        a.xi_out = s->xi[s->cur ^ 1];
-        a.vb_in = s->vbar[s->cur];
-        a.vb_out = s->vbar[s->cur ^ 1];
+        a.xi_in = (const T *)s->xi[s->cur];
+        a.xi_out = (T *)s->xi[s->cur ^ 1];
+        a.vb_in = (const T *)s->vbar[s->cur];
+        a.vb_out = (T *)s->vbar[s->cur ^ 1];
         if (dq_dev) {
             a.dq = dq_dev + it;
-            potts_fused_kernel<K, true><<<grid, kWaves * kWave, lds, s->ctx->stream>>>(a);
+            potts_fused_kernel<K, St, true><<<grid, kThreads, lds, s->ctx->stream>>>(a);
         } else {
             a.dq = nullptr;
-            potts_fused_kernel<K, false><<<grid, kWaves * kWave, lds, s->ctx->stream>>>(a);
+            potts_fused_kernel<K, St, false><<<grid, kThreads, lds, s->ctx->stream>>>(a);
         }
         s->cur ^= 1;
     }
@@ -339,14 +458,24 @@ void sct::fused_release(sct_potts_solver *s)
     s->n_bands = 0;
 }
 
-int sct::fused_iterate(sct_potts_solver *s, int iters, float *dq_dev)
+template <typename St>
+static int fused_iterate_k(sct_potts_solver *s, int iters, float *dq_dev)
 {
     switch (s->K) {
-    case 2: return launch_fused<2>(s, iters, dq_dev);
-    case 3: return launch_fused<3>(s, iters, dq_dev);
-    case 4: return launch_fused<4>(s, iters, dq_dev);
-    case 5: return launch_fused<5>(s, iters, dq_dev);
-    case 6: return launch_fused<6>(s, iters, dq_dev);
+    case 2: return launch_fused<2, St>(s, iters, dq_dev);
+    case 3: return launch_fused<3, St>(s, iters, dq_dev);
+    case 4: return launch_fused<4, St>(s, iters, dq_dev);
+    case 5: return launch_fused<5, St>(s, iters, dq_dev);
+    case 6: return launch_fused<6, St>(s, iters, dq_dev);
     default: return SCT_ERR_INVALID;
+    }
+}
+
+int sct::fused_iterate(sct_potts_solver *s, int iters, float *dq_dev)
+{
+    switch (s->storage) {
+    case SCT_STORAGE_BF16: return fused_iterate_k<StorageBF16>(s, iters, dq_dev);
+    case SCT_STORAGE_F16: return fused_iterate_k<StorageF16>(s, iters, dq_dev);
+    default: return fused_iterate_k<StorageF32>(s, iters, dq_dev);
     }
 }

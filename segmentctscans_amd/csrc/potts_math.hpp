@@ -71,9 +71,46 @@ __device__ __forceinline__ void project_simplex(float (&q)[K])
     }
 }
 
-// Dykstra's alternating projection onto {|q_i1 - q_i2| <= 1 for all pairs},
-// potts-min-partition.cpp:202-238: at most 20 sweeps, stop when the largest
-// correction norm of a sweep is below 1e-6.  Returns the sweeps performed.
+// One sweep of Dykstra's alternating projection onto {|q_i1 - q_i2| <= 1 for
+// all pairs}, potts-min-partition.cpp:204-234.  r holds the correction of
+// each pair.  Returns the largest squared correction norm of the sweep.
+template <int K>
+__device__ __forceinline__ float dykstra_sweep(float (&q)[K][3], float (&r)[K * (K - 1) / 2][3], DykstraLimits lim)
+{
+    float m2 = 0.0f;
+#pragma unroll
+    for (int i1 = 0; i1 < K - 1; ++i1) {
+#pragma unroll
+        for (int i2 = i1 + 1; i2 < K; ++i2) {
+            const int p = (2 * K - 1 - i1) * i1 / 2 + i2 - i1 - 1;
+            const float h0 = (q[i2][0] - q[i1][0]) + r[p][0];
+            const float h1 = (q[i2][1] - q[i1][1]) + r[p][1];
+            const float h2 = (q[i2][2] - q[i1][2]) + r[p][2];
+            const float ss = __fmaf_rn(h2, h2, __fmaf_rn(h1, h1, h0 * h0));
+            float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
+            if (ss > lim.t1) {
+                const float l = sqrtf(ss);
+                const float s = (l - 1.0f) / l;
+                e0 = s * h0;
+                e1 = s * h1;
+                e2 = s * h2;
+            }
+            const float d0 = 0.5f * (e0 - r[p][0]);
+            const float d1 = 0.5f * (e1 - r[p][1]);
+            const float d2 = 0.5f * (e2 - r[p][2]);
+            const float dn2 = __fmaf_rn(d2, d2, __fmaf_rn(d1, d1, d0 * d0));
+            m2 = (dn2 < m2) ? m2 : dn2;
+            q[i1][0] += d0; q[i1][1] += d1; q[i1][2] += d2;
+            q[i2][0] -= d0; q[i2][1] -= d1; q[i2][2] -= d2;
+            r[p][0] = e0; r[p][1] = e1; r[p][2] = e2;
+        }
+    }
+    return m2;
+}
+
+// Dykstra's projection, potts-min-partition.cpp:202-238: at most 20 sweeps,
+// stop when the largest correction norm of a sweep is below 1e-6.  Returns
+// the sweeps performed.
 template <int K>
 __device__ __forceinline__ int dykstra(float (&q)[K][3], DykstraLimits lim)
 {
@@ -84,47 +121,48 @@ __device__ __forceinline__ int dykstra(float (&q)[K][3], DykstraLimits lim)
     int sweep = 0;
     while (sweep < 20) {
         ++sweep;
-        float m2 = 0.0f;
-#pragma unroll
-        for (int i1 = 0; i1 < K - 1; ++i1) {
-#pragma unroll
-            for (int i2 = i1 + 1; i2 < K; ++i2) {
-                const int p = (2 * K - 1 - i1) * i1 / 2 + i2 - i1 - 1;
-                const float h0 = (q[i2][0] - q[i1][0]) + r[p][0];
-                const float h1 = (q[i2][1] - q[i1][1]) + r[p][1];
-                const float h2 = (q[i2][2] - q[i1][2]) + r[p][2];
-                const float ss = __fmaf_rn(h2, h2, __fmaf_rn(h1, h1, h0 * h0));
-                float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
-                if (ss > lim.t1) {
-                    const float l = sqrtf(ss);
-                    const float s = (l - 1.0f) / l;
-                    e0 = s * h0;
-                    e1 = s * h1;
-                    e2 = s * h2;
-                }
-                const float d0 = 0.5f * (e0 - r[p][0]);
-                const float d1 = 0.5f * (e1 - r[p][1]);
-                const float d2 = 0.5f * (e2 - r[p][2]);
-                const float dn2 = __fmaf_rn(d2, d2, __fmaf_rn(d1, d1, d0 * d0));
-                m2 = (dn2 < m2) ? m2 : dn2;
-                q[i1][0] += d0; q[i1][1] += d1; q[i1][2] += d2;
-                q[i2][0] -= d0; q[i2][1] -= d1; q[i2][2] -= d2;
-                r[p][0] = e0; r[p][1] = e1; r[p][2] = e2;
-            }
-        }
-        if (m2 < lim.stol) break;
+        if (dykstra_sweep<K>(q, r, lim) < lim.stol) break;
     }
     return sweep;
 }
 
-// Xi update of one voxel, potts-min-partition.cpp:135-243.
+// True when Dykstra stops after its first sweep: with zero corrections every
+// pair is already feasible, the sweep changes q only by adding +0 to the
+// first index of each pair (turning -0 into +0), and its correction norm is 0.
+// In that case q is updated accordingly.
+template <int K>
+__device__ __forceinline__ bool dykstra_first_sweep_done(float (&q)[K][3], DykstraLimits lim)
+{
+    bool ok = true;
+#pragma unroll
+    for (int i1 = 0; i1 < K - 1; ++i1) {
+#pragma unroll
+        for (int i2 = i1 + 1; i2 < K; ++i2) {
+            const float h0 = (q[i2][0] - q[i1][0]) + 0.0f;
+            const float h1 = (q[i2][1] - q[i1][1]) + 0.0f;
+            const float h2 = (q[i2][2] - q[i1][2]) + 0.0f;
+            ok = ok && !(__fmaf_rn(h2, h2, __fmaf_rn(h1, h1, h0 * h0)) > lim.t1);
+        }
+    }
+    if (ok) {
+#pragma unroll
+        for (int i = 0; i < K - 1; ++i) {
+            q[i][0] += 0.0f;
+            q[i][1] += 0.0f;
+            q[i][2] += 0.0f;
+        }
+    }
+    return ok;
+}
+
+// Input of the Xi projection for one voxel, potts-min-partition.cpp:135-200:
+// q = Xi + tau2 grad V-bar over all K labels (the K-th components implicit).
 //   vb[i]      V-bar at x (explicit components)
 //   vbn[a][i]  V-bar at x + e_a, used when nx[a]
-//   xi[i][a]   in: Xi at x, out: projected Xi
+//   xi[i][a]   Xi at x
 template <int K>
-__device__ __forceinline__ int xi_update(const float (&vb)[K - 1], const float (&vbn)[3][K - 1],
-                                         const bool (&nx)[3], float (&xi)[K - 1][3], float tau2,
-                                         DykstraLimits lim)
+__device__ __forceinline__ void xi_input(const float (&vb)[K - 1], const float (&vbn)[3][K - 1], const bool (&nx)[3],
+                                         const float (&xi)[K - 1][3], float tau2, float (&q)[K][3])
 {
     float vf[K];
     float vk = 1.0f;
@@ -134,7 +172,6 @@ __device__ __forceinline__ int xi_update(const float (&vb)[K - 1], const float (
         vk = vk - vb[i];
     }
     vf[K - 1] = vk;
-    float q[K][3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         float pn[K];
@@ -160,6 +197,16 @@ __device__ __forceinline__ int xi_update(const float (&vb)[K - 1], const float (
         }
         q[K - 1][a] = __fmaf_rn(pn[K - 1], tau2, xk);
     }
+}
+
+// Xi update of one voxel, potts-min-partition.cpp:135-243: xi <- Pi_K(q).
+template <int K>
+__device__ __forceinline__ int xi_update(const float (&vb)[K - 1], const float (&vbn)[3][K - 1],
+                                         const bool (&nx)[3], float (&xi)[K - 1][3], float tau2,
+                                         DykstraLimits lim)
+{
+    float q[K][3];
+    xi_input<K>(vb, vbn, nx, xi, tau2, q);
     const int sweeps = dykstra<K>(q, lim);
 #pragma unroll
     for (int i = 0; i < K - 1; ++i)
@@ -267,5 +314,35 @@ __device__ __forceinline__ int argmax_label(const float (&v)[K - 1])
     }
     return lab;
 }
+
+}  // namespace sct
+
+namespace sct {
+
+// Storage of Xi and V-bar.  Arithmetic is FP32 throughout; loads widen and
+// stores round to nearest even.
+struct StorageF32 {
+    using T = float;
+    static __device__ __forceinline__ float ld(T x) { return x; }
+    static __device__ __forceinline__ T st(float f) { return f; }
+};
+
+struct StorageBF16 {
+    using T = uint16_t;   // bfloat16 bit pattern
+    static __device__ __forceinline__ float ld(T x) { return __uint_as_float(uint32_t(x) << 16); }
+    static __device__ __forceinline__ T st(float f)
+    {
+        uint32_t x = __float_as_uint(f);
+        if ((x & 0x7fffffffu) > 0x7f800000u) return T((x | 0x00400000u) >> 16);   // quiet NaN
+        x += 0x7fffu + ((x >> 16) & 1u);
+        return T(x >> 16);
+    }
+};
+
+struct StorageF16 {
+    using T = _Float16;
+    static __device__ __forceinline__ float ld(T x) { return float(x); }
+    static __device__ __forceinline__ T st(float f) { return T(f); }   // v_cvt_f16_f32, round to nearest even
+};
 
 }  // namespace sct

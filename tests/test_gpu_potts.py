@@ -132,3 +132,43 @@ def test_invalid_arguments(ctx, c1_phantom):
         S.potts_min_partition(ctx, g, [0.5], 30.0, TAU, TAU, 5)
     with pytest.raises(S.SctError):
         S.potts_min_partition(ctx, g, [0.2, 0.5], 30.0, TAU, TAU, -1)
+
+
+@pytest.mark.parametrize("storage", ["bf16", "fp16"])
+@pytest.mark.parametrize("K", [2, 3, 5])
+def test_reduced_storage_bitwise_vs_mirror(ctx, storage, K):
+    """Xi and V-bar stored as bfloat16 / half, arithmetic FP32: bitwise equal to
+    the FP32 mirror with the same storage rounding."""
+    ph = make_phantom(70, 30, 2.0, 5.0, seed=40 + K)
+    c = np.linspace(0.2, 0.7, K).astype(np.float32)
+    g = S.upload(ctx, ph.field)
+    s = S.PottsSolver(ctx, g, c, 30.0, TAU, TAU, dq_trace=True, storage=storage)
+    s.iterate(15)
+    V, xi = s.state()
+    res = s.labels().to_host()
+    dq = s.dq()
+    lay = O.Layout.from_mapping(ph.field.mapping)
+    r = O.potts(lay, ph.field.data, c, 30.0, TAU, TAU, 15, precision="f32", want_state=True, storage=storage)
+    m = ph.field.mapping.valid_mask()
+    assert np.array_equal(_bits(V[:, m]), _bits(r["V"][:, m]))
+    assert np.array_equal(_bits(xi[:, m]), _bits(r["xi"][:, m]))
+    assert np.array_equal(res[m], r["res"][m])
+    assert np.array_equal(_bits(dq), _bits(r["dq"]))
+
+
+@pytest.mark.parametrize("storage", ["bf16", "fp16"])
+def test_reduced_storage_c1_agreement_vs_fp64(ctx, c1_phantom, storage):
+    ph = c1_phantom
+    c = ph.normalized_labels()
+    g = S.upload(ctx, ph.field)
+    s = S.PottsSolver(ctx, g, c, 30.0, TAU, TAU, storage=storage)
+    s.iterate(500)
+    V, _ = s.state()
+    res = s.labels().to_host()
+    lay = O.Layout.from_mapping(ph.field.mapping)
+    r64 = O.potts(lay, ph.field.data, c, 30.0, TAU, TAU, 500, precision="f64", want_state=True)
+    m = ph.field.mapping.valid_mask()
+    agree = (res[m] == r64["res"][m]).mean()
+    vinf = np.abs(V[:, m] - r64["V"][:, m]).max()
+    print(f"C1 {storage} storage: label agreement vs FP64 {agree:.6f}, ||V - V_fp64||_inf {vinf:.3e}")
+    assert agree >= 0.999
