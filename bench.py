@@ -23,12 +23,36 @@ import numpy as np
 
 T0 = time.time()
 
-# Algorithmic bytes per in-mask voxel-iteration: RW Xi 3(K-1), RW V (K-1),
-# RW V-bar (K-1), R u; BF16/FP16 storage halves Xi and V-bar.
-def algo_bytes(K: int, storage: str) -> int:
-    if storage == "fp32":
-        return 4 * (10 * (K - 1) + 1)
-    return 24 * (K - 1) + 4
+# Algorithmic HBM bytes per in-mask voxel of one launch of each kernel: every
+# array the kernel must read or write, once (neighbour re-reads excluded).
+# sb = bytes per stored Xi / V-bar element (4 FP32, 2 BF16/FP16); V and u FP32.
+#   Xi kernel  R V-bar (K-1), R+W Xi 3(K-1)                     7(K-1) sb
+#   V kernel   R Xi 3(K-1), R+W V (K-1), W V-bar (K-1), R u     4(K-1) sb + 8(K-1) + 4
+#   fused      R+W Xi 3(K-1), R+W V-bar (K-1), R+W V (K-1), R u  8(K-1) sb + 8(K-1) + 4
+def kernel_bytes(K: int, storage: str) -> dict:
+    sb = 4 if storage == "fp32" else 2
+    return {"potts_xi_kernel": 7 * (K - 1) * sb,
+            "potts_v_kernel": 4 * (K - 1) * sb + 8 * (K - 1) + 4,
+            "potts_fused_kernel": 8 * (K - 1) * sb + 8 * (K - 1) + 4}
+
+
+KERNEL_NAMES = {1: ["potts_fused_kernel"], 2: ["potts_xi_kernel", "potts_v_kernel"]}
+
+
+def committed_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of kernel on workload, from the committed rocprofv3
+    --pmc summary profiles/<kernel>_<workload>_traffic.md (a line
+    'traffic_bytes_per_launch: N'), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"{kernel}_{workload}_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1].strip()), os.path.relpath(path, os.path.dirname(path) + "/..")
+    except OSError:
+        pass
+    return None, None
 
 
 HBM_PEAK_GBS = 8000.0
@@ -60,6 +84,25 @@ def cpu_baseline(ph, c, tau, iters: int, size: int) -> dict:
                       f"FP32 host mirror of the reference update ({dt:.1f} s)"}
 
 
+def copy_calibration(mib: int) -> int:
+    """A known-size copy: FETCH_SIZE and WRITE_SIZE of its kernel should both read mib MiB."""
+    import ctypes as C
+    from segmentctscans_amd.hiptime import hip
+    h = hip()
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipFree.argtypes = [C.c_void_p]
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert h.hipMalloc(C.byref(a), n) == 0 and h.hipMalloc(C.byref(b), n) == 0
+    for _ in range(3):
+        assert h.hipMemcpy(b, a, n, 3) == 0   # hipMemcpyDeviceToDevice
+    h.hipFree(a)
+    h.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 3}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,10 +115,17 @@ def main() -> int:
     ap.add_argument("--radius", type=float, default=460.0)
     ap.add_argument("--rmax", type=float, default=40.0)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--cpu-iters", type=int, default=100)
     ap.add_argument("--cpu-size", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-storage comparison run")
+    ap.add_argument("--bf16-steps", type=int, default=30)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB mebibytes (hipMemcpy's copy "
+                         "kernel), to calibrate the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the solver has no multi-GPU path in this build "
@@ -101,25 +151,32 @@ def main() -> int:
     solver.iterate(args.warmup)
     ctx.synchronize()
 
-    tl = EventTimeline(ctx.stream, args.steps + 1)
+    tl = EventTimeline(ctx.stream, 2)
     ctx.synchronize()
     w0 = time.perf_counter()
     tl.mark()
-    for _ in range(args.steps):
-        solver.iterate(1)
-        tl.mark()
+    kms = solver.iterate_timed(args.steps)   # events between the kernel launches
+    tl.mark()
     ctx.synchronize()
     wall = time.perf_counter() - w0
-    durs = np.array(tl.durations())
+    total_ms = float(tl.durations()[0])
     tl.close()
-    total_ms = float(durs.sum())
     ms_step = total_ms / args.steps
     value = vox * args.steps / (total_ms / 1e3)
-    log(f"timed {args.steps} steps: {ms_step:.3f} ms/step (events), wall {wall:.3f} s; "
-        f"min {durs.min():.3f} max {durs.max():.3f} ms")
+    names = KERNEL_NAMES[kms.shape[1]]
+    kavg = [float(x) for x in kms.astype(np.float64).mean(axis=0)]
+    log(f"timed {args.steps} steps: {ms_step:.3f} ms/step (events), wall {wall:.3f} s; per kernel "
+        + ", ".join(f"{n} {a:.3f} ms" for n, a in zip(names, kavg)))
 
-    bpv = algo_bytes(K, args.storage)
-    achieved = bpv * vox / (ms_step / 1e3) / 1e9
+    kb = kernel_bytes(K, args.storage)
+    dom = int(np.argmax(kavg))
+    dom_name = names[dom]
+    achieved = kb[dom_name] * vox / (kavg[dom] / 1e3) / 1e9
+    step_bytes = sum(kb[n] for n in names)
+    step_gbs = step_bytes * vox / (ms_step / 1e3) / 1e9
+    workload = "potts_k3_fp32" if args.storage == "fp32" else f"potts_k3_{args.storage}_storage"
+    traffic, traffic_src = committed_traffic(dom_name, workload) if (
+        args.n, args.side, args.radius, args.rmax, args.seed) == (1426, 1024, 460.0, 40.0, 3) else (None, None)
     result = {
         "metric": "Potts voxel-iters/s + % HBM peak; BF16 vs FP32 speedup; adjoint/forward ratio",
         "value": value,
@@ -134,7 +191,7 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "potts_k3_fp32" if args.storage == "fp32" else f"potts_k3_{args.storage}_storage",
+            "workload": workload,
             "volume_n_rows_cols": [args.n, args.side, args.side],
             "cylinder_radius": args.radius,
             "in_mask_voxels": vox,
@@ -143,24 +200,44 @@ def main() -> int:
             "kernel": args.kernel,
             "phantom_seed": args.seed,
         },
-        "hbm_peak_frac": achieved / HBM_PEAK_GBS,
+        "kernel_ms": {n: float(a) for n, a in zip(names, kavg)},
+        "hbm_peak_frac": step_gbs / HBM_PEAK_GBS,
+        "step_bytes_per_voxel_iter": step_bytes,
         "roofline": {
             "bound": "hbm",
+            "kernel": dom_name,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "bytes_per_voxel_iter": bpv,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
+            "algorithmic_bytes_per_launch": kb[dom_name] * vox,
+            "bytes_per_voxel": kb[dom_name],
         },
     }
     solver.close()
+    if args.storage == "fp32" and not args.no_bf16:
+        sb = S.PottsSolver(ctx, g, c, lam, tau, tau, storage="bf16", kernel=args.kernel)
+        sb.iterate(2)
+        ctx.synchronize()
+        tb = EventTimeline(ctx.stream, 2)
+        tb.mark()
+        sb.iterate(args.bf16_steps)
+        tb.mark()
+        ctx.synchronize()
+        ms_bf16 = float(tb.durations()[0]) / args.bf16_steps
+        tb.close()
+        sb.close()
+        result["bf16_storage"] = {"ms_per_step": ms_bf16, "speedup_vs_fp32": ms_step / ms_bf16,
+                                  "steps": args.bf16_steps}
+        log(f"BF16 storage: {ms_bf16:.3f} ms/step, speedup {ms_step / ms_bf16:.3f}")
     g.close()
     if not args.no_cpu_baseline:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(ph, c, tau, args.cpu_iters, args.cpu_size)
         log(f"CPU baseline {result['cpu_baseline']['value']:.3e} voxel-iters/s")
-    print(json.dumps(result), flush=True)
+    print(json.dumps(result, default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)
     return 0
 
 

@@ -100,9 +100,9 @@ uint64_t sct_field_data_size(const sct_field *field); /* in-mask voxels         
 #define SCT_STORAGE_BF16 1   /* Xi and V-bar stored as bfloat16, arithmetic FP32 */
 #define SCT_STORAGE_F16  2   /* Xi and V-bar stored as IEEE half                 */
 
-#define SCT_KERNEL_AUTO    0
+#define SCT_KERNEL_AUTO    0  /* the two-pass kernels (fastest measured on MI355X) */
 #define SCT_KERNEL_TWOPASS 1  /* reference-shaped: Xi pass, then V pass       */
-#define SCT_KERNEL_FUSED   2  /* one pass per iteration, z-marching LDS tiles */
+#define SCT_KERNEL_FUSED   2  /* one pass per iteration, col-marching LDS tiles */
 
 typedef struct sct_potts_opts {
     uint32_t size;          /* sizeof(sct_potts_opts), for forward compatibility  */
@@ -134,6 +134,14 @@ int  sct_potts_solver_create(sct_ctx *ctx, const sct_field *g, int K, const floa
                              float lambda, float tau1, float tau2,
                              const sct_potts_opts *opts, sct_potts_solver **out);
 int  sct_potts_solver_iterate(sct_potts_solver *s, int iters);
+/* Kernel launches per iteration: 2 for the two-pass kernels (Xi, then V),
+ * 1 for the fused kernel. */
+int  sct_potts_solver_kernels(const sct_potts_solver *s);
+/* iterate, with HIP events on the library stream between the launches:
+ * kernel_ms[i * kernels + j] receives the duration of launch j of iteration i
+ * (host array of iters * sct_potts_solver_kernels(s)).  Synchronises.  No
+ * counterpart in the reference, which has no kernel timing. */
+int  sct_potts_solver_iterate_timed(sct_potts_solver *s, int iters, float *kernel_ms);
 int  sct_potts_solver_labels(sct_potts_solver *s, sct_field *res);
 /* Downloads V ((K-1) x span) and Xi (3(K-1) x span), component-major, in the
  * input layout; either pointer may be NULL.  Synchronises. */

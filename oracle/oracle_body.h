@@ -276,6 +276,12 @@ int FN(or_potts)(const or_layout *m, const float *u, int K, const float *c,
                 for (uint32_t row = row_begin(m, col); row < row_end(m, col); ++row)
                     for (uint32_t nn = S.s0; nn < S.s1; ++nn)
                         FN(potts_xi_voxel)(&S, K, nn, row, col, (REAL)tau2, xi, vbar);
+            /* Reduced-precision storage: Xi+ is rounded where it is stored, and
+             * the V step below reads the stored values. */
+            if (storage) {
+#pragma omp parallel for schedule(static)
+                for (size_t k = 0; k < (size_t)3 * (K - 1) * wspan; ++k) xi[k] = STORE_ROUND(xi[k], storage);
+            }
             REAL mx = 0;
 #pragma omp parallel for schedule(dynamic, 16) reduction(max : mx)
             for (uint32_t col = m->col_begin; col < m->col_end; ++col)
@@ -286,12 +292,6 @@ int FN(or_potts)(const or_layout *m, const float *u, int K, const float *c,
                         if (d > mx) mx = d;
                     }
             if (dq) dq[it] = mx;
-            /* Reduced-precision storage: the V step above used Xi+ as computed;
-             * the stored Xi (input of the next iteration) is rounded. */
-            if (storage) {
-#pragma omp parallel for schedule(static)
-                for (size_t k = 0; k < (size_t)3 * (K - 1) * wspan; ++k) xi[k] = STORE_ROUND(xi[k], storage);
-            }
         }
         /* update_kernel over the slab interior, :491-538 and :663-674 */
         uint32_t r0 = (S.s0 == 0) ? S.s0 : S.s0 + (uint32_t)halo;

@@ -78,8 +78,9 @@ int or_dykstra_f(int K, float *q, int max_sweeps, double tol);
 int or_potts_d(const or_layout *m, const float *u, int K, const float *c,
                float lambda, float tau1, float tau2, int iters, int slab_w,
                float *res, double *V_out, double *xi_out, double *dq, int storage, int nthreads);
-/* storage: 0 FP32; 1 Xi and V-bar stored as bfloat16; 2 as IEEE half (the
- * V step of an iteration uses that iteration's Xi+ before rounding). */
+/* storage: 0 FP32; 1 Xi and V-bar stored as bfloat16; 2 as IEEE half: both
+ * are rounded (to nearest even) where they are stored and every later read,
+ * the V step of the same iteration included, sees the stored values. */
 int or_potts_f(const or_layout *m, const float *u, int K, const float *c,
                float lambda, float tau1, float tau2, int iters, int slab_w,
                float *res, float *V_out, float *xi_out, float *dq, int storage, int nthreads);

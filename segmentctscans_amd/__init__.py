@@ -329,6 +329,16 @@ class PottsSolver:
         self.ctx.check(N.lib().sct_potts_solver_iterate(self.handle, n))
         self.iters += n
 
+    def iterate_timed(self, n: int) -> np.ndarray:
+        """iterate(n), returning the duration in ms of every kernel launch,
+        shape (n, kernels per iteration): (Xi, V) for the two-pass kernels,
+        (fused,) for the fused kernel.  HIP events on the library stream."""
+        nk = N.lib().sct_potts_solver_kernels(self.handle)
+        out = np.zeros((n, nk), np.float32)
+        self.ctx.check(N.lib().sct_potts_solver_iterate_timed(self.handle, n, out.ctypes.data_as(N.f32p)))
+        self.iters += n
+        return out
+
     def labels(self) -> DeviceField:
         res = empty_like(self.ctx, self.g.mapping)
         self.ctx.check(N.lib().sct_potts_solver_labels(self.handle, res.handle))

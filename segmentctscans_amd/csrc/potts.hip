@@ -79,67 +79,184 @@ __global__ __launch_bounds__(256) void potts_init_kernel(WorkGeom g, const float
     }
 }
 
-// grad_kernel, potts-min-partition.cpp:96-247: Xi <- Pi_K(Xi + tau2 grad V-bar), in place.
-template <int K>
-__global__ __launch_bounds__(256) void potts_xi_kernel(WorkGeom g, float *__restrict__ xi,
-                                                       const float *__restrict__ vbar, float tau2,
+// Per-component base pointers of the two-pass kernels; every component is
+// addressed as base[comp] + 32-bit element offset (line * wstride + n).
+template <int K, typename St>
+struct TwoPassArgs {
+    typename St::T *xi[3 * (K - 1)];
+    typename St::T *vbar[K - 1];
+    float *V[K - 1];
+    const float *u;
+};
+
+constexpr int kXiItems = 512;        // work-list capacity per workgroup
+constexpr int kXiChunksPerWave = 4;  // 64-voxel chunks per wave
+
+// grad_kernel, potts-min-partition.cpp:96-247: Xi <- Pi_K(Xi + tau2 grad V-bar),
+// in place (each voxel reads its own Xi and its neighbours' V-bar only).
+//   A  every lane forms the Dykstra input of its voxels; most are feasible and
+//      Dykstra stops after one sweep, the rest join a work list in LDS;
+//   B  the work list is drained one sweep per loop trip, each lane taking a new
+//      item as soon as its item converges, the result going back to LDS;
+//   C  every lane stores the Xi+ of its voxels in full waves.
+template <int K, typename St>
+__global__ __launch_bounds__(256) void potts_xi_kernel(WorkGeom g, TwoPassArgs<K, St> a, float tau2,
                                                        DykstraLimits lim)
 {
-    uint32_t line, n;
-    if (!wave_voxel(g, line, n)) return;
-    const LineNbr nb = g.nbr[line];
-    const uint64_t w = uint64_t(line) * g.wstride + n;
-    const bool nx[3] = {n + 1 < g.len, nb.nrow >= 0, nb.ncol >= 0};
-    const uint64_t wn[3] = {w + 1, uint64_t(nb.nrow) * g.wstride + n, uint64_t(nb.ncol) * g.wstride + n};
-    float vb[K - 1], vbn[3][K - 1], x[K - 1][3];
+    constexpr int KM = K - 1, NP = K * (K - 1) / 2;
+    __shared__ float qb[kXiItems][K][3];   // inputs, then results, of the queued voxels
+    __shared__ int nq, next;
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    if (threadIdx.x == 0) {
+        nq = 0;
+        next = 0;
+    }
+    __syncthreads();
+    const uint64_t total = uint64_t(g.lines) * g.chunks;
+    float res[kXiChunksPerWave][KM][3];
+    int pos[kXiChunksPerWave];
+    uint32_t wofs[kXiChunksPerWave];
+    bool valid[kXiChunksPerWave];
 #pragma unroll
-    for (int i = 0; i < K - 1; ++i) {
-        vb[i] = vbar[i * g.wspan + w];
+    for (int k = 0; k < kXiChunksPerWave; ++k) {
+        const uint64_t gw = (uint64_t(blockIdx.x) * (blockDim.x / kWave) + w) * kXiChunksPerWave + k;
+        const uint32_t line = uint32_t(gw / g.chunks);
+        const uint32_t n = uint32_t(gw % g.chunks) * kWave + lane;
+        pos[k] = -1;
+        valid[k] = gw < total && n < g.len;
+        wofs[k] = valid[k] ? line * g.wstride + n : 0;
+        bool queued = false;
+        if (valid[k]) {
+            const LineNbr nb = g.nbr[line];
+            const uint32_t w0 = wofs[k];
+            const bool nx[3] = {n + 1 < g.len, nb.nrow >= 0, nb.ncol >= 0};
+            const uint32_t wn[3] = {w0 + 1, uint32_t(nb.nrow) * g.wstride + n, uint32_t(nb.ncol) * g.wstride + n};
+            float vb[KM], vbn[3][KM], x[KM][3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            vbn[a][i] = nx[a] ? vbar[i * g.wspan + wn[a]] : 0.0f;
-            x[i][a] = xi[(3 * i + a) * g.wspan + w];
+            for (int i = 0; i < KM; ++i) {
+                vb[i] = St::ld(a.vbar[i][w0]);
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) {
+                    vbn[ax][i] = nx[ax] ? St::ld(a.vbar[i][wn[ax]]) : 0.0f;
+                    x[i][ax] = St::ld(a.xi[3 * i + ax][w0]);
+                }
+            }
+            float q[K][3];
+            xi_input<K>(vb, vbn, nx, x, tau2, q);
+            queued = !dykstra_first_sweep_done<K>(q, lim);
+            const uint64_t mask = __ballot(queued);
+            int p = 0;
+            if (mask) {
+                const int leader = __ffsll((unsigned long long)mask) - 1;
+                if (lane == leader) p = atomicAdd(&nq, __popcll(mask));
+                p = __shfl(p, leader, kWave) + __popcll(mask & ((1ull << lane) - 1ull));
+            }
+            if (queued && p < kXiItems) {
+#pragma unroll
+                for (int i = 0; i < K; ++i)
+#pragma unroll
+                    for (int ax = 0; ax < 3; ++ax) qb[p][i][ax] = q[i][ax];
+                pos[k] = p;
+            } else if (queued) {
+                dykstra<K>(q, lim);   // work list full: solve in place
+            }
+#pragma unroll
+            for (int i = 0; i < KM; ++i)
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) res[k][i][ax] = q[i][ax];
         }
     }
-    xi_update<K>(vb, vbn, nx, x, tau2, lim);
+    __syncthreads();   // work list complete
+    const int nwl = min(nq, kXiItems);
+    if (w * kWave < nwl) {
+        float q[K][3], r[NP][3];
+        int sw = 0, t = 0;
+        bool have = false, drained = false;
+        while (true) {
+            const bool need = !have && !drained;
+            const uint64_t mask = __ballot(need);
+            if (mask) {
+                const int leader = __ffsll((unsigned long long)mask) - 1;
+                int base = 0;
+                if (lane == leader) base = atomicAdd(&next, __popcll(mask));
+                base = __shfl(base, leader, kWave);
+                if (need) {
+                    t = base + __popcll(mask & ((1ull << lane) - 1ull));
+                    if (t < nwl) {
 #pragma unroll
-    for (int i = 0; i < K - 1; ++i)
+                        for (int i = 0; i < K; ++i)
 #pragma unroll
-        for (int a = 0; a < 3; ++a) xi[(3 * i + a) * g.wspan + w] = x[i][a];
+                            for (int ax = 0; ax < 3; ++ax) q[i][ax] = qb[t][i][ax];
+#pragma unroll
+                        for (int pp = 0; pp < NP; ++pp) r[pp][0] = r[pp][1] = r[pp][2] = 0.0f;
+                        sw = 0;
+                        have = true;
+                    } else {
+                        drained = true;
+                    }
+                }
+            }
+            if (!__any(have)) break;
+            if (have) {
+                const float m2 = dykstra_sweep<K>(q, r, lim);
+                ++sw;
+                if (m2 < lim.stol || sw == 20) {
+#pragma unroll
+                    for (int i = 0; i < KM; ++i)
+#pragma unroll
+                        for (int ax = 0; ax < 3; ++ax) qb[t][i][ax] = q[i][ax];
+                    have = false;
+                }
+            }
+        }
+    }
+    __syncthreads();   // results of the work list in LDS
+#pragma unroll
+    for (int k = 0; k < kXiChunksPerWave; ++k) {
+        if (!valid[k]) continue;
+        if (pos[k] >= 0) {
+#pragma unroll
+            for (int i = 0; i < KM; ++i)
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) res[k][i][ax] = qb[pos[k]][i][ax];
+        }
+#pragma unroll
+        for (int i = 0; i < KM; ++i)
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) a.xi[3 * i + ax][wofs[k]] = St::st(res[k][i][ax]);
+    }
 }
 
 // div_kernel, potts-min-partition.cpp:349-488: V <- Pi_S(V + tau1 (div Xi - G)),
 // V-bar <- 2 V+ - V.  With TRACE, the block maximum of sum_k dV_k^2 goes to *dq.
-template <int K, bool TRACE>
-__global__ __launch_bounds__(256) void potts_v_kernel(WorkGeom g, const float *__restrict__ xi,
-                                                      float *__restrict__ V, float *__restrict__ vbar,
-                                                      const float *__restrict__ u, Labels<K> lab,
+template <int K, typename St, bool TRACE>
+__global__ __launch_bounds__(256) void potts_v_kernel(WorkGeom g, TwoPassArgs<K, St> a, Labels<K> lab,
                                                       float lambda, float tau1, float *dq)
 {
     uint32_t line, n;
     float d = 0.0f;
     if (wave_voxel(g, line, n)) {
         const LineNbr nb = g.nbr[line];
-        const uint64_t w = uint64_t(line) * g.wstride + n;
+        const uint32_t w = line * g.wstride + n;
         const bool px[3] = {n > 0, nb.prow >= 0, nb.pcol >= 0};
         const bool nx[3] = {n + 1 < g.len, nb.nrow >= 0, nb.ncol >= 0};
-        const uint64_t wp[3] = {w - 1, uint64_t(nb.prow) * g.wstride + n, uint64_t(nb.pcol) * g.wstride + n};
+        const uint32_t wp[3] = {w - 1, uint32_t(nb.prow) * g.wstride + n, uint32_t(nb.pcol) * g.wstride + n};
         float xo[K - 1][3], xp[3][K - 1], v[K - 1], vb[K - 1];
 #pragma unroll
         for (int i = 0; i < K - 1; ++i) {
-            v[i] = V[i * g.wspan + w];
+            v[i] = a.V[i][w];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                xo[i][a] = xi[(3 * i + a) * g.wspan + w];
-                xp[a][i] = px[a] ? xi[(3 * i + a) * g.wspan + wp[a]] : 0.0f;
+            for (int ax = 0; ax < 3; ++ax) {
+                xo[i][ax] = St::ld(a.xi[3 * i + ax][w]);
+                xp[ax][i] = px[ax] ? St::ld(a.xi[3 * i + ax][wp[ax]]) : 0.0f;
             }
         }
-        const float uu = u[uint64_t(line) * g.in_stride + g.s0 + n];
+        const float uu = a.u[line * g.in_stride + g.s0 + n];
         d = v_update<K>(xo, xp, px, nx, v, vb, uu, lab.c, lambda, tau1);
 #pragma unroll
         for (int i = 0; i < K - 1; ++i) {
-            V[i * g.wspan + w] = v[i];
-            vbar[i * g.wspan + w] = vb[i];
+            a.V[i][w] = v[i];
+            a.vbar[i][w] = St::st(vb[i]);
         }
     }
     if constexpr (TRACE) {
@@ -218,28 +335,47 @@ int launch_init(sct_potts_solver *s)
     return SCT_OK;
 }
 
+template <int K, typename St>
+int twopass_iterate(sct_potts_solver *s, const WorkGeom &g, int iters, float *dq_dev)
+{
+    using T = typename St::T;
+    Labels<K> lab;
+    for (int i = 0; i < K; ++i) lab.c[i] = s->c[i];
+    const DykstraLimits lim = dykstra_limits();
+    TwoPassArgs<K, St> a;
+    for (int j = 0; j < 3 * (K - 1); ++j) a.xi[j] = (T *)s->xi[0] + j * s->wspan;
+    for (int i = 0; i < K - 1; ++i) {
+        a.vbar[i] = (T *)s->vbar[0] + i * s->wspan;
+        a.V[i] = s->V + i * s->wspan;
+    }
+    a.u = (const float *)s->g->data;
+    const uint64_t waves = uint64_t(g.lines) * g.chunks;
+    const uint32_t xi_grid = uint32_t((waves + 4 * kXiChunksPerWave - 1) / (4 * kXiChunksPerWave));
+    const uint32_t v_grid = grid_for(g);
+    for (int i = 0; i < iters; ++i) {
+        potts_mark(s);
+        potts_xi_kernel<K, St><<<xi_grid, 256, 0, s->ctx->stream>>>(g, a, s->tau2, lim);
+        potts_mark(s);
+        if (dq_dev)
+            potts_v_kernel<K, St, true><<<v_grid, 256, 0, s->ctx->stream>>>(g, a, lab, s->lambda, s->tau1, dq_dev + i);
+        else
+            potts_v_kernel<K, St, false><<<v_grid, 256, 0, s->ctx->stream>>>(g, a, lab, s->lambda, s->tau1, nullptr);
+    }
+    SCT_HIP(s->ctx, hipGetLastError());
+    return SCT_OK;
+}
+
 template <int K>
 int launch_iterate(sct_potts_solver *s, int iters, float *dq_dev)
 {
     const WorkGeom g = make_geom(s);
     if (!g.lines || !g.len) return SCT_OK;
     if (s->kernel == SCT_KERNEL_FUSED) return fused_iterate(s, iters, dq_dev);
-    Labels<K> lab;
-    for (int i = 0; i < K; ++i) lab.c[i] = s->c[i];
-    const DykstraLimits lim = dykstra_limits();
-    const uint32_t grid = grid_for(g);
-    for (int i = 0; i < iters; ++i) {
-        float *xi = (float *)s->xi[0], *vbar = (float *)s->vbar[0];
-        potts_xi_kernel<K><<<grid, 256, 0, s->ctx->stream>>>(g, xi, vbar, s->tau2, lim);
-        if (dq_dev)
-            potts_v_kernel<K, true><<<grid, 256, 0, s->ctx->stream>>>(g, xi, s->V, vbar, (const float *)s->g->data,
-                                                                     lab, s->lambda, s->tau1, dq_dev + i);
-        else
-            potts_v_kernel<K, false><<<grid, 256, 0, s->ctx->stream>>>(g, xi, s->V, vbar, (const float *)s->g->data,
-                                                                      lab, s->lambda, s->tau1, nullptr);
+    switch (s->storage) {
+    case SCT_STORAGE_BF16: return twopass_iterate<K, StorageBF16>(s, g, iters, dq_dev);
+    case SCT_STORAGE_F16: return twopass_iterate<K, StorageF16>(s, g, iters, dq_dev);
+    default: return twopass_iterate<K, StorageF32>(s, g, iters, dq_dev);
     }
-    SCT_HIP(s->ctx, hipGetLastError());
-    return SCT_OK;
 }
 
 template <int K>
@@ -278,10 +414,7 @@ uint64_t bytes_per_voxel(int K, int kernel, int storage)
 
 int pick_kernel(const sct_potts_opts &o)
 {
-    // Reduced-precision storage needs the fused kernel: its V step consumes
-    // Xi+ before Xi+ is rounded for storage.
-    if (o.storage != SCT_STORAGE_F32) return SCT_KERNEL_FUSED;
-    return o.kernel == SCT_KERNEL_TWOPASS ? SCT_KERNEL_TWOPASS : SCT_KERNEL_FUSED;
+    return o.kernel == SCT_KERNEL_FUSED ? SCT_KERNEL_FUSED : SCT_KERNEL_TWOPASS;
 }
 
 void release_buffers(sct_potts_solver *s)
@@ -431,6 +564,12 @@ extern "C" int sct_potts_solver_create(sct_ctx *ctx, const sct_field *g, int K, 
     return SCT_OK;
 }
 
+extern "C" int sct_potts_solver_kernels(const sct_potts_solver *s)
+{
+    if (!s) return SCT_ERR_INVALID;
+    return s->kernel == SCT_KERNEL_FUSED ? 1 : 2;
+}
+
 extern "C" int sct_potts_solver_iterate(sct_potts_solver *s, int iters)
 {
     if (!s || iters < 0) return SCT_ERR_INVALID;
@@ -453,6 +592,36 @@ extern "C" int sct_potts_solver_iterate(sct_potts_solver *s, int iters)
     }
     int rc = do_iterate(s, iters, dq);
     if (!rc) s->it += iters;
+    return rc;
+}
+
+extern "C" int sct_potts_solver_iterate_timed(sct_potts_solver *s, int iters, float *kernel_ms)
+{
+    if (!s || iters < 0 || (iters && !kernel_ms)) return SCT_ERR_INVALID;
+    const int nk = sct_potts_solver_kernels(s);
+    std::vector<hipEvent_t> ev(size_t(iters) * nk + 1, nullptr);
+    int rc = SCT_OK;
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+            rc = sct_fail(s->ctx, SCT_ERR_HIP, "hipEventCreate failed");
+            break;
+        }
+    if (!rc) {
+        s->marks = ev.data();
+        s->n_marks = 0;
+        rc = sct_potts_solver_iterate(s, iters);
+        s->marks = nullptr;
+    }
+    if (!rc && hipEventRecord(ev.back(), s->ctx->stream) != hipSuccess)
+        rc = sct_fail(s->ctx, SCT_ERR_HIP, "hipEventRecord failed");
+    if (!rc && hipEventSynchronize(ev.back()) != hipSuccess)
+        rc = sct_fail(s->ctx, SCT_ERR_HIP, "hipEventSynchronize failed");
+    for (size_t k = 0; !rc && k + 1 < ev.size(); ++k)
+        if (hipEventElapsedTime(&kernel_ms[k], ev[k], ev[k + 1]) != hipSuccess)
+            rc = sct_fail(s->ctx, SCT_ERR_HIP, "hipEventElapsedTime failed");
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
     return rc;
 }
 

@@ -53,6 +53,10 @@ struct sct_potts_solver {
     float *d_dq = nullptr;
     int dq_cap = 0;
     int it = 0;   // iterations done on this slab
+    // kernel timing: when marks is set, an event is recorded before every
+    // kernel launch of an iteration (sct_potts_solver_iterate_timed)
+    hipEvent_t *marks = nullptr;
+    int n_marks = 0;
     // fused-kernel tables (device)
     sct::ColInfo *d_cols = nullptr;
     sct::BandInfo *d_bands = nullptr;
@@ -60,6 +64,11 @@ struct sct_potts_solver {
 };
 
 namespace sct {
+
+inline void potts_mark(sct_potts_solver *s)
+{
+    if (s->marks) (void)hipEventRecord(s->marks[s->n_marks++], s->ctx->stream);
+}
 
 int potts_plan(sct_ctx *ctx, const sct_field *g, int K, int kernel, int iters, const sct_potts_opts *o,
                PottsPlan *plan);

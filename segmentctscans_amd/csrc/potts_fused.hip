@@ -15,17 +15,20 @@
 //      points.  Most points are already feasible, so Dykstra stops after its
 //      first sweep; that case is finished in place.  The others are appended
 //      to a work list in LDS.
-//   B  all 512 lanes drain the work list, one Dykstra sweep per loop trip and a
-//      new item as soon as one converges, so a wave's time follows the total
-//      number of sweeps rather than the slowest of its 64 points.
+//   B  the work list is drained one Dykstra sweep per loop trip, each lane
+//      taking a new item as soon as its item converges, so a wave's time
+//      follows the total number of sweeps rather than the slowest of its 64
+//      points; only as many waves join as there are items per 64 lanes.
 //   C  owner lanes store Xi+ and compute V+, V-bar+ of their tile points, with
 //      Xi+ of the -n / -row neighbours from LDS and of the -col neighbour from
 //      their registers (kept from the previous column).
 // The next column's HBM loads are issued before B and consumed after it.
 // Xi and V-bar are double-buffered in HBM (read the previous iteration, write
 // the next), so no tile ever sees a neighbour's half-updated state; V and u
-// are only touched at a tile's own voxels.  Arithmetic per voxel is that of
-// the two-pass kernels (potts_math.hpp), so results are bitwise identical.
+// are only touched at a tile's own voxels.  Xi+ is rounded to the storage type
+// as soon as it is formed, so the V step reads exactly what is stored.
+// Arithmetic per voxel is that of the two-pass kernels (potts_math.hpp), so
+// results are bitwise identical.
 #include "potts.hpp"
 
 #include <algorithm>
@@ -46,17 +49,17 @@ constexpr int kThreads = kWaves * 64;
 constexpr int kRowsPerWave = kTRP / kWaves;
 constexpr int kXcds = 8;             // MI355X: workgroups are dealt round-robin to 8 XCDs
 
+// Per-component base pointers: the kernel addresses every component as
+// base[comp] + 32-bit element offset (line * wstride + n < 2^32 is checked on
+// the host), so addresses stay in scalar registers.
 template <int K, typename St>
 struct FusedArgs {
-    const ColInfo *cols;
-    const BandInfo *bands;
     uint32_t nbands, ntn, len, wstride, in_stride, s0, col_begin, col_end;
-    uint64_t wspan;
-    const typename St::T *xi_in;
-    typename St::T *xi_out;
-    const typename St::T *vb_in;
-    typename St::T *vb_out;
-    float *V;
+    const typename St::T *xi_in[3 * (K - 1)];
+    typename St::T *xi_out[3 * (K - 1)];
+    const typename St::T *vb_in[K - 1];
+    typename St::T *vb_out[K - 1];
+    float *V[K - 1];
     const float *u;
     float c[K];
     float tau1, tau2, lambda;
@@ -64,12 +67,24 @@ struct FusedArgs {
     float *dq;
 };
 
+// Wave-uniform value in a scalar register.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ ColInfo load_col(const ColInfo *cols, int c)
+{
+    const ColInfo *p = cols + uniform(c);
+    return ColInfo{uint32_t(uniform(int(p->line0))), uint32_t(uniform(int(p->rb))), uint32_t(uniform(int(p->re))), 0};
+}
+
+constexpr int kMaxItems = 512;        // work-list capacity per column (overflow is solved in place)
+constexpr int kItemsPerLane = 1;      // waves join the work list while each lane gets this many items
+
 template <int K>
 struct FusedLds {
     float box[2][K - 1][kBoxR][kBoxN];   // V-bar of columns c and c + 1 (slot = col & 1)
-    float xl[kTRP][3 * (K - 1)][kTNP];    // Dykstra input, then Xi+, of the P points
-    float qk[kTRP][3][kTNP];              // implicit K-th component of queued inputs
-    uint16_t wl[kTRP * kTNP];             // work list: point = row * 64 + lane
+    float xl[kTRP][3 * (K - 1)][kTNP];    // Xi+ of the P points
+    float qb[kMaxItems][K][3];            // Dykstra inputs of the queued points
+    uint16_t wl[kMaxItems];               // queued point = row * 64 + lane
     int wl_n, wl_next;
     float red[kWaves];
 };
@@ -80,7 +95,9 @@ __device__ __forceinline__ bool row_live(const ColInfo &ci, int r)
 }
 
 template <int K, typename St, bool TRACE>
-__global__ __launch_bounds__(kThreads, (K <= 3 ? 4 : 2)) void potts_fused_kernel(FusedArgs<K, St> a)
+__global__ __launch_bounds__(kThreads, (K <= 3 ? 4 : 2)) void potts_fused_kernel(FusedArgs<K, St> a,
+                                                                                  const ColInfo *__restrict__ cols,
+                                                                                  const BandInfo *__restrict__ bands)
 {
     constexpr int KM = K - 1;
     constexpr int NP = K * (K - 1) / 2;
@@ -93,10 +110,10 @@ __global__ __launch_bounds__(kThreads, (K <= 3 ? 4 : 2)) void potts_fused_kernel
     const uint32_t xcd = blockIdx.x % kXcds, k = blockIdx.x / kXcds;
     const uint32_t b = (k / a.ntn) * kXcds + xcd, j = k % a.ntn;
     if (b >= a.nbands) return;   // uniform per workgroup
-    const BandInfo band = a.bands[b];
+    const BandInfo band = bands[b];
     const int r0 = int(band.r0);
     const int n0 = int(j) * kTN;
-    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave, w = uniform(threadIdx.x / kWave);
     const int len = int(a.len);
     const int n = n0 - 1 + lane;
     const bool n_ok = n >= 0 && n < len;
@@ -113,8 +130,8 @@ __global__ __launch_bounds__(kThreads, (K <= 3 ? 4 : 2)) void potts_fused_kernel
     };
 
     auto fetch = [&](int c, Col &f) {
-        const ColInfo cc = a.cols[c];
-        const ColInfo cn = (c + 1 < int(a.col_end)) ? a.cols[c + 1] : dead;
+        const ColInfo cc = load_col(cols, c);
+        const ColInfo cn = (c + 1 < int(a.col_end)) ? load_col(cols, c + 1) : dead;
 #pragma unroll
         for (int p = 0; p < kRowsPerWave; ++p) {
             const int pr = w * kRowsPerWave + p;
@@ -126,15 +143,15 @@ __global__ __launch_bounds__(kThreads, (K <= 3 ? 4 : 2)) void potts_fused_kernel
                 f.xin[p][i][0] = f.xin[p][i][1] = f.xin[p][i][2] = 0.0f;
             }
             if (n_ok && row_live(cc, r)) {
-                const uint64_t line = uint64_t(cc.line0 + uint32_t(r - int(cc.rb)));
-                const uint64_t off = line * a.wstride + uint64_t(n);
+                const uint32_t line = cc.line0 + uint32_t(r - int(cc.rb));
+                const uint32_t off = line * a.wstride + uint32_t(n);
 #pragma unroll
                 for (int i = 0; i < KM; ++i)
 #pragma unroll
-                    for (int ax = 0; ax < 3; ++ax) f.xin[p][i][ax] = St::ld(a.xi_in[(3 * i + ax) * a.wspan + off]);
+                    for (int ax = 0; ax < 3; ++ax) f.xin[p][i][ax] = St::ld(a.xi_in[3 * i + ax][off]);
                 if (pr >= 1 && t_lane) {
 #pragma unroll
-                    for (int i = 0; i < KM; ++i) f.vv[p][i] = a.V[i * a.wspan + off];
+                    for (int i = 0; i < KM; ++i) f.vv[p][i] = a.V[i][off];
                     f.uu[p] = a.u[line * a.in_stride + a.s0 + uint32_t(n)];
                 }
             }
@@ -147,10 +164,9 @@ __global__ __launch_bounds__(kThreads, (K <= 3 ? 4 : 2)) void potts_fused_kernel
                 const int i = pair / kBoxR, q = pair % kBoxR;
                 const int r = r0 - 1 + q;
                 if (row_live(cn, r)) {
-                    const typename St::T *src =
-                        a.vb_in + i * a.wspan + uint64_t(cn.line0 + uint32_t(r - int(cn.rb))) * a.wstride;
-                    if (n_ok) f.bx[t] = St::ld(src[n]);
-                    if (lane == 0 && n2 < len) f.bx2[t] = St::ld(src[n2]);
+                    const uint32_t off = (cn.line0 + uint32_t(r - int(cn.rb))) * a.wstride;
+                    if (n_ok) f.bx[t] = St::ld(a.vb_in[i][off + uint32_t(n)]);
+                    if (lane == 0 && n2 < len) f.bx2[t] = St::ld(a.vb_in[i][off + uint32_t(n2)]);
                 }
             }
         }
@@ -182,7 +198,7 @@ __global__ __launch_bounds__(kThreads, (K <= 3 ? 4 : 2)) void potts_fused_kernel
         commit_box(nxt, int(band.cs) & 1);
     } else {
         // no previous column to fetch from: load the first column's V-bar box directly
-        const ColInfo ci = a.cols[band.cs];
+        const ColInfo ci = load_col(cols, int(band.cs));
 #pragma unroll
         for (int t = 0; t < BOXP; ++t) {
             const int pair = w + t * kWaves;
@@ -191,10 +207,9 @@ __global__ __launch_bounds__(kThreads, (K <= 3 ? 4 : 2)) void potts_fused_kernel
                 const int r = r0 - 1 + q;
                 float v0 = 0.0f, v1 = 0.0f;
                 if (row_live(ci, r)) {
-                    const typename St::T *src =
-                        a.vb_in + i * a.wspan + uint64_t(ci.line0 + uint32_t(r - int(ci.rb))) * a.wstride;
-                    if (n_ok) v0 = St::ld(src[n]);
-                    if (lane == 0 && n2 < len) v1 = St::ld(src[n2]);
+                    const uint32_t off = (ci.line0 + uint32_t(r - int(ci.rb))) * a.wstride;
+                    if (n_ok) v0 = St::ld(a.vb_in[i][off + uint32_t(n)]);
+                    if (lane == 0 && n2 < len) v1 = St::ld(a.vb_in[i][off + uint32_t(n2)]);
                 }
                 L.box[int(band.cs) & 1][i][q][lane] = v0;
                 if (lane == 0) L.box[int(band.cs) & 1][i][q][kTNP] = v1;
@@ -203,10 +218,10 @@ __global__ __launch_bounds__(kThreads, (K <= 3 ? 4 : 2)) void potts_fused_kernel
     }
     fetch(int(band.cs), cur);
 
-    for (int c = int(band.cs); c < int(band.ce); ++c) {
-        const ColInfo cc = a.cols[c];
-        const ColInfo cn = (c + 1 < int(a.col_end)) ? a.cols[c + 1] : dead;
-        const ColInfo cp = (c > int(a.col_begin)) ? a.cols[c - 1] : dead;
+    auto step = [&](const int c, const Col &cur, Col &nxt) {
+        const ColInfo cc = load_col(cols, c);
+        const ColInfo cn = (c + 1 < int(a.col_end)) ? load_col(cols, c + 1) : dead;
+        const ColInfo cp = (c > int(a.col_begin)) ? load_col(cols, c - 1) : dead;
         commit_box(cur, (c + 1) & 1);
         if (threadIdx.x == 0) {
             L.wl_n = 0;
@@ -237,67 +252,78 @@ __global__ __launch_bounds__(kThreads, (K <= 3 ? 4 : 2)) void potts_fused_kernel
                 float q[K][3];
                 xi_input<K>(vb, vbn, nx, cur.xin[p], a.tau2, q);
                 queued = !dykstra_first_sweep_done<K>(q, a.lim);
-#pragma unroll
-                for (int i = 0; i < KM; ++i)
-#pragma unroll
-                    for (int ax = 0; ax < 3; ++ax) L.xl[pr][3 * i + ax][lane] = q[i][ax];
-                if (queued) {
-#pragma unroll
-                    for (int ax = 0; ax < 3; ++ax) L.qk[pr][ax][lane] = q[KM][ax];
+                // wave-aggregated append to the work list
+                const uint64_t mask = __ballot(queued);
+                int pos = 0;
+                if (mask) {
+                    const int leader = __ffsll((unsigned long long)mask) - 1;
+                    if (lane == leader) pos = atomicAdd(&L.wl_n, __popcll(mask));
+                    pos = __shfl(pos, leader, kWave) + __popcll(mask & ((1ull << lane) - 1ull));
                 }
-            }
-            // wave-aggregated append to the work list
-            const uint64_t mask = __ballot(queued);
-            if (mask) {
-                int base = 0;
-                const int leader = __ffsll((unsigned long long)mask) - 1;
-                if (lane == leader) base = atomicAdd(&L.wl_n, __popcll(mask));
-                base = __shfl(base, leader, kWave);
-                if (queued) {
-                    const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+                if (queued && pos < kMaxItems) {
+#pragma unroll
+                    for (int i = 0; i < K; ++i)
+#pragma unroll
+                        for (int ax = 0; ax < 3; ++ax) L.qb[pos][i][ax] = q[i][ax];
                     L.wl[pos] = uint16_t(pr * kTNP + lane);
+                } else {
+                    if (queued) dykstra<K>(q, a.lim);   // work list full: solve in place
+#pragma unroll
+                    for (int i = 0; i < KM; ++i)
+#pragma unroll
+                        for (int ax = 0; ax < 3; ++ax) L.xl[pr][3 * i + ax][lane] = St::ld(St::st(q[i][ax]));
                 }
             }
         }
         __syncthreads();   // S2: work list complete
 
         // ---- B: drain the work list, one sweep per trip ----
+        // Only as many waves as keep about kItemsPerLane items per lane busy
+        // join; the others go straight to the barrier.
         {
-            const int nwl = L.wl_n;
-            float q[K][3], r[NP][3];
-            int sw = 0, pt = 0;
-            bool have = false, drained = nwl == 0;
-            while (true) {
-                if (!have && !drained) {
-                    const int t = atomicAdd(&L.wl_next, 1);
-                    if (t < nwl) {
-                        pt = L.wl[t];
-                        const int pr = pt / kTNP, ln = pt % kTNP;
+            const int nwl = min(L.wl_n, kMaxItems);
+            const int waves_needed = (nwl + kItemsPerLane * kWave - 1) / (kItemsPerLane * kWave);
+            if (w < waves_needed) {
+                float q[K][3], r[NP][3];
+                int sw = 0, t = 0;
+                bool have = false, drained = false;
+                while (true) {
+                    const bool need = !have && !drained;
+                    const uint64_t mask = __ballot(need);
+                    if (mask) {
+                        const int leader = __ffsll((unsigned long long)mask) - 1;
+                        int base = 0;
+                        if (lane == leader) base = atomicAdd(&L.wl_next, __popcll(mask));
+                        base = __shfl(base, leader, kWave);
+                        if (need) {
+                            t = base + __popcll(mask & ((1ull << lane) - 1ull));
+                            if (t < nwl) {
 #pragma unroll
-                        for (int i = 0; i < KM; ++i)
+                                for (int i = 0; i < K; ++i)
 #pragma unroll
-                            for (int ax = 0; ax < 3; ++ax) q[i][ax] = L.xl[pr][3 * i + ax][ln];
+                                    for (int ax = 0; ax < 3; ++ax) q[i][ax] = L.qb[t][i][ax];
 #pragma unroll
-                        for (int ax = 0; ax < 3; ++ax) q[KM][ax] = L.qk[pr][ax][ln];
-#pragma unroll
-                        for (int pp = 0; pp < NP; ++pp) r[pp][0] = r[pp][1] = r[pp][2] = 0.0f;
-                        sw = 0;
-                        have = true;
-                    } else {
-                        drained = true;
+                                for (int pp = 0; pp < NP; ++pp) r[pp][0] = r[pp][1] = r[pp][2] = 0.0f;
+                                sw = 0;
+                                have = true;
+                            } else {
+                                drained = true;
+                            }
+                        }
                     }
-                }
-                if (!__any(have)) break;
-                if (have) {
-                    const float m2 = dykstra_sweep<K>(q, r, a.lim);
-                    ++sw;
-                    if (m2 < a.lim.stol || sw == 20) {
-                        const int pr = pt / kTNP, ln = pt % kTNP;
+                    if (!__any(have)) break;
+                    if (have) {
+                        const float m2 = dykstra_sweep<K>(q, r, a.lim);
+                        ++sw;
+                        if (m2 < a.lim.stol || sw == 20) {
+                            const int pt = L.wl[t];
+                            const int pr = pt / kTNP, ln = pt % kTNP;
 #pragma unroll
-                        for (int i = 0; i < KM; ++i)
+                            for (int i = 0; i < KM; ++i)
 #pragma unroll
-                            for (int ax = 0; ax < 3; ++ax) L.xl[pr][3 * i + ax][ln] = q[i][ax];
-                        have = false;
+                                for (int ax = 0; ax < 3; ++ax) L.xl[pr][3 * i + ax][ln] = St::ld(St::st(q[i][ax]));
+                            have = false;
+                        }
                     }
                 }
             }
@@ -315,11 +341,11 @@ __global__ __launch_bounds__(kThreads, (K <= 3 ? 4 : 2)) void potts_fused_kernel
 #pragma unroll
                 for (int ax = 0; ax < 3; ++ax) own[i][ax] = pval[p] ? L.xl[pr][3 * i + ax][lane] : 0.0f;
             if (pval[p] && pr >= 1 && t_lane) {
-                const uint64_t off = uint64_t(cc.line0 + uint32_t(r - int(cc.rb))) * a.wstride + uint64_t(n);
+                const uint32_t off = (cc.line0 + uint32_t(r - int(cc.rb))) * a.wstride + uint32_t(n);
 #pragma unroll
                 for (int i = 0; i < KM; ++i)
 #pragma unroll
-                    for (int ax = 0; ax < 3; ++ax) a.xi_out[(3 * i + ax) * a.wspan + off] = St::st(own[i][ax]);
+                    for (int ax = 0; ax < 3; ++ax) a.xi_out[3 * i + ax][off] = St::st(own[i][ax]);
                 const bool px[3] = {n > 0, r > int(cc.rb), row_live(cp, r)};
                 const bool nx[3] = {n + 1 < len, r + 1 < int(cc.re), row_live(cn, r)};
                 float xp[3][KM];
@@ -330,12 +356,15 @@ __global__ __launch_bounds__(kThreads, (K <= 3 ? 4 : 2)) void potts_fused_kernel
                     xp[2][i] = prev[p][i][2];
                 }
                 float vbo[KM];
-                const float d = v_update<K>(own, xp, px, nx, cur.vv[p], vbo, cur.uu[p], a.c, a.lambda, a.tau1);
+                float vv[KM];
+#pragma unroll
+                for (int i = 0; i < KM; ++i) vv[i] = cur.vv[p][i];
+                const float d = v_update<K>(own, xp, px, nx, vv, vbo, cur.uu[p], a.c, a.lambda, a.tau1);
                 if constexpr (TRACE) dqmax = fmaxf(dqmax, d);
 #pragma unroll
                 for (int i = 0; i < KM; ++i) {
-                    a.V[i * a.wspan + off] = cur.vv[p][i];
-                    a.vb_out[i * a.wspan + off] = St::st(vbo[i]);
+                    a.V[i][off] = vv[i];
+                    a.vb_out[i][off] = St::st(vbo[i]);
                 }
             }
 #pragma unroll
@@ -343,7 +372,10 @@ __global__ __launch_bounds__(kThreads, (K <= 3 ? 4 : 2)) void potts_fused_kernel
 #pragma unroll
                 for (int ax = 0; ax < 3; ++ax) prev[p][i][ax] = own[i][ax];
         }
-        cur = nxt;
+    };
+    for (int c = int(band.cs); c < int(band.ce); c += 2) {
+        step(c, cur, nxt);
+        if (c + 1 < int(band.ce)) step(c + 1, nxt, cur);
     }
 
     if constexpr (TRACE) {
@@ -364,8 +396,6 @@ int launch_fused(sct_potts_solver *s, int iters, float *dq_dev)
     using T = typename St::T;
     const Geometry &geo = s->g->geo;
     FusedArgs<K, St> a;
-    a.cols = s->d_cols;
-    a.bands = s->d_bands;
     a.nbands = s->n_bands;
     a.len = s->s1 - s->s0;
     a.ntn = (a.len + kTN - 1) / kTN;
@@ -374,8 +404,7 @@ int launch_fused(sct_potts_solver *s, int iters, float *dq_dev)
     a.s0 = s->s0;
     a.col_begin = geo.col_begin;
     a.col_end = geo.col_end;
-    a.wspan = s->wspan;
-    a.V = s->V;
+    for (int i = 0; i < K - 1; ++i) a.V[i] = s->V + i * s->wspan;
     a.u = (const float *)s->g->data;
     for (int i = 0; i < K; ++i) a.c[i] = s->c[i];
     a.tau1 = s->tau1;
@@ -395,16 +424,21 @@ int launch_fused(sct_potts_solver *s, int iters, float *dq_dev)
     }
     const uint32_t grid = ((a.nbands + kXcds - 1) / kXcds) * kXcds * a.ntn;
     for (int it = 0; it < iters; ++it) {
-        a.xi_in = (const T *)s->xi[s->cur];
-        a.xi_out = (T *)s->xi[s->cur ^ 1];
-        a.vb_in = (const T *)s->vbar[s->cur];
-        a.vb_out = (T *)s->vbar[s->cur ^ 1];
+        for (int j = 0; j < 3 * (K - 1); ++j) {
+            a.xi_in[j] = (const T *)s->xi[s->cur] + j * s->wspan;
+            a.xi_out[j] = (T *)s->xi[s->cur ^ 1] + j * s->wspan;
+        }
+        for (int i = 0; i < K - 1; ++i) {
+            a.vb_in[i] = (const T *)s->vbar[s->cur] + i * s->wspan;
+            a.vb_out[i] = (T *)s->vbar[s->cur ^ 1] + i * s->wspan;
+        }
+        potts_mark(s);
         if (dq_dev) {
             a.dq = dq_dev + it;
-            potts_fused_kernel<K, St, true><<<grid, kThreads, lds, s->ctx->stream>>>(a);
+            potts_fused_kernel<K, St, true><<<grid, kThreads, lds, s->ctx->stream>>>(a, s->d_cols, s->d_bands);
         } else {
             a.dq = nullptr;
-            potts_fused_kernel<K, St, false><<<grid, kThreads, lds, s->ctx->stream>>>(a);
+            potts_fused_kernel<K, St, false><<<grid, kThreads, lds, s->ctx->stream>>>(a, s->d_cols, s->d_bands);
         }
         s->cur ^= 1;
     }

@@ -110,7 +110,7 @@ def test_memory_driven_slabbing(ctx):
     m = ph.field.mapping.valid_mask()
     one = S.potts_min_partition(ctx, g, c, 30.0, TAU_REF, TAU_REF, 5).to_host()
     lines = ph.field.mapping.lines
-    per_slice = 4 * lines * 9 * 2          # fused kernel, K = 3
+    per_slice = lines * 4 * 5 * 2          # two-pass kernels, K = 3: Xi, V-bar, V in FP32
     few = S.potts_min_partition(ctx, g, c, 30.0, TAU_REF, TAU_REF, 5, mem_available=per_slice * 70)
     assert np.array_equal(few.to_host()[m], one[m])
     assert S.potts_min_partition(ctx, g, c, 30.0, TAU_REF, TAU_REF, 5, mem_available=per_slice * 8) is None
@@ -134,15 +134,16 @@ def test_invalid_arguments(ctx, c1_phantom):
         S.potts_min_partition(ctx, g, [0.2, 0.5], 30.0, TAU, TAU, -1)
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("storage", ["bf16", "fp16"])
 @pytest.mark.parametrize("K", [2, 3, 5])
-def test_reduced_storage_bitwise_vs_mirror(ctx, storage, K):
+def test_reduced_storage_bitwise_vs_mirror(ctx, storage, K, kernel):
     """Xi and V-bar stored as bfloat16 / half, arithmetic FP32: bitwise equal to
     the FP32 mirror with the same storage rounding."""
     ph = make_phantom(70, 30, 2.0, 5.0, seed=40 + K)
     c = np.linspace(0.2, 0.7, K).astype(np.float32)
     g = S.upload(ctx, ph.field)
-    s = S.PottsSolver(ctx, g, c, 30.0, TAU, TAU, dq_trace=True, storage=storage)
+    s = S.PottsSolver(ctx, g, c, 30.0, TAU, TAU, dq_trace=True, storage=storage, kernel=kernel)
     s.iterate(15)
     V, xi = s.state()
     res = s.labels().to_host()
@@ -172,3 +173,24 @@ def test_reduced_storage_c1_agreement_vs_fp64(ctx, c1_phantom, storage):
     vinf = np.abs(V[:, m] - r64["V"][:, m]).max()
     print(f"C1 {storage} storage: label agreement vs FP64 {agree:.6f}, ||V - V_fp64||_inf {vinf:.3e}")
     assert agree >= 0.999
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_iterate_timed_matches_iterate(ctx, kernel):
+    """The timed stepping call runs the same iterations: same state bitwise,
+    one positive duration per kernel launch."""
+    ph = make_phantom(40, 24, 2.0, 5.0, seed=31)
+    c = ph.normalized_labels()
+    g = S.upload(ctx, ph.field)
+    a = S.PottsSolver(ctx, g, c, 30.0, TAU, TAU, kernel=kernel)
+    b = S.PottsSolver(ctx, g, c, 30.0, TAU, TAU, kernel=kernel)
+    a.iterate(7)
+    t = b.iterate_timed(7)
+    assert t.shape == (7, 2 if kernel == "twopass" else 1)
+    assert np.all(t > 0)
+    Va, xa = a.state()
+    Vb, xb = b.state()
+    assert np.array_equal(_bits(Va), _bits(Vb)) and np.array_equal(_bits(xa), _bits(xb))
+    a.close()
+    b.close()
+    g.close()
