@@ -182,6 +182,13 @@ int sct_uv_hist(sct_ctx *ctx, const sct_field *u, const sct_field *v,
 int sct_uv_hist_auto(sct_ctx *ctx, const sct_field *u, const sct_field *v, uint32_t nbins,
                      uint32_t *counts, uint32_t *stride, float *ranges);
 
+/* ---- diagnostics (no counterpart in the reference) ------------------------ */
+/* Checks on the GPU that the Dykstra projection's range-specific correctly
+ * rounded square root (x in [1, 2^64)) and quotient (b in [1, 2^64),
+ * a in [2^-23, b)) equal IEEE sqrtf and division on n random operands;
+ * *bad_sqrt and *bad_div receive the numbers of differing results. */
+int sct_selftest_cr_math(sct_ctx *ctx, uint64_t n, uint64_t seed, uint64_t *bad_sqrt, uint64_t *bad_div);
+
 #ifdef __cplusplus
 }
 #endif

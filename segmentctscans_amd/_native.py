@@ -64,6 +64,8 @@ SIGNATURES = {
                                           C.c_float, C.POINTER(SctPottsOpts), C.POINTER(C.c_void_p)]),
     "sct_potts_solver_iterate": (C.c_int, [C.c_void_p, C.c_int]),
     "sct_potts_solver_kernels": (C.c_int, [C.c_void_p]),
+    "sct_selftest_cr_math": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_uint64)]),
     "sct_potts_solver_iterate_timed": (C.c_int, [C.c_void_p, C.c_int, f32p]),
     "sct_potts_solver_labels": (C.c_int, [C.c_void_p, C.c_void_p]),
     "sct_potts_solver_state": (C.c_int, [C.c_void_p, f32p, f32p]),

@@ -46,10 +46,12 @@ struct WorkGeom {
     uint32_t s0;         // slab start in input coordinates
 };
 
-// One wave = 64 consecutive n of one line.
+// One wave = 64 consecutive n of one line; 256-thread blocks in XCD runs of 64
+// (about 11 lines of 1426 voxels).
 __device__ __forceinline__ bool wave_voxel(const WorkGeom &g, uint32_t &line, uint32_t &n)
 {
-    const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x, 64);
+    const uint64_t gw = (uint64_t(b) * blockDim.x + threadIdx.x) / kWave;
     line = uint32_t(gw / g.chunks);
     n = uint32_t(gw % g.chunks) * kWave + (threadIdx.x % kWave);
     return line < g.lines && n < g.len;
@@ -89,8 +91,9 @@ struct TwoPassArgs {
     const float *u;
 };
 
-constexpr int kXiItems = 512;        // work-list capacity per workgroup
 constexpr int kXiChunksPerWave = 4;  // 64-voxel chunks per wave
+constexpr int kXiItems = 512;        // work-list capacity per workgroup
+constexpr int kXiWorkWaves = 4;      // waves that drain the work list
 
 // grad_kernel, potts-min-partition.cpp:96-247: Xi <- Pi_K(Xi + tau2 grad V-bar),
 // in place (each voxel reads its own Xi and its neighbours' V-bar only).
@@ -113,13 +116,14 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(WorkGeom g, TwoPassArgs<K
     }
     __syncthreads();
     const uint64_t total = uint64_t(g.lines) * g.chunks;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, 16);
     float res[kXiChunksPerWave][KM][3];
     int pos[kXiChunksPerWave];
     uint32_t wofs[kXiChunksPerWave];
     bool valid[kXiChunksPerWave];
 #pragma unroll
     for (int k = 0; k < kXiChunksPerWave; ++k) {
-        const uint64_t gw = (uint64_t(blockIdx.x) * (blockDim.x / kWave) + w) * kXiChunksPerWave + k;
+        const uint64_t gw = (uint64_t(blk) * (blockDim.x / kWave) + w) * kXiChunksPerWave + k;
         const uint32_t line = uint32_t(gw / g.chunks);
         const uint32_t n = uint32_t(gw % g.chunks) * kWave + lane;
         pos[k] = -1;
@@ -168,7 +172,7 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(WorkGeom g, TwoPassArgs<K
     }
     __syncthreads();   // work list complete
     const int nwl = min(nq, kXiItems);
-    if (w * kWave < nwl) {
+    if (w < kXiWorkWaves && w * kWave < nwl) {
         float q[K][3], r[NP][3];
         int sw = 0, t = 0;
         bool have = false, drained = false;

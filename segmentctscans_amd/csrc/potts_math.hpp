@@ -17,6 +17,34 @@ namespace sct {
 //                                             (double)sqrtf >= 1e-6)
 // Both follow from sqrtf being correctly rounded and monotone; the host
 // computes them once by bisection over float bit patterns.
+// Correctly rounded square root for finite x >= 1.  The hardware estimate
+// (within 1 ulp) moves to a neighbouring float when the residual x - s s' says
+// that float is closer: the correction the compiler applies to sqrtf, without
+// the scaling it adds for denormal and tiny inputs, which cannot occur here.
+__device__ __forceinline__ float sqrt_cr_ge1(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sdn = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sup = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = (__fmaf_rn(-sdn, s, x) <= 0.0f) ? sdn : s;
+    r = (__fmaf_rn(-sup, s, x) > 0.0f) ? sup : r;
+    return r;
+}
+
+// Correctly rounded a / b for normal a, b whose quotient and intermediate
+// products stay far from overflow and underflow (here 2^-23 <= a < b): a
+// refined reciprocal, a quotient and two residual corrections, the sequence
+// the compiler emits for division minus its range scaling and special-case
+// fix-up.
+__device__ __forceinline__ float div_cr_normal(float a, float b)
+{
+    const float r0 = __builtin_amdgcn_rcpf(b);
+    const float r = __fmaf_rn(__fmaf_rn(-b, r0, 1.0f), r0, r0);
+    const float q0 = a * r;
+    const float q1 = __fmaf_rn(__fmaf_rn(-b, q0, a), r, q0);
+    return __fmaf_rn(__fmaf_rn(-b, q1, a), r, q1);
+}
+
 struct DykstraLimits {
     float t1, stol;
 };
@@ -89,8 +117,9 @@ __device__ __forceinline__ float dykstra_sweep(float (&q)[K][3], float (&r)[K * 
             const float ss = __fmaf_rn(h2, h2, __fmaf_rn(h1, h1, h0 * h0));
             float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
             if (ss > lim.t1) {
-                const float l = sqrtf(ss);
-                const float s = (l - 1.0f) / l;
+                // ss > t1 gives l > 1, so l - 1 >= 2^-23 and l - 1 < l
+                const float l = sqrt_cr_ge1(ss);
+                const float s = div_cr_normal(l - 1.0f, l);
                 e0 = s * h0;
                 e1 = s * h1;
                 e2 = s * h2;

@@ -74,6 +74,20 @@ int sct_geometry_copy(sct_ctx *ctx, const Geometry &src, Geometry &dst);
 
 // ---- device helpers --------------------------------------------------------
 constexpr int kWave = 64;
+constexpr uint32_t kXcdCount = 8;   // MI355X: 8 XCDs, each with its own L2
+
+// Workgroup b of a grid of nb is dispatched to XCD b % 8.  xcd_block(b, nb, G)
+// is the logical block it works on: within every run of 8G logical blocks,
+// XCD x takes the G consecutive blocks [xG, xG + G), so neighbouring work
+// shares an L2 while the whole grid still advances as one front.  Blocks past
+// the last full run keep their index.  A bijection on [0, nb).
+__host__ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb, uint32_t G)
+{
+    const uint32_t run = kXcdCount * G;
+    if (b >= nb / run * run) return b;
+    const uint32_t x = b % kXcdCount, s = b / kXcdCount;
+    return (s / G) * run + x * G + s % G;
+}
 
 __device__ __forceinline__ float wave_max(float v)
 {

@@ -132,3 +132,14 @@ def test_device_memory(ctx):
     avail, total = S.device_memory(ctx)
     assert 0 < avail <= total
     assert total > 200 * 2**30   # MI355X: 288 GB HBM3E
+
+
+def test_dykstra_cr_sqrt_and_division_match_ieee(ctx):
+    """The Dykstra sweep's square root and quotient skip the compiler's range
+    scaling; on their ranges (x, b in [1, 2^64), a in [2^-23, b)) they must be
+    IEEE correctly rounded: 2^26 random operands each, bit for bit."""
+    import ctypes as C
+    from segmentctscans_amd import _native as N
+    bs, bd = C.c_uint64(), C.c_uint64()
+    ctx.check(N.lib().sct_selftest_cr_math(ctx.handle, 1 << 26, 12345, C.byref(bs), C.byref(bd)))
+    assert bs.value == 0 and bd.value == 0, (bs.value, bd.value)
