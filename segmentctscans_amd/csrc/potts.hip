@@ -40,6 +40,7 @@ struct WorkGeom {
     uint32_t lines;      // number of lines
     uint32_t len;        // slab length along n
     uint32_t chunks;     // ceil(len / 64)
+    uint32_t vchunks;    // ceil(len / (64 VPL)), two-pass kernels
     uint32_t wstride;    // work line stride
     uint64_t wspan;      // lines * wstride (component stride)
     uint32_t in_stride;  // input line stride
@@ -91,7 +92,37 @@ struct TwoPassArgs {
     const float *u;
 };
 
-constexpr int kXiChunksPerWave = 4;  // 64-voxel chunks per wave
+// The two-pass kernels give every lane VPL consecutive voxels along n, so that
+// each lane moves 8 bytes (FP32, VPL = 2) or 8 bytes (BF16 / FP16, VPL = 4)
+// per array access: 16-bit loads and stores one element per lane run the
+// memory system at half the rate of 32-bit ones.  A chunk is 64 lanes x VPL
+// voxels of one line.  Lanes past the end of a line keep their vector inside
+// the line's padding (wstride is a multiple of 32 >= len rounded up to VPL),
+// which nothing reads for a valid voxel.
+template <typename St>
+constexpr int vpl_of() { return sizeof(typename St::T) == 4 ? 2 : 4; }
+
+// VPL consecutive stored elements, one load or store.
+template <typename St, int N>
+struct Packed {
+    using T = typename St::T;
+    typedef T V __attribute__((ext_vector_type(N)));
+    static __device__ __forceinline__ void ld(const T *p, float (&o)[N])
+    {
+        const V v = *reinterpret_cast<const V *>(p);
+#pragma unroll
+        for (int j = 0; j < N; ++j) o[j] = St::ld(v[j]);
+    }
+    static __device__ __forceinline__ void st(T *p, const float (&x)[N])
+    {
+        V v;
+#pragma unroll
+        for (int j = 0; j < N; ++j) v[j] = St::st(x[j]);
+        *reinterpret_cast<V *>(p) = v;
+    }
+};
+
+constexpr int kXiVoxelsPerLane = 4;  // per lane and workgroup: kXiVoxelsPerLane / VPL chunks per wave
 constexpr int kXiItems = 512;        // work-list capacity per workgroup
 constexpr int kXiWorkWaves = 4;      // waves that drain the work list
 
@@ -101,12 +132,14 @@ constexpr int kXiWorkWaves = 4;      // waves that drain the work list
 //      Dykstra stops after one sweep, the rest join a work list in LDS;
 //   B  the work list is drained one sweep per loop trip, each lane taking a new
 //      item as soon as its item converges, the result going back to LDS;
-//   C  every lane stores the Xi+ of its voxels in full waves.
+//   C  every lane stores the Xi+ of its voxels, VPL at a time.
 template <int K, typename St>
 __global__ __launch_bounds__(256) void potts_xi_kernel(WorkGeom g, TwoPassArgs<K, St> a, float tau2,
                                                        DykstraLimits lim)
 {
     constexpr int KM = K - 1, NP = K * (K - 1) / 2;
+    constexpr int VPL = vpl_of<St>(), CPW = kXiVoxelsPerLane / VPL;
+    using P = Packed<St, VPL>;
     __shared__ float qb[kXiItems][K][3];   // inputs, then results, of the queued voxels
     __shared__ int nq, next;
     const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
@@ -115,39 +148,60 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(WorkGeom g, TwoPassArgs<K
         next = 0;
     }
     __syncthreads();
-    const uint64_t total = uint64_t(g.lines) * g.chunks;
+    const uint64_t total = uint64_t(g.lines) * g.vchunks;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, 16);
-    float res[kXiChunksPerWave][KM][3];
-    int pos[kXiChunksPerWave];
-    uint32_t wofs[kXiChunksPerWave];
-    bool valid[kXiChunksPerWave];
+    float res[CPW][VPL][KM][3];
+    int pos[CPW][VPL];
+    uint32_t wofs[CPW];
+    int nv[CPW];   // valid voxels of this lane in chunk k
 #pragma unroll
-    for (int k = 0; k < kXiChunksPerWave; ++k) {
-        const uint64_t gw = (uint64_t(blk) * (blockDim.x / kWave) + w) * kXiChunksPerWave + k;
-        const uint32_t line = uint32_t(gw / g.chunks);
-        const uint32_t n = uint32_t(gw % g.chunks) * kWave + lane;
-        pos[k] = -1;
-        valid[k] = gw < total && n < g.len;
-        wofs[k] = valid[k] ? line * g.wstride + n : 0;
-        bool queued = false;
-        if (valid[k]) {
-            const LineNbr nb = g.nbr[line];
-            const uint32_t w0 = wofs[k];
-            const bool nx[3] = {n + 1 < g.len, nb.nrow >= 0, nb.ncol >= 0};
-            const uint32_t wn[3] = {w0 + 1, uint32_t(nb.nrow) * g.wstride + n, uint32_t(nb.ncol) * g.wstride + n};
-            float vb[KM], vbn[3][KM], x[KM][3];
+    for (int k = 0; k < CPW; ++k) {
+        const uint64_t gw = (uint64_t(blk) * (blockDim.x / kWave) + w) * CPW + k;
+        const uint32_t line = uint32_t(gw / g.vchunks);
+        const uint32_t n0 = uint32_t(gw % g.vchunks) * (kWave * VPL) + lane * VPL;
+        nv[k] = (gw < total && n0 < g.len) ? int(min(uint32_t(VPL), g.len - n0)) : 0;
+        wofs[k] = nv[k] ? line * g.wstride + n0 : 0;
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) pos[k][j] = -1;
+        if (gw >= total) continue;   // uniform over the wave
+        const LineNbr nb = g.nbr[line];
+        const uint32_t w0 = wofs[k];
+        float vb[KM][VPL], vr[KM][VPL], vc[KM][VPL], x[KM][3][VPL], vn[KM][VPL];
+#pragma unroll
+        for (int i = 0; i < KM; ++i) {
+#pragma unroll
+            for (int j = 0; j < VPL; ++j) vb[i][j] = vr[i][j] = vc[i][j] = 0.0f;
+            if (nv[k]) {
+                P::ld(a.vbar[i] + w0, vb[i]);
+                if (nb.nrow >= 0) P::ld(a.vbar[i] + uint32_t(nb.nrow) * g.wstride + n0, vr[i]);
+                if (nb.ncol >= 0) P::ld(a.vbar[i] + uint32_t(nb.ncol) * g.wstride + n0, vc[i]);
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) P::ld(a.xi[3 * i + ax] + w0, x[i][ax]);
+            }
+            // V-bar at n + 1: the next element of the lane, the next lane's first,
+            // or (last lane) the first of the next chunk
+#pragma unroll
+            for (int j = 0; j + 1 < VPL; ++j) vn[i][j] = vb[i][j + 1];
+            vn[i][VPL - 1] = __shfl_down(vb[i][0], 1, kWave);
+            if (lane == kWave - 1 && nv[k] == VPL && n0 + VPL < g.len) vn[i][VPL - 1] = St::ld(a.vbar[i][w0 + VPL]);
+        }
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) {
+            const bool valid = j < nv[k];
+            const bool nx[3] = {n0 + j + 1 < g.len, nb.nrow >= 0, nb.ncol >= 0};
+            float vbj[KM], vbn[3][KM], xj[KM][3];
 #pragma unroll
             for (int i = 0; i < KM; ++i) {
-                vb[i] = St::ld(a.vbar[i][w0]);
+                vbj[i] = vb[i][j];
+                vbn[0][i] = vn[i][j];
+                vbn[1][i] = vr[i][j];
+                vbn[2][i] = vc[i][j];
 #pragma unroll
-                for (int ax = 0; ax < 3; ++ax) {
-                    vbn[ax][i] = nx[ax] ? St::ld(a.vbar[i][wn[ax]]) : 0.0f;
-                    x[i][ax] = St::ld(a.xi[3 * i + ax][w0]);
-                }
+                for (int ax = 0; ax < 3; ++ax) xj[i][ax] = x[i][ax][j];
             }
             float q[K][3];
-            xi_input<K>(vb, vbn, nx, x, tau2, q);
-            queued = !dykstra_first_sweep_done<K>(q, lim);
+            xi_input<K>(vbj, vbn, nx, xj, tau2, q);
+            const bool queued = valid && !dykstra_first_sweep_done<K>(q, lim);
             const uint64_t mask = __ballot(queued);
             int p = 0;
             if (mask) {
@@ -160,14 +214,14 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(WorkGeom g, TwoPassArgs<K
                 for (int i = 0; i < K; ++i)
 #pragma unroll
                     for (int ax = 0; ax < 3; ++ax) qb[p][i][ax] = q[i][ax];
-                pos[k] = p;
+                pos[k][j] = p;
             } else if (queued) {
                 dykstra<K>(q, lim);   // work list full: solve in place
             }
 #pragma unroll
             for (int i = 0; i < KM; ++i)
 #pragma unroll
-                for (int ax = 0; ax < 3; ++ax) res[k][i][ax] = q[i][ax];
+                for (int ax = 0; ax < 3; ++ax) res[k][j][i][ax] = valid ? q[i][ax] : 0.0f;
         }
     }
     __syncthreads();   // work list complete
@@ -216,51 +270,94 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(WorkGeom g, TwoPassArgs<K
     }
     __syncthreads();   // results of the work list in LDS
 #pragma unroll
-    for (int k = 0; k < kXiChunksPerWave; ++k) {
-        if (!valid[k]) continue;
-        if (pos[k] >= 0) {
-#pragma unroll
-            for (int i = 0; i < KM; ++i)
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) res[k][i][ax] = qb[pos[k]][i][ax];
-        }
+    for (int k = 0; k < CPW; ++k) {
+        if (!nv[k]) continue;
 #pragma unroll
         for (int i = 0; i < KM; ++i)
 #pragma unroll
-            for (int ax = 0; ax < 3; ++ax) a.xi[3 * i + ax][wofs[k]] = St::st(res[k][i][ax]);
+            for (int ax = 0; ax < 3; ++ax) {
+                float o[VPL];
+#pragma unroll
+                for (int j = 0; j < VPL; ++j) o[j] = pos[k][j] >= 0 ? qb[pos[k][j]][i][ax] : res[k][j][i][ax];
+                P::st(a.xi[3 * i + ax] + wofs[k], o);
+            }
     }
 }
 
 // div_kernel, potts-min-partition.cpp:349-488: V <- Pi_S(V + tau1 (div Xi - G)),
 // V-bar <- 2 V+ - V.  With TRACE, the block maximum of sum_k dV_k^2 goes to *dq.
+// One chunk per wave; 256-thread blocks in XCD runs of 64 / VPL (about 11 lines).
 template <int K, typename St, bool TRACE>
 __global__ __launch_bounds__(256) void potts_v_kernel(WorkGeom g, TwoPassArgs<K, St> a, Labels<K> lab,
                                                       float lambda, float tau1, float *dq)
 {
-    uint32_t line, n;
+    constexpr int KM = K - 1, VPL = vpl_of<St>();
+    using P = Packed<St, VPL>;
+    using PF = Packed<StorageF32, VPL>;
+    const int lane = threadIdx.x % kWave;
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x, 64 / VPL);
+    const uint64_t gw = (uint64_t(b) * blockDim.x + threadIdx.x) / kWave;
     float d = 0.0f;
-    if (wave_voxel(g, line, n)) {
+    if (gw < uint64_t(g.lines) * g.vchunks) {   // uniform over the wave
+        const uint32_t line = uint32_t(gw / g.vchunks);
+        const uint32_t n0 = uint32_t(gw % g.vchunks) * (kWave * VPL) + lane * VPL;
+        const int nv = n0 < g.len ? int(min(uint32_t(VPL), g.len - n0)) : 0;
         const LineNbr nb = g.nbr[line];
-        const uint32_t w = line * g.wstride + n;
-        const bool px[3] = {n > 0, nb.prow >= 0, nb.pcol >= 0};
-        const bool nx[3] = {n + 1 < g.len, nb.nrow >= 0, nb.ncol >= 0};
-        const uint32_t wp[3] = {w - 1, uint32_t(nb.prow) * g.wstride + n, uint32_t(nb.pcol) * g.wstride + n};
-        float xo[K - 1][3], xp[3][K - 1], v[K - 1], vb[K - 1];
+        const uint32_t w0 = line * g.wstride + n0;
+        float xo[KM][3][VPL], xr[KM][VPL], xc[KM][VPL], xn[KM][VPL], v[KM][VPL], u[VPL];
 #pragma unroll
-        for (int i = 0; i < K - 1; ++i) {
-            v[i] = a.V[i][w];
+        for (int i = 0; i < KM; ++i) {
 #pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                xo[i][ax] = St::ld(a.xi[3 * i + ax][w]);
-                xp[ax][i] = px[ax] ? St::ld(a.xi[3 * i + ax][wp[ax]]) : 0.0f;
+            for (int j = 0; j < VPL; ++j) {
+                xr[i][j] = xc[i][j] = v[i][j] = 0.0f;
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) xo[i][ax][j] = 0.0f;
+            }
+            if (nv) {
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) P::ld(a.xi[3 * i + ax] + w0, xo[i][ax]);
+                if (nb.prow >= 0) P::ld(a.xi[3 * i + 1] + uint32_t(nb.prow) * g.wstride + n0, xr[i]);
+                if (nb.pcol >= 0) P::ld(a.xi[3 * i + 2] + uint32_t(nb.pcol) * g.wstride + n0, xc[i]);
+                PF::ld(a.V[i] + w0, v[i]);
+            }
+            // Xi_n at n - 1: the previous element of the lane, the previous
+            // lane's last, or (first lane) the last of the previous chunk
+#pragma unroll
+            for (int j = 1; j < VPL; ++j) xn[i][j] = xo[i][0][j - 1];
+            xn[i][0] = __shfl_up(xo[i][0][VPL - 1], 1, kWave);
+            if (lane == 0 && nv && n0 > 0) xn[i][0] = St::ld(a.xi[3 * i][w0 - 1]);
+        }
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) u[j] = j < nv ? a.u[line * g.in_stride + g.s0 + n0 + j] : 0.0f;
+        float vb[KM][VPL];
+#pragma unroll
+        for (int j = 0; j < VPL; ++j) {
+            const bool px[3] = {n0 + j > 0, nb.prow >= 0, nb.pcol >= 0};
+            const bool nx[3] = {n0 + j + 1 < g.len, nb.nrow >= 0, nb.ncol >= 0};
+            float xoj[KM][3], xpj[3][KM], vj[KM], vbj[KM];
+#pragma unroll
+            for (int i = 0; i < KM; ++i) {
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) xoj[i][ax] = xo[i][ax][j];
+                xpj[0][i] = xn[i][j];
+                xpj[1][i] = xr[i][j];
+                xpj[2][i] = xc[i][j];
+                vj[i] = v[i][j];
+            }
+            const float dj = v_update<K>(xoj, xpj, px, nx, vj, vbj, u[j], lab.c, lambda, tau1);
+            if (j < nv) d = fmaxf(d, dj);
+#pragma unroll
+            for (int i = 0; i < KM; ++i) {
+                v[i][j] = vj[i];
+                vb[i][j] = vbj[i];
             }
         }
-        const float uu = a.u[line * g.in_stride + g.s0 + n];
-        d = v_update<K>(xo, xp, px, nx, v, vb, uu, lab.c, lambda, tau1);
+        if (nv) {
 #pragma unroll
-        for (int i = 0; i < K - 1; ++i) {
-            a.V[i][w] = v[i];
-            a.vbar[i][w] = St::st(vb[i]);
+            for (int i = 0; i < KM; ++i) {
+                PF::st(a.V[i] + w0, v[i]);
+                P::st(a.vbar[i] + w0, vb[i]);
+            }
         }
     }
     if constexpr (TRACE) {
@@ -353,17 +450,21 @@ int twopass_iterate(sct_potts_solver *s, const WorkGeom &g, int iters, float *dq
         a.V[i] = s->V + i * s->wspan;
     }
     a.u = (const float *)s->g->data;
-    const uint64_t waves = uint64_t(g.lines) * g.chunks;
-    const uint32_t xi_grid = uint32_t((waves + 4 * kXiChunksPerWave - 1) / (4 * kXiChunksPerWave));
-    const uint32_t v_grid = grid_for(g);
+    constexpr int VPL = vpl_of<St>();
+    WorkGeom gv = g;
+    gv.vchunks = (g.len + kWave * VPL - 1) / (kWave * VPL);
+    const uint64_t waves = uint64_t(g.lines) * gv.vchunks;
+    const uint64_t xi_waves = (waves + kXiVoxelsPerLane / VPL - 1) / (kXiVoxelsPerLane / VPL);
+    const uint32_t xi_grid = uint32_t((xi_waves + 3) / 4);
+    const uint32_t v_grid = uint32_t((waves + 3) / 4);
     for (int i = 0; i < iters; ++i) {
         potts_mark(s);
-        potts_xi_kernel<K, St><<<xi_grid, 256, 0, s->ctx->stream>>>(g, a, s->tau2, lim);
+        potts_xi_kernel<K, St><<<xi_grid, 256, 0, s->ctx->stream>>>(gv, a, s->tau2, lim);
         potts_mark(s);
         if (dq_dev)
-            potts_v_kernel<K, St, true><<<v_grid, 256, 0, s->ctx->stream>>>(g, a, lab, s->lambda, s->tau1, dq_dev + i);
+            potts_v_kernel<K, St, true><<<v_grid, 256, 0, s->ctx->stream>>>(gv, a, lab, s->lambda, s->tau1, dq_dev + i);
         else
-            potts_v_kernel<K, St, false><<<v_grid, 256, 0, s->ctx->stream>>>(g, a, lab, s->lambda, s->tau1, nullptr);
+            potts_v_kernel<K, St, false><<<v_grid, 256, 0, s->ctx->stream>>>(gv, a, lab, s->lambda, s->tau1, nullptr);
     }
     SCT_HIP(s->ctx, hipGetLastError());
     return SCT_OK;
