@@ -112,10 +112,12 @@ typedef struct sct_potts_opts {
     int64_t  mem_available; /* >=0: pretend device_memory().available is this     */
     float   *dq_trace;      /* optional host [iters]: max_x sum_k (dV_k)^2 per
                                iteration (last slab when several)                 */
-    double  *gap_trace;     /* optional host [iters / gap_every]: relative
-                               primal-dual gap, FP64 (extension)                  */
+    double  *gap_trace;     /* optional host [gap_capacity]: relative primal-dual
+                               gap (E - D) / E in FP64 after every gap_every-th
+                               iteration (extension; last slab when several)      */
     int32_t  gap_every;     /* iterations between gap evaluations (0: never)      */
-    int32_t  reserved;
+    int32_t  gap_capacity;  /* entries of gap_trace; sct_potts_min_partition
+                               takes 0 as iters / gap_every                       */
 } sct_potts_opts;
 
 void sct_potts_opts_init(sct_potts_opts *opts);
@@ -143,6 +145,11 @@ int  sct_potts_solver_kernels(const sct_potts_solver *s);
  * counterpart in the reference, which has no kernel timing. */
 int  sct_potts_solver_iterate_timed(sct_potts_solver *s, int iters, float *kernel_ms);
 int  sct_potts_solver_labels(sct_potts_solver *s, sct_field *res);
+/* FP64 primal energy E(V) and dual energy D(Xi) of the current state for the
+ * saddle function <V, G> + <grad V, Xi>, G_i = lambda (u - c_i)^2; E >= D and
+ * E - D -> 0 as the iteration converges.  Synchronises.  No counterpart in the
+ * reference, which runs a fixed number of iterations. */
+int  sct_potts_solver_gap(sct_potts_solver *s, double *primal, double *dual);
 /* Downloads V ((K-1) x span) and Xi (3(K-1) x span), component-major, in the
  * input layout; either pointer may be NULL.  Synchronises. */
 int  sct_potts_solver_state(sct_potts_solver *s, float *V, float *xi);

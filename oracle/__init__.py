@@ -50,6 +50,11 @@ def lib() -> C.CDLL:
                                  C.c_int, C.c_int, f32p, f64p, f64p, f64p, C.c_int, C.c_int]
         L.or_potts_f.argtypes = [C.POINTER(OrLayout), f32p, C.c_int, f32p, C.c_float, C.c_float, C.c_float,
                                  C.c_int, C.c_int, f32p, f32p, f32p, f32p, C.c_int, C.c_int]
+        L.or_support_d.argtypes = [C.c_int, f64p]
+        L.or_support_d.restype = C.c_double
+        L.or_support_irls_d.argtypes = [C.c_int, f64p]
+        L.or_support_irls_d.restype = C.c_double
+        L.or_potts_gap_d.argtypes = [C.POINTER(OrLayout), f32p, C.c_int, f32p, C.c_float, f64p, f64p, f64p, f64p]
         L.or_bf16_round.argtypes = [C.c_float]
         L.or_bf16_round.restype = C.c_float
         L.or_f16_round.argtypes = [C.c_float]
@@ -128,6 +133,28 @@ def potts(layout: Layout, u: np.ndarray, c, lam, tau1, tau2, iters, *, precision
         out["V"] = V.reshape(K - 1, layout.span)
         out["xi"] = xi.reshape(3 * (K - 1), layout.span)
     return out
+
+
+def support(p, method: str = "auto") -> float:
+    """h_K(p) for p of shape (K, 3) with rows summing to 0 (K = 2..6)."""
+    p = np.ascontiguousarray(p, np.float64)
+    fn = lib().or_support_irls_d if method == "irls" else lib().or_support_d
+    return fn(p.shape[0], _ptr(p, f64p))
+
+
+def potts_gap(layout: Layout, u, c, lam, V, xi) -> tuple[float, float]:
+    """FP64 primal and dual energies (E, D) of the Potts saddle problem for the
+    state V ((K-1) x span) and Xi (3(K-1) x span)."""
+    c = np.ascontiguousarray(c, np.float32)
+    u = np.ascontiguousarray(u, np.float32)
+    V = np.ascontiguousarray(V, np.float64)
+    xi = np.ascontiguousarray(xi, np.float64)
+    E, D = C.c_double(), C.c_double()
+    lay = layout.c()
+    rc = lib().or_potts_gap_d(C.byref(lay), _ptr(u, f32p), c.size, _ptr(c, f32p), lam, _ptr(V, f64p),
+                              _ptr(xi, f64p), C.byref(E), C.byref(D))
+    assert rc == 0
+    return E.value, D.value
 
 
 def store_round(x, storage: str) -> np.ndarray:

@@ -311,3 +311,153 @@ void or_uv_hist(const or_layout *m, const float *u, const float *v,
                 counts[(size_t)yb * stride + (size_t)xb] += 1;
             }
 }
+
+/* ---- FP64 duality gap of the Potts saddle problem ---------------------------
+ * Not in the reference; the definition the library's gap follows:
+ *   L(V, Xi) = <V, G> + <grad V, Xi>,  G_i = lambda (u - c_i)^2,
+ *   V in the simplex, Xi in K = {|xi_i - xi_j| <= 1, i < j} per voxel.
+ *   primal E(V)  = <V, G> + sum_x h_K(grad V(x)),
+ *   dual   D(Xi) = sum_x min_i (G_i - (div Xi)_i),
+ * with h_K the support function of K.  By conic duality h_K(p) is the
+ * cheapest flow on the complete graph with node supplies p_i:
+ *   h_K(p) = min sum_{i<j} |f_ij|  s.t.  p_i = sum_{j>i} f_ij - sum_{j<i} f_ji.
+ * K = 2: |p_0|.  K = 3: the Fermat-Weber cost of the points {0, p_0, -p_1}.
+ * K >= 4: iteratively reweighted least squares (conductances = current flow
+ * magnitudes, node K-1 grounded), 256 steps from unit conductances, taking the
+ * cheapest iterate; every iterate is a feasible flow, so the value is an
+ * upper bound of h_K and E an upper bound of the primal energy. */
+#define OR_IRLS_STEPS 256
+
+static double norm3(const double *v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+static double fermat3(const double *p0, const double *p1)
+{
+    /* points A = 0, B = p0, C = -p1 */
+    double s[3], cr[3];
+    for (int a = 0; a < 3; ++a) s[a] = p0[a] + p1[a];
+    const double ab = norm3(p0), ac = norm3(p1), bc = norm3(s);
+    /* angle >= 120 degrees at a vertex: the median is that vertex */
+    if (-dot3(p0, p1) <= -0.5 * ab * ac) return ab + ac;          /* at A: <B, C> */
+    if (dot3(p0, s) <= -0.5 * ab * bc) return ab + bc;            /* at B: <A-B, C-B> */
+    if (dot3(p1, s) <= -0.5 * ac * bc) return ac + bc;            /* at C: <A-C, B-C> */
+    cr[0] = p0[1] * p1[2] - p0[2] * p1[1];
+    cr[1] = p0[2] * p1[0] - p0[0] * p1[2];
+    cr[2] = p0[0] * p1[1] - p0[1] * p1[0];
+    return sqrt(0.5 * (ab * ab + ac * ac + bc * bc) + sqrt(3.0) * norm3(cr));
+}
+
+static double irls_flow(int K, const double (*p)[3])
+{
+    double c[15], L[25], ch[25], phi[6][3], best = INFINITY, scale = 0.0;
+    int np = K * (K - 1) / 2, M = K - 1;
+    for (int i = 0; i < K; ++i) scale = fmax(scale, norm3(p[i]));
+    if (scale == 0.0) return 0.0;
+    for (int e = 0; e < np; ++e) c[e] = 1.0;
+    for (int step = 0; step < OR_IRLS_STEPS; ++step) {
+        memset(L, 0, sizeof(L));
+        for (int i = 0, e = 0; i < K; ++i)
+            for (int j = i + 1; j < K; ++j, ++e) {
+                if (i < M) L[i * M + i] += c[e];
+                if (j < M) L[j * M + j] += c[e];
+                if (i < M && j < M) { L[i * M + j] -= c[e]; L[j * M + i] -= c[e]; }
+            }
+        /* Cholesky of the grounded Laplacian */
+        for (int r = 0; r < M; ++r)
+            for (int q = 0; q <= r; ++q) {
+                double acc = L[r * M + q];
+                for (int t = 0; t < q; ++t) acc -= ch[r * M + t] * ch[q * M + t];
+                ch[r * M + q] = (r == q) ? sqrt(acc) : acc / ch[q * M + q];
+            }
+        for (int a = 0; a < 3; ++a) {
+            double y[6];
+            for (int r = 0; r < M; ++r) {
+                double acc = p[r][a];
+                for (int t = 0; t < r; ++t) acc -= ch[r * M + t] * y[t];
+                y[r] = acc / ch[r * M + r];
+            }
+            for (int r = M - 1; r >= 0; --r) {
+                double acc = y[r];
+                for (int t = r + 1; t < M; ++t) acc -= ch[t * M + r] * phi[t][a];
+                phi[r][a] = acc / ch[r * M + r];
+            }
+            phi[M][a] = 0.0;
+        }
+        double cost = 0.0;
+        for (int i = 0, e = 0; i < K; ++i)
+            for (int j = i + 1; j < K; ++j, ++e) {
+                double f[3];
+                for (int a = 0; a < 3; ++a) f[a] = c[e] * (phi[i][a] - phi[j][a]);
+                const double fn = norm3(f);
+                cost += fn;
+                c[e] = fmax(fn, 1e-12 * scale);
+            }
+        if (cost < best) best = cost;
+    }
+    return best;
+}
+
+double or_support_d(int K, const double *p)
+{
+    const double (*pp)[3] = (const double (*)[3])p;
+    if (K == 2) return norm3(pp[0]);
+    if (K == 3) return fermat3(pp[0], pp[1]);
+    return irls_flow(K, pp);
+}
+
+double or_support_irls_d(int K, const double *p) { return irls_flow(K, (const double (*)[3])p); }
+
+int or_potts_gap_d(const or_layout *m, const float *u, int K, const float *c, float lambda,
+                   const double *V, const double *xi, double *primal, double *dual)
+{
+    if (K < 2 || K > 6) return 1;
+    const size_t span = m->base[m->width];
+    double E = 0.0, D = 0.0;
+    for (uint32_t col = m->col_begin; col < m->col_end; ++col)
+        for (uint32_t row = row_begin(m, col); row < row_end(m, col); ++row)
+            for (uint32_t n = 0; n < m->n_images; ++n) {
+                const size_t o = line_of(m, row, col) * m->nstride + n;
+                const int nx[3] = {n + 1 < m->n_images, has_next_row(m, row, col), has_next_col(m, row, col)};
+                const int px[3] = {n > 0, has_prev_row(m, row, col), has_prev_col(m, row, col)};
+                const size_t on[3] = {o + 1, o + m->nstride, nx[2] ? line_of(m, row, col + 1) * m->nstride + n : 0};
+                const size_t op[3] = {o - 1, o - m->nstride, px[2] ? line_of(m, row, col - 1) * m->nstride + n : 0};
+                double v[6], p[6][3], dv[6], g[6];
+                double vk = 1.0;
+                for (int i = 0; i < K - 1; ++i) { v[i] = V[i * span + o]; vk -= v[i]; }
+                v[K - 1] = vk;
+                for (int a = 0; a < 3; ++a) {
+                    double sk = 0.0;
+                    for (int i = 0; i < K - 1; ++i) {
+                        p[i][a] = nx[a] ? V[i * span + on[a]] - v[i] : 0.0;
+                        sk -= p[i][a];
+                    }
+                    p[K - 1][a] = sk;
+                }
+                for (int i = 0; i < K; ++i) dv[i] = 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    double own_k = 0.0, prev_k = 0.0;
+                    for (int i = 0; i < K - 1; ++i) {
+                        const double own = xi[(3 * i + a) * span + o];
+                        const double prev = px[a] ? xi[(3 * i + a) * span + op[a]] : 0.0;
+                        if (nx[a]) dv[i] += own;
+                        if (px[a]) dv[i] -= prev;
+                        own_k -= own;
+                        prev_k -= prev;
+                    }
+                    if (nx[a]) dv[K - 1] += own_k;
+                    if (px[a]) dv[K - 1] -= prev_k;
+                }
+                double lin = 0.0, mn = INFINITY;
+                for (int i = 0; i < K; ++i) {
+                    const double d = (double)u[o] - (double)c[i];
+                    g[i] = (double)lambda * d * d;
+                    lin += v[i] * g[i];
+                    mn = fmin(mn, g[i] - dv[i]);
+                }
+                E += lin + or_support_d(K, &p[0][0]);
+                D += mn;
+            }
+    *primal = E;
+    *dual = D;
+    return 0;
+}

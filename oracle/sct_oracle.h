@@ -84,6 +84,14 @@ int or_potts_d(const or_layout *m, const float *u, int K, const float *c,
 int or_potts_f(const or_layout *m, const float *u, int K, const float *c,
                float lambda, float tau1, float tau2, int iters, int slab_w,
                float *res, float *V_out, float *xi_out, float *dq, int storage, int nthreads);
+/* FP64 duality gap of the Potts saddle problem (no counterpart in the
+ * reference; definition in sct_oracle.c).  V: (K-1) x span, xi: 3(K-1) x span,
+ * component-major in the input layout. */
+double or_support_d(int K, const double *p);        /* p: K x 3, sum_i p_i = 0 */
+double or_support_irls_d(int K, const double *p);   /* the K >= 4 method, any K >= 2 */
+int or_potts_gap_d(const or_layout *m, const float *u, int K, const float *c, float lambda,
+                   const double *V, const double *xi, double *primal, double *dual);
+
 float or_bf16_round(float f);
 float or_f16_round(float f);
 

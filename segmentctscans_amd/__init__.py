@@ -269,6 +269,7 @@ def download(ctx: Context, dfield: DeviceField, out: Field) -> None:
 # ---------------------------------------------------------------------------
 def _opts(storage="fp32", kernel="auto", slab_w=None, mem_available=None, dq=None,
           gap=None, gap_every=0) -> N.SctPottsOpts:
+    """gap: host float64 array receiving the relative gap every gap_every iterations."""
     o = N.SctPottsOpts()
     N.lib().sct_potts_opts_init(C.byref(o))
     o.storage = N.STORAGE[storage]
@@ -280,32 +281,37 @@ def _opts(storage="fp32", kernel="auto", slab_w=None, mem_available=None, dq=Non
     if gap is not None:
         o.gap_trace = gap.ctypes.data_as(N.f64p)
         o.gap_every = int(gap_every)
+        o.gap_capacity = int(gap.size)
     return o
 
 
 def potts_min_partition(ctx: Context, g: DeviceField, ci, lam: float, tau1: float, tau2: float,
                         iters: int, *, storage="fp32", kernel="auto", slab_w=None,
-                        mem_available=None, dq_trace=False):
+                        mem_available=None, dq_trace=False, gap_every=0):
     """potts_min_partition<K>(ctx, g, cpu_map, ci, lambda, tau1, tau2, iters).
 
     Returns a DeviceField of c[argmax] labels, or None when the slab cannot
     hold the halo (the reference's null return, potts-min-partition.cpp:571-577).
-    With dq_trace=True returns (field, dq[iters]).
+    With dq_trace=True returns (field, dq[iters]); with gap_every > 0 the
+    relative FP64 gap after every gap_every-th iteration is appended to the
+    returned tuple.
     """
     c = np.ascontiguousarray(ci, np.float32)
     K = c.size
     if not 2 <= K <= max_phases:
         raise ValueError("K must be in [2, 6]")
     dq = np.zeros(max(iters, 1), np.float32) if dq_trace else None
-    o = _opts(storage, kernel, slab_w, mem_available, dq)
+    gap = np.full(max(iters // gap_every, 1), np.nan) if gap_every > 0 else None
+    o = _opts(storage, kernel, slab_w, mem_available, dq, gap, gap_every)
     h = C.c_void_p()
     rc = N.lib().sct_potts_min_partition(ctx.handle, g.handle, K, c.ctypes.data_as(N.f32p), lam, tau1,
                                          tau2, iters, C.byref(o), C.byref(h))
+    extra = ([dq[:iters]] if dq_trace else []) + ([gap[:iters // gap_every]] if gap is not None else [])
     if rc == N.SCT_ERR_NOMEM:
-        return (None, None) if dq_trace else None
+        return (None, *[None] * len(extra)) if extra else None
     ctx.check(rc)
     res = DeviceField(ctx, h, g.mapping)
-    return (res, dq[:iters]) if dq_trace else res
+    return (res, *extra) if extra else res
 
 
 class PottsSolver:
@@ -317,7 +323,7 @@ class PottsSolver:
         self.c = np.ascontiguousarray(ci, np.float32)
         self.K = self.c.size
         self._dq = np.zeros(1, np.float32) if dq_trace else None
-        self._gap = np.zeros(max(gap_capacity, 1), np.float64) if gap_every else None
+        self._gap = np.full(max(gap_capacity, 1), np.nan) if gap_every else None
         self._o = _opts(storage, kernel, None, None, self._dq, self._gap, gap_every)
         h = C.c_void_p()
         ctx.check(N.lib().sct_potts_solver_create(ctx.handle, g.handle, self.K, self.c.ctypes.data_as(N.f32p),
@@ -351,6 +357,19 @@ class PottsSolver:
         self.ctx.check(N.lib().sct_potts_solver_state(self.handle, V.ctypes.data_as(N.f32p),
                                                       xi.ctypes.data_as(N.f32p)))
         return V, xi
+
+    def gap(self) -> tuple[float, float]:
+        """FP64 primal and dual energies (E, D) of the current state."""
+        E, D = C.c_double(), C.c_double()
+        self.ctx.check(N.lib().sct_potts_solver_gap(self.handle, C.byref(E), C.byref(D)))
+        return E.value, D.value
+
+    def gap_trace(self) -> np.ndarray:
+        """Relative gaps (E - D) / E recorded every gap_every iterations so far."""
+        if self._gap is None:
+            return np.zeros(0)
+        k = min(self.iters // self._o.gap_every, self._gap.size)
+        return self._gap[:k].copy()
 
     def dq(self) -> np.ndarray:
         out = np.zeros(self.iters, np.float32)

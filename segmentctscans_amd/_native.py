@@ -33,7 +33,7 @@ class SctPottsOpts(C.Structure):
         ("size", C.c_uint32), ("storage", C.c_int32), ("kernel", C.c_int32),
         ("slab_w", C.c_int32), ("mem_available", C.c_int64),
         ("dq_trace", f32p), ("gap_trace", f64p), ("gap_every", C.c_int32),
-        ("reserved", C.c_int32),
+        ("gap_capacity", C.c_int32),
     ]
 
 
@@ -64,6 +64,7 @@ SIGNATURES = {
                                           C.c_float, C.POINTER(SctPottsOpts), C.POINTER(C.c_void_p)]),
     "sct_potts_solver_iterate": (C.c_int, [C.c_void_p, C.c_int]),
     "sct_potts_solver_kernels": (C.c_int, [C.c_void_p]),
+    "sct_potts_solver_gap": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "sct_selftest_cr_math": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_uint64)]),
     "sct_potts_solver_iterate_timed": (C.c_int, [C.c_void_p, C.c_int, f32p]),

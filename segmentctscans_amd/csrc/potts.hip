@@ -507,6 +507,31 @@ int launch_labels(sct_potts_solver *s, float *res, uint32_t r0, uint32_t r1)
 
 int do_init(sct_potts_solver *s) { SCT_DISPATCH_K(s->K, launch_init, s); }
 int do_iterate(sct_potts_solver *s, int iters, float *dq) { SCT_DISPATCH_K(s->K, launch_iterate, s, iters, dq); }
+
+// do_iterate with the FP64 gap evaluated after every gap_every-th iteration of
+// the slab (count: iterations already done on it) into gap_trace[count / gap_every - 1].
+int iterate_with_gap(sct_potts_solver *s, int iters, float *dq, int &count, int capacity)
+{
+    const int every = s->opts.gap_every;
+    if (every <= 0 || !s->opts.gap_trace) {
+        count += iters;
+        return do_iterate(s, iters, dq);
+    }
+    int done = 0;
+    while (done < iters) {
+        const int chunk = std::min(iters - done, every - count % every);
+        int rc = do_iterate(s, chunk, dq ? dq + done : nullptr);
+        if (rc) return rc;
+        done += chunk;
+        count += chunk;
+        if (count % every == 0 && count / every <= capacity) {
+            double E = 0.0, D = 0.0;
+            if ((rc = potts_gap(s, &E, &D))) return rc;
+            s->opts.gap_trace[count / every - 1] = E != 0.0 ? (E - D) / E : 0.0;
+        }
+    }
+    return SCT_OK;
+}
 int do_labels(sct_potts_solver *s, float *res, uint32_t r0, uint32_t r1) { SCT_DISPATCH_K(s->K, launch_labels, s, res, r0, r1); }
 
 // Bytes of solver state per voxel of a slab: Xi 3(K-1) and V-bar (K-1) in the
@@ -560,6 +585,8 @@ int validate(sct_ctx *ctx, const sct_field *g, int K, const float *c, int iters,
     if (opts && (opts->storage < SCT_STORAGE_F32 || opts->storage > SCT_STORAGE_F16))
         return sct_fail(ctx, SCT_ERR_INVALID, "unknown storage type");
     if (iters < 0) return sct_fail(ctx, SCT_ERR_INVALID, "iters must be >= 0");
+    if (opts && (opts->gap_every < 0 || opts->gap_capacity < 0))
+        return sct_fail(ctx, SCT_ERR_INVALID, "gap_every and gap_capacity must be >= 0");
     return SCT_OK;
 }
 
@@ -695,7 +722,8 @@ extern "C" int sct_potts_solver_iterate(sct_potts_solver *s, int iters)
         }
         dq = s->d_dq + s->it;
     }
-    int rc = do_iterate(s, iters, dq);
+    int count = s->it;
+    int rc = iterate_with_gap(s, iters, dq, count, s->opts.gap_capacity);
     if (!rc) s->it += iters;
     return rc;
 }
@@ -841,7 +869,9 @@ extern "C" int sct_potts_min_partition(sct_ctx *ctx, const sct_field *g, int K, 
             if (e != hipSuccess) rc = sct_hip_fail(ctx, e, "hipMemsetAsync");
         }
         if (!rc) rc = do_init(&s);
-        if (!rc) rc = do_iterate(&s, iters, d_dq);
+        int count = 0;
+        if (!rc) rc = iterate_with_gap(&s, iters, d_dq, count, o.gap_capacity > 0 ? o.gap_capacity
+                                                                                 : iters / std::max(1, o.gap_every));
         const uint32_t r0 = s.s0 == 0 ? s.s0 : s.s0 + uint32_t(plan.halo);
         const uint32_t r1 = int(s.s1) == n ? s.s1 : s.s1 - uint32_t(plan.halo);
         if (!rc) rc = do_labels(&s, (float *)res->data, r0, r1);

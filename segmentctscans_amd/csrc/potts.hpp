@@ -65,6 +65,9 @@ struct sct_potts_solver {
 
 namespace sct {
 
+// FP64 primal and dual energies of the current state (gap.hip).  Synchronises.
+int potts_gap(sct_potts_solver *s, double *primal, double *dual);
+
 inline void potts_mark(sct_potts_solver *s)
 {
     if (s->marks) (void)hipEventRecord(s->marks[s->n_marks++], s->ctx->stream);
