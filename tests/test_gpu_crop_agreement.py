@@ -1,0 +1,50 @@
+"""Agreement with the FP64 oracle on the full-size workload's own data: a 256^3
+box cut from the centre of the 1426 x 1024 x 1024 phantom (cylinder radius
+460, spheres U[4, 40], seed 3), solved separately (box domain, Neumann
+edges) in FP32 and with BF16 / FP16 storage on the GPU and in FP64 on the
+host, K = 3, lambda = 30, tau = 1/sqrt(12), 200 iterations.
+
+Bar: label agreement >= 99.9% for every storage type; ||V - V_fp64||_inf
+< 1e-2 for FP32 (printed for all)."""
+import numpy as np
+import pytest
+
+import oracle as O
+import segmentctscans_amd as S
+from segmentctscans_amd.phantom import crop_box, make_phantom
+
+pytestmark = pytest.mark.gpu
+
+TAU = float(np.float32(1.0 / np.sqrt(12.0)))
+ITERS = 200
+
+
+@pytest.fixture(scope="module")
+def crop():
+    ph = make_phantom(1426, 1024, 4.0, 40.0, seed=3, r=460.0)
+    c = ph.normalized_labels()
+    m = ph.field.mapping
+    size = 256
+    box = crop_box(ph, (m.n_images - size) // 2, (m.height - size) // 2, (m.width - size) // 2, size)
+    lay = O.Layout.from_mapping(box.mapping)
+    r64 = O.potts(lay, box.data, c, 30.0, TAU, TAU, ITERS, precision="f64", want_state=True)
+    return box, c, r64
+
+
+@pytest.mark.parametrize("storage", ["fp32", "bf16", "fp16"])
+def test_crop_256_agreement_vs_fp64(ctx, crop, storage):
+    box, c, r64 = crop
+    g = S.upload(ctx, box)
+    s = S.PottsSolver(ctx, g, c, 30.0, TAU, TAU, storage=storage)
+    s.iterate(ITERS)
+    V, _ = s.state()
+    res = s.labels().to_host()
+    m = box.mapping.valid_mask()
+    agree = (res[m] == r64["res"][m]).mean()
+    vinf = np.abs(V[:, m] - r64["V"][:, m]).max()
+    print(f"256^3 crop, {storage}: label agreement vs FP64 {agree:.6f}, ||V - V_fp64||_inf {vinf:.3e}")
+    assert agree >= 0.999
+    if storage == "fp32":
+        assert vinf < 1e-2
+    s.close()
+    g.close()
