@@ -189,6 +189,25 @@ int sct_uv_hist(sct_ctx *ctx, const sct_field *u, const sct_field *v,
 int sct_uv_hist_auto(sct_ctx *ctx, const sct_field *u, const sct_field *v, uint32_t nbins,
                      uint32_t *counts, uint32_t *stride, float *ranges);
 
+/* ---- batched Potts (no counterpart in the reference, whose driver solves one
+ * volume per call, potts-model.cpp:164) ------------------------------------ */
+typedef struct sct_potts_problem {
+    const sct_field *g;     /* input field (F32); problems may share one      */
+    int32_t K;              /* 2..6                                           */
+    float   c[6];           /* normalised label intensities c[0..K-1]         */
+    float   lambda;
+} sct_potts_problem;
+
+#define SCT_BATCH_GRAPH 1   /* capture all iterations into one hipGraph launch */
+
+/* Solves n problems, iters primal-dual iterations each, with common tau1 and
+ * tau2 and storage (SCT_STORAGE_*): out[p] receives a new field of c[argmax]
+ * labels for problem p, equal to what sct_potts_min_partition returns for it
+ * (one slab).  Problems of one K share each Xi and V launch; *elapsed_ms
+ * (optional) receives the event time of the iterations.  Synchronises. */
+int sct_potts_batch(sct_ctx *ctx, int n, const sct_potts_problem *problems, float tau1, float tau2,
+                    int iters, int storage, int flags, sct_field **out, float *elapsed_ms);
+
 /* ---- diagnostics (no counterpart in the reference) ------------------------ */
 /* Checks on the GPU that the Dykstra projection's range-specific correctly
  * rounded square root (x in [1, 2^64)) and quotient (b in [1, 2^64),

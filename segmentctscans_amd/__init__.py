@@ -314,6 +314,32 @@ def potts_min_partition(ctx: Context, g: DeviceField, ci, lam: float, tau1: floa
     return (res, *extra) if extra else res
 
 
+def potts_batch(ctx: Context, problems, tau1: float, tau2: float, iters: int, *, storage="fp32",
+                graph=True) -> tuple[list, float]:
+    """Many Potts solves in shared launches (no counterpart in the reference).
+
+    problems: sequence of (DeviceField g, labels c, lambda).  Returns the
+    label fields (one per problem, as potts_min_partition would return them)
+    and the event time of the iterations in ms; graph=True runs all
+    iterations as one captured hipGraph launch."""
+    n = len(problems)
+    arr = (N.SctPottsProblem * max(n, 1))()
+    for i, (g, c, lam) in enumerate(problems):
+        c = np.ascontiguousarray(c, np.float32)
+        if not 2 <= c.size <= max_phases:
+            raise ValueError("K must be in [2, 6]")
+        arr[i].g = g.handle
+        arr[i].K = c.size
+        for k in range(c.size):
+            arr[i].c[k] = float(c[k])
+        arr[i].lam = lam
+    outs = (C.c_void_p * max(n, 1))()
+    ms = C.c_float()
+    ctx.check(N.lib().sct_potts_batch(ctx.handle, n, arr, tau1, tau2, iters, N.STORAGE[storage],
+                                      N.SCT_BATCH_GRAPH if graph else 0, outs, C.byref(ms)))
+    return [DeviceField(ctx, C.c_void_p(outs[i]), problems[i][0].mapping) for i in range(n)], ms.value
+
+
 class PottsSolver:
     """Stepping interface over the same kernels (single slab)."""
 

@@ -28,6 +28,13 @@ class SctLayout(C.Structure):
     ]
 
 
+class SctPottsProblem(C.Structure):
+    _fields_ = [("g", C.c_void_p), ("K", C.c_int32), ("c", C.c_float * 6), ("lam", C.c_float)]
+
+
+SCT_BATCH_GRAPH = 1
+
+
 class SctPottsOpts(C.Structure):
     _fields_ = [
         ("size", C.c_uint32), ("storage", C.c_int32), ("kernel", C.c_int32),
@@ -64,6 +71,8 @@ SIGNATURES = {
                                           C.c_float, C.POINTER(SctPottsOpts), C.POINTER(C.c_void_p)]),
     "sct_potts_solver_iterate": (C.c_int, [C.c_void_p, C.c_int]),
     "sct_potts_solver_kernels": (C.c_int, [C.c_void_p]),
+    "sct_potts_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
+                                  C.POINTER(C.c_void_p), C.POINTER(C.c_float)]),
     "sct_potts_solver_gap": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "sct_selftest_cr_math": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_uint64)]),

@@ -9,12 +9,14 @@
 #include "sct_internal.hpp"
 #include "potts_math.hpp"
 #include "potts.hpp"
+#include "twopass.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 
 using namespace sct;
+using namespace sct::tp;
 
 namespace {
 
@@ -35,401 +37,73 @@ float half_to_float(uint16_t h)
     return float(v);
 }
 
-struct WorkGeom {
-    const LineNbr *nbr;
-    uint32_t lines;      // number of lines
-    uint32_t len;        // slab length along n
-    uint32_t chunks;     // ceil(len / 64)
-    uint32_t vchunks;    // ceil(len / (64 VPL)), two-pass kernels
-    uint32_t wstride;    // work line stride
-    uint64_t wspan;      // lines * wstride (component stride)
-    uint32_t in_stride;  // input line stride
-    uint32_t s0;         // slab start in input coordinates
-};
-
-// One wave = 64 consecutive n of one line; 256-thread blocks in XCD runs of 64
-// (about 11 lines of 1426 voxels).
-__device__ __forceinline__ bool wave_voxel(const WorkGeom &g, uint32_t &line, uint32_t &n)
-{
-    const uint32_t b = xcd_block(blockIdx.x, gridDim.x, 64);
-    const uint64_t gw = (uint64_t(b) * blockDim.x + threadIdx.x) / kWave;
-    line = uint32_t(gw / g.chunks);
-    n = uint32_t(gw % g.chunks) * kWave + (threadIdx.x % kWave);
-    return line < g.lines && n < g.len;
-}
-
-template <int K>
-struct Labels {
-    float c[K];
-};
-
-// init_v_kernel, potts-min-partition.cpp:56-94.
-template <int K, typename St>
-__global__ __launch_bounds__(256) void potts_init_kernel(WorkGeom g, const float *__restrict__ u,
-                                                         float *__restrict__ V, typename St::T *__restrict__ vbar,
-                                                         Labels<K> lab)
-{
-    uint32_t line, n;
-    if (!wave_voxel(g, line, n)) return;
-    const float x = u[uint64_t(line) * g.in_stride + g.s0 + n];
-    const int cl = nearest_label<K>(x, lab.c);
-    const uint64_t w = uint64_t(line) * g.wstride + n;
-#pragma unroll
-    for (int i = 0; i < K - 1; ++i) {
-        const float o = (i == cl) ? 1.0f : 0.0f;
-        V[i * g.wspan + w] = o;
-        vbar[i * g.wspan + w] = St::st(o);
-    }
-}
-
-// Per-component base pointers of the two-pass kernels; every component is
-// addressed as base[comp] + 32-bit element offset (line * wstride + n).
-template <int K, typename St>
-struct TwoPassArgs {
-    typename St::T *xi[3 * (K - 1)];
-    typename St::T *vbar[K - 1];
-    float *V[K - 1];
-    const float *u;
-};
-
-// The two-pass kernels give every lane VPL consecutive voxels along n, so that
-// each lane moves 8 bytes (FP32, VPL = 2) or 8 bytes (BF16 / FP16, VPL = 4)
-// per array access: 16-bit loads and stores one element per lane run the
-// memory system at half the rate of 32-bit ones.  A chunk is 64 lanes x VPL
-// voxels of one line.  Lanes past the end of a line keep their vector inside
-// the line's padding (wstride is a multiple of 32 >= len rounded up to VPL),
-// which nothing reads for a valid voxel.
-template <typename St>
-constexpr int vpl_of() { return sizeof(typename St::T) == 4 ? 2 : 4; }
-
-// VPL consecutive stored elements, one load or store.
-template <typename St, int N>
-struct Packed {
-    using T = typename St::T;
-    typedef T V __attribute__((ext_vector_type(N)));
-    static __device__ __forceinline__ void ld(const T *p, float (&o)[N])
-    {
-        const V v = *reinterpret_cast<const V *>(p);
-#pragma unroll
-        for (int j = 0; j < N; ++j) o[j] = St::ld(v[j]);
-    }
-    static __device__ __forceinline__ void st(T *p, const float (&x)[N])
-    {
-        V v;
-#pragma unroll
-        for (int j = 0; j < N; ++j) v[j] = St::st(x[j]);
-        *reinterpret_cast<V *>(p) = v;
-    }
-};
-
-constexpr int kXiVoxelsPerLane = 4;  // per lane and workgroup: kXiVoxelsPerLane / VPL chunks per wave
-constexpr int kXiItems = 512;        // work-list capacity per workgroup
-constexpr int kXiWorkWaves = 4;      // waves that drain the work list
-
-// grad_kernel, potts-min-partition.cpp:96-247: Xi <- Pi_K(Xi + tau2 grad V-bar),
-// in place (each voxel reads its own Xi and its neighbours' V-bar only).
-//   A  every lane forms the Dykstra input of its voxels; most are feasible and
-//      Dykstra stops after one sweep, the rest join a work list in LDS;
-//   B  the work list is drained one sweep per loop trip, each lane taking a new
-//      item as soon as its item converges, the result going back to LDS;
-//   C  every lane stores the Xi+ of its voxels, VPL at a time.
-template <int K, typename St>
-__global__ __launch_bounds__(256) void potts_xi_kernel(WorkGeom g, TwoPassArgs<K, St> a, float tau2,
-                                                       DykstraLimits lim)
-{
-    constexpr int KM = K - 1, NP = K * (K - 1) / 2;
-    constexpr int VPL = vpl_of<St>(), CPW = kXiVoxelsPerLane / VPL;
-    using P = Packed<St, VPL>;
-    __shared__ float qb[kXiItems][K][3];   // inputs, then results, of the queued voxels
-    __shared__ int nq, next;
-    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
-    if (threadIdx.x == 0) {
-        nq = 0;
-        next = 0;
-    }
-    __syncthreads();
-    const uint64_t total = uint64_t(g.lines) * g.vchunks;
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, 16);
-    float res[CPW][VPL][KM][3];
-    int pos[CPW][VPL];
-    uint32_t wofs[CPW];
-    int nv[CPW];   // valid voxels of this lane in chunk k
-#pragma unroll
-    for (int k = 0; k < CPW; ++k) {
-        const uint64_t gw = (uint64_t(blk) * (blockDim.x / kWave) + w) * CPW + k;
-        const uint32_t line = uint32_t(gw / g.vchunks);
-        const uint32_t n0 = uint32_t(gw % g.vchunks) * (kWave * VPL) + lane * VPL;
-        nv[k] = (gw < total && n0 < g.len) ? int(min(uint32_t(VPL), g.len - n0)) : 0;
-        wofs[k] = nv[k] ? line * g.wstride + n0 : 0;
-#pragma unroll
-        for (int j = 0; j < VPL; ++j) pos[k][j] = -1;
-        if (gw >= total) continue;   // uniform over the wave
-        const LineNbr nb = g.nbr[line];
-        const uint32_t w0 = wofs[k];
-        float vb[KM][VPL], vr[KM][VPL], vc[KM][VPL], x[KM][3][VPL], vn[KM][VPL];
-#pragma unroll
-        for (int i = 0; i < KM; ++i) {
-#pragma unroll
-            for (int j = 0; j < VPL; ++j) vb[i][j] = vr[i][j] = vc[i][j] = 0.0f;
-            if (nv[k]) {
-                P::ld(a.vbar[i] + w0, vb[i]);
-                if (nb.nrow >= 0) P::ld(a.vbar[i] + uint32_t(nb.nrow) * g.wstride + n0, vr[i]);
-                if (nb.ncol >= 0) P::ld(a.vbar[i] + uint32_t(nb.ncol) * g.wstride + n0, vc[i]);
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) P::ld(a.xi[3 * i + ax] + w0, x[i][ax]);
-            }
-            // V-bar at n + 1: the next element of the lane, the next lane's first,
-            // or (last lane) the first of the next chunk
-#pragma unroll
-            for (int j = 0; j + 1 < VPL; ++j) vn[i][j] = vb[i][j + 1];
-            vn[i][VPL - 1] = __shfl_down(vb[i][0], 1, kWave);
-            if (lane == kWave - 1 && nv[k] == VPL && n0 + VPL < g.len) vn[i][VPL - 1] = St::ld(a.vbar[i][w0 + VPL]);
-        }
-#pragma unroll
-        for (int j = 0; j < VPL; ++j) {
-            const bool valid = j < nv[k];
-            const bool nx[3] = {n0 + j + 1 < g.len, nb.nrow >= 0, nb.ncol >= 0};
-            float vbj[KM], vbn[3][KM], xj[KM][3];
-#pragma unroll
-            for (int i = 0; i < KM; ++i) {
-                vbj[i] = vb[i][j];
-                vbn[0][i] = vn[i][j];
-                vbn[1][i] = vr[i][j];
-                vbn[2][i] = vc[i][j];
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) xj[i][ax] = x[i][ax][j];
-            }
-            float q[K][3];
-            xi_input<K>(vbj, vbn, nx, xj, tau2, q);
-            const bool queued = valid && !dykstra_first_sweep_done<K>(q, lim);
-            const uint64_t mask = __ballot(queued);
-            int p = 0;
-            if (mask) {
-                const int leader = __ffsll((unsigned long long)mask) - 1;
-                if (lane == leader) p = atomicAdd(&nq, __popcll(mask));
-                p = __shfl(p, leader, kWave) + __popcll(mask & ((1ull << lane) - 1ull));
-            }
-            if (queued && p < kXiItems) {
-#pragma unroll
-                for (int i = 0; i < K; ++i)
-#pragma unroll
-                    for (int ax = 0; ax < 3; ++ax) qb[p][i][ax] = q[i][ax];
-                pos[k][j] = p;
-            } else if (queued) {
-                dykstra<K>(q, lim);   // work list full: solve in place
-            }
-#pragma unroll
-            for (int i = 0; i < KM; ++i)
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) res[k][j][i][ax] = valid ? q[i][ax] : 0.0f;
-        }
-    }
-    __syncthreads();   // work list complete
-    const int nwl = min(nq, kXiItems);
-    if (w < kXiWorkWaves && w * kWave < nwl) {
-        float q[K][3], r[NP][3];
-        int sw = 0, t = 0;
-        bool have = false, drained = false;
-        while (true) {
-            const bool need = !have && !drained;
-            const uint64_t mask = __ballot(need);
-            if (mask) {
-                const int leader = __ffsll((unsigned long long)mask) - 1;
-                int base = 0;
-                if (lane == leader) base = atomicAdd(&next, __popcll(mask));
-                base = __shfl(base, leader, kWave);
-                if (need) {
-                    t = base + __popcll(mask & ((1ull << lane) - 1ull));
-                    if (t < nwl) {
-#pragma unroll
-                        for (int i = 0; i < K; ++i)
-#pragma unroll
-                            for (int ax = 0; ax < 3; ++ax) q[i][ax] = qb[t][i][ax];
-#pragma unroll
-                        for (int pp = 0; pp < NP; ++pp) r[pp][0] = r[pp][1] = r[pp][2] = 0.0f;
-                        sw = 0;
-                        have = true;
-                    } else {
-                        drained = true;
-                    }
-                }
-            }
-            if (!__any(have)) break;
-            if (have) {
-                const float m2 = dykstra_sweep<K>(q, r, lim);
-                ++sw;
-                if (m2 < lim.stol || sw == 20) {
-#pragma unroll
-                    for (int i = 0; i < KM; ++i)
-#pragma unroll
-                        for (int ax = 0; ax < 3; ++ax) qb[t][i][ax] = q[i][ax];
-                    have = false;
-                }
-            }
-        }
-    }
-    __syncthreads();   // results of the work list in LDS
-#pragma unroll
-    for (int k = 0; k < CPW; ++k) {
-        if (!nv[k]) continue;
-#pragma unroll
-        for (int i = 0; i < KM; ++i)
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                float o[VPL];
-#pragma unroll
-                for (int j = 0; j < VPL; ++j) o[j] = pos[k][j] >= 0 ? qb[pos[k][j]][i][ax] : res[k][j][i][ax];
-                P::st(a.xi[3 * i + ax] + wofs[k], o);
-            }
-    }
-}
-
-// div_kernel, potts-min-partition.cpp:349-488: V <- Pi_S(V + tau1 (div Xi - G)),
-// V-bar <- 2 V+ - V.  With TRACE, the block maximum of sum_k dV_k^2 goes to *dq.
-// One chunk per wave; 256-thread blocks in XCD runs of 64 / VPL (about 11 lines).
-template <int K, typename St, bool TRACE>
-__global__ __launch_bounds__(256) void potts_v_kernel(WorkGeom g, TwoPassArgs<K, St> a, Labels<K> lab,
-                                                      float lambda, float tau1, float *dq)
-{
-    constexpr int KM = K - 1, VPL = vpl_of<St>();
-    using P = Packed<St, VPL>;
-    using PF = Packed<StorageF32, VPL>;
-    const int lane = threadIdx.x % kWave;
-    const uint32_t b = xcd_block(blockIdx.x, gridDim.x, 64 / VPL);
-    const uint64_t gw = (uint64_t(b) * blockDim.x + threadIdx.x) / kWave;
-    float d = 0.0f;
-    if (gw < uint64_t(g.lines) * g.vchunks) {   // uniform over the wave
-        const uint32_t line = uint32_t(gw / g.vchunks);
-        const uint32_t n0 = uint32_t(gw % g.vchunks) * (kWave * VPL) + lane * VPL;
-        const int nv = n0 < g.len ? int(min(uint32_t(VPL), g.len - n0)) : 0;
-        const LineNbr nb = g.nbr[line];
-        const uint32_t w0 = line * g.wstride + n0;
-        float xo[KM][3][VPL], xr[KM][VPL], xc[KM][VPL], xn[KM][VPL], v[KM][VPL], u[VPL];
-#pragma unroll
-        for (int i = 0; i < KM; ++i) {
-#pragma unroll
-            for (int j = 0; j < VPL; ++j) {
-                xr[i][j] = xc[i][j] = v[i][j] = 0.0f;
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) xo[i][ax][j] = 0.0f;
-            }
-            if (nv) {
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) P::ld(a.xi[3 * i + ax] + w0, xo[i][ax]);
-                if (nb.prow >= 0) P::ld(a.xi[3 * i + 1] + uint32_t(nb.prow) * g.wstride + n0, xr[i]);
-                if (nb.pcol >= 0) P::ld(a.xi[3 * i + 2] + uint32_t(nb.pcol) * g.wstride + n0, xc[i]);
-                PF::ld(a.V[i] + w0, v[i]);
-            }
-            // Xi_n at n - 1: the previous element of the lane, the previous
-            // lane's last, or (first lane) the last of the previous chunk
-#pragma unroll
-            for (int j = 1; j < VPL; ++j) xn[i][j] = xo[i][0][j - 1];
-            xn[i][0] = __shfl_up(xo[i][0][VPL - 1], 1, kWave);
-            if (lane == 0 && nv && n0 > 0) xn[i][0] = St::ld(a.xi[3 * i][w0 - 1]);
-        }
-#pragma unroll
-        for (int j = 0; j < VPL; ++j) u[j] = j < nv ? a.u[line * g.in_stride + g.s0 + n0 + j] : 0.0f;
-        float vb[KM][VPL];
-#pragma unroll
-        for (int j = 0; j < VPL; ++j) {
-            const bool px[3] = {n0 + j > 0, nb.prow >= 0, nb.pcol >= 0};
-            const bool nx[3] = {n0 + j + 1 < g.len, nb.nrow >= 0, nb.ncol >= 0};
-            float xoj[KM][3], xpj[3][KM], vj[KM], vbj[KM];
-#pragma unroll
-            for (int i = 0; i < KM; ++i) {
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) xoj[i][ax] = xo[i][ax][j];
-                xpj[0][i] = xn[i][j];
-                xpj[1][i] = xr[i][j];
-                xpj[2][i] = xc[i][j];
-                vj[i] = v[i][j];
-            }
-            const float dj = v_update<K>(xoj, xpj, px, nx, vj, vbj, u[j], lab.c, lambda, tau1);
-            if (j < nv) d = fmaxf(d, dj);
-#pragma unroll
-            for (int i = 0; i < KM; ++i) {
-                v[i][j] = vj[i];
-                vb[i][j] = vbj[i];
-            }
-        }
-        if (nv) {
-#pragma unroll
-            for (int i = 0; i < KM; ++i) {
-                PF::st(a.V[i] + w0, v[i]);
-                P::st(a.vbar[i] + w0, vb[i]);
-            }
-        }
-    }
-    if constexpr (TRACE) {
-        __shared__ float red[4];
-        d = wave_max(d);
-        if ((threadIdx.x % kWave) == 0) red[threadIdx.x / kWave] = d;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            float m = red[0];
-            for (int i = 1; i < int(blockDim.x / kWave); ++i) m = fmaxf(m, red[i]);
-            atomic_max_nonneg(dq, m);
-        }
-    }
-}
-
-// update_kernel, potts-min-partition.cpp:491-538: res = c[argmax V] on [r0, r1).
-template <int K>
-__global__ __launch_bounds__(256) void potts_labels_kernel(WorkGeom g, const float *__restrict__ V,
-                                                           float *__restrict__ res, Labels<K> lab,
-                                                           uint32_t r0, uint32_t r1)
-{
-    uint32_t line, n;
-    if (!wave_voxel(g, line, n)) return;
-    const uint32_t na = g.s0 + n;
-    if (na < r0 || na >= r1) return;
-    const uint64_t w = uint64_t(line) * g.wstride + n;
-    float v[K - 1];
-#pragma unroll
-    for (int i = 0; i < K - 1; ++i) v[i] = V[i * g.wspan + w];
-    res[uint64_t(line) * g.in_stride + na] = lab.c[argmax_label<K>(v)];
-}
-
-uint32_t grid_for(const WorkGeom &g)
-{
-    const uint64_t waves = uint64_t(g.lines) * g.chunks;
-    return uint32_t((waves + 3) / 4);
-}
-
 }  // namespace
 
 namespace {
 
-WorkGeom make_geom(const sct_potts_solver *s)
+// The lines of the solver's slab, for the two-pass kernels with storage St.
+template <typename St>
+VolumeLines volume_lines(const sct_potts_solver *s)
 {
     const Geometry &geo = s->g->geo;
-    WorkGeom g;
-    g.nbr = geo.d_nbr;
-    g.lines = geo.lines;
-    g.len = s->s1 - s->s0;
-    g.chunks = (g.len + kWave - 1) / kWave;
-    g.wstride = s->wstride;
-    g.wspan = s->wspan;
-    g.in_stride = geo.nstride;
-    g.s0 = s->s0;
-    return g;
+    constexpr uint32_t cw = kWave * vpl_of<St>();
+    VolumeLines L;
+    L.nbr = geo.d_nbr;
+    L.lines = geo.lines;
+    L.len = s->s1 - s->s0;
+    L.vchunks = (L.len + cw - 1) / cw;
+    L.wstride = s->wstride;
+    L.in_stride = geo.nstride;
+    L.s0 = s->s0;
+    return L;
+}
+
+// Base pointers of the state of Xi / V-bar buffer set b.
+template <int K, typename St>
+TwoPassArgs<K, St> twopass_args(const sct_potts_solver *s, int b)
+{
+    using T = typename St::T;
+    TwoPassArgs<K, St> a;
+    for (int j = 0; j < 3 * (K - 1); ++j) a.xi[j] = (T *)s->xi[b] + j * s->wspan;
+    for (int i = 0; i < K - 1; ++i) {
+        a.vbar[i] = (T *)s->vbar[b] + i * s->wspan;
+        a.V[i] = s->V + i * s->wspan;
+    }
+    a.u = (const float *)s->g->data;
+    return a;
+}
+
+template <int K>
+OneSolve<K> one_solve(const sct_potts_solver *s)
+{
+    OneSolve<K> o;
+    for (int i = 0; i < K; ++i) o.p.c[i] = s->c[i];
+    o.p.lambda = s->lambda;
+    return o;
+}
+
+bool slab_empty(const sct_potts_solver *s) { return !s->g->geo.lines || s->s1 <= s->s0; }
+
+template <int K, typename St>
+int init_st(sct_potts_solver *s)
+{
+    const VolumeLines L = volume_lines<St>(s);
+    potts_init_kernel<K, St><<<wave_grid(L.count()), 256, 0, s->ctx->stream>>>(L, twopass_args<K, St>(s, 0), one_solve<K>(s));
+    SCT_HIP(s->ctx, hipGetLastError());
+    return SCT_OK;
 }
 
 template <int K>
 int launch_init(sct_potts_solver *s)
 {
-    const WorkGeom g = make_geom(s);
-    Labels<K> lab;
-    for (int i = 0; i < K; ++i) lab.c[i] = s->c[i];
-    if (g.lines && g.len) {
-        const float *u = (const float *)s->g->data;
-        if (s->storage == SCT_STORAGE_BF16)
-            potts_init_kernel<K, StorageBF16><<<grid_for(g), 256, 0, s->ctx->stream>>>(g, u, s->V, (uint16_t *)s->vbar[0], lab);
-        else if (s->storage == SCT_STORAGE_F16)
-            potts_init_kernel<K, StorageF16><<<grid_for(g), 256, 0, s->ctx->stream>>>(g, u, s->V, (_Float16 *)s->vbar[0], lab);
-        else
-            potts_init_kernel<K, StorageF32><<<grid_for(g), 256, 0, s->ctx->stream>>>(g, u, s->V, (float *)s->vbar[0], lab);
-        SCT_HIP(s->ctx, hipGetLastError());
+    if (!slab_empty(s)) {
+        int rc;
+        switch (s->storage) {
+        case SCT_STORAGE_BF16: rc = init_st<K, StorageBF16>(s); break;
+        case SCT_STORAGE_F16: rc = init_st<K, StorageF16>(s); break;
+        default: rc = init_st<K, StorageF32>(s); break;
+        }
+        if (rc) return rc;
     }
     s->cur = 0;
     SCT_HIP(s->ctx, hipMemsetAsync(s->xi[0], 0, storage_bytes(s->storage) * 3 * (K - 1) * s->wspan, s->ctx->stream));
@@ -437,34 +111,21 @@ int launch_init(sct_potts_solver *s)
 }
 
 template <int K, typename St>
-int twopass_iterate(sct_potts_solver *s, const WorkGeom &g, int iters, float *dq_dev)
+int twopass_iterate(sct_potts_solver *s, int iters, float *dq_dev)
 {
-    using T = typename St::T;
-    Labels<K> lab;
-    for (int i = 0; i < K; ++i) lab.c[i] = s->c[i];
     const DykstraLimits lim = dykstra_limits();
-    TwoPassArgs<K, St> a;
-    for (int j = 0; j < 3 * (K - 1); ++j) a.xi[j] = (T *)s->xi[0] + j * s->wspan;
-    for (int i = 0; i < K - 1; ++i) {
-        a.vbar[i] = (T *)s->vbar[0] + i * s->wspan;
-        a.V[i] = s->V + i * s->wspan;
-    }
-    a.u = (const float *)s->g->data;
-    constexpr int VPL = vpl_of<St>();
-    WorkGeom gv = g;
-    gv.vchunks = (g.len + kWave * VPL - 1) / (kWave * VPL);
-    const uint64_t waves = uint64_t(g.lines) * gv.vchunks;
-    const uint64_t xi_waves = (waves + kXiVoxelsPerLane / VPL - 1) / (kXiVoxelsPerLane / VPL);
-    const uint32_t xi_grid = uint32_t((xi_waves + 3) / 4);
-    const uint32_t v_grid = uint32_t((waves + 3) / 4);
+    const TwoPassArgs<K, St> a = twopass_args<K, St>(s, 0);
+    const VolumeLines L = volume_lines<St>(s);
+    const OneSolve<K> prm = one_solve<K>(s);
+    const uint32_t xg = xi_grid<St>(L.count()), vg = wave_grid(L.count());
     for (int i = 0; i < iters; ++i) {
         potts_mark(s);
-        potts_xi_kernel<K, St><<<xi_grid, 256, 0, s->ctx->stream>>>(gv, a, s->tau2, lim);
+        potts_xi_kernel<K, St><<<xg, 256, 0, s->ctx->stream>>>(L, a, s->tau2, lim);
         potts_mark(s);
         if (dq_dev)
-            potts_v_kernel<K, St, true><<<v_grid, 256, 0, s->ctx->stream>>>(gv, a, lab, s->lambda, s->tau1, dq_dev + i);
+            potts_v_kernel<K, St, true><<<vg, 256, 0, s->ctx->stream>>>(L, a, prm, s->tau1, dq_dev + i);
         else
-            potts_v_kernel<K, St, false><<<v_grid, 256, 0, s->ctx->stream>>>(gv, a, lab, s->lambda, s->tau1, nullptr);
+            potts_v_kernel<K, St, false><<<vg, 256, 0, s->ctx->stream>>>(L, a, prm, s->tau1, nullptr);
     }
     SCT_HIP(s->ctx, hipGetLastError());
     return SCT_OK;
@@ -473,24 +134,23 @@ int twopass_iterate(sct_potts_solver *s, const WorkGeom &g, int iters, float *dq
 template <int K>
 int launch_iterate(sct_potts_solver *s, int iters, float *dq_dev)
 {
-    const WorkGeom g = make_geom(s);
-    if (!g.lines || !g.len) return SCT_OK;
+    if (slab_empty(s)) return SCT_OK;
     if (s->kernel == SCT_KERNEL_FUSED) return fused_iterate(s, iters, dq_dev);
     switch (s->storage) {
-    case SCT_STORAGE_BF16: return twopass_iterate<K, StorageBF16>(s, g, iters, dq_dev);
-    case SCT_STORAGE_F16: return twopass_iterate<K, StorageF16>(s, g, iters, dq_dev);
-    default: return twopass_iterate<K, StorageF32>(s, g, iters, dq_dev);
+    case SCT_STORAGE_BF16: return twopass_iterate<K, StorageBF16>(s, iters, dq_dev);
+    case SCT_STORAGE_F16: return twopass_iterate<K, StorageF16>(s, iters, dq_dev);
+    default: return twopass_iterate<K, StorageF32>(s, iters, dq_dev);
     }
 }
 
 template <int K>
 int launch_labels(sct_potts_solver *s, float *res, uint32_t r0, uint32_t r1)
 {
-    const WorkGeom g = make_geom(s);
-    if (!g.lines || !g.len || r0 >= r1) return SCT_OK;
-    Labels<K> lab;
-    for (int i = 0; i < K; ++i) lab.c[i] = s->c[i];
-    potts_labels_kernel<K><<<grid_for(g), 256, 0, s->ctx->stream>>>(g, s->V, res, lab, r0, r1);
+    if (slab_empty(s) || r0 >= r1) return SCT_OK;
+    // V is FP32 in every storage mode; the FP32 chunking reads it.
+    const VolumeLines L = volume_lines<StorageF32>(s);
+    potts_labels_kernel<K, StorageF32><<<wave_grid(L.count()), 256, 0, s->ctx->stream>>>(
+        L, twopass_args<K, StorageF32>(s, 0), one_solve<K>(s), res, r0, r1);
     SCT_HIP(s->ctx, hipGetLastError());
     return SCT_OK;
 }

@@ -1,0 +1,63 @@
+"""Batched Potts solves (no counterpart in the reference): every problem of a
+batch gets exactly the labels its own potts_min_partition call returns --
+mixed shapes (cylinders and boxes), K = 2..5, two lambdas per volume, FP32 and
+BF16 storage, iterations launched directly or as one captured hipGraph."""
+import numpy as np
+import pytest
+
+import segmentctscans_amd as S
+from segmentctscans_amd.phantom import make_phantom
+
+pytestmark = pytest.mark.gpu
+
+TAU = float(np.float32(1.0 / np.sqrt(12.0)))
+SHAPES = [  # n, height, width, box?, K
+    (20, 18, 22, False, 3), (33, 25, 25, True, 4), (48, 30, 20, False, 5),
+    (17, 40, 31, True, 3), (26, 26, 26, False, 2), (70, 12, 9, True, 4)]
+
+
+@pytest.fixture(scope="module")
+def volumes(ctx):
+    out = []
+    for i, (n, h, w, box, K) in enumerate(SHAPES):
+        r = 4.0 * max(h, w) if box else 0.45 * min(h, w)
+        ph = make_phantom(n, max(h, w), 2.0, 5.0, seed=70 + i, height=h, width=w,
+                          xc=(w - 1) / 2, yc=(h - 1) / 2, r=r)
+        c = np.linspace(0.2, 0.7, K).astype(np.float32)
+        out.append((ph, S.upload(ctx, ph.field), c))
+    return out
+
+
+def _problems(volumes):
+    return [(g, c, lam) for (_, g, c) in volumes for lam in (20.0, 45.0)]
+
+
+@pytest.mark.parametrize("storage", ["fp32", "bf16"])
+def test_batch_equals_single_solves(ctx, volumes, storage):
+    probs = _problems(volumes)
+    outs, ms = S.potts_batch(ctx, probs, TAU, TAU, 30, storage=storage, graph=True)
+    assert len(outs) == len(probs) and ms > 0
+    for (g, c, lam), o in zip(probs, outs):
+        want = S.potts_min_partition(ctx, g, c, lam, TAU, TAU, 30, storage=storage).to_host()
+        m = g.mapping.valid_mask()
+        assert np.array_equal(o.to_host()[m], want[m]), (g.mapping.n_images, c.size, lam)
+        o.close()
+
+
+def test_batch_graph_equals_direct_launches(ctx, volumes):
+    probs = _problems(volumes)
+    a, _ = S.potts_batch(ctx, probs, TAU, TAU, 12, graph=True)
+    b, _ = S.potts_batch(ctx, probs, TAU, TAU, 12, graph=False)
+    for x, y, (g, _, _) in zip(a, b, probs):
+        m = g.mapping.valid_mask()
+        assert np.array_equal(x.to_host()[m], y.to_host()[m])
+        x.close()
+        y.close()
+
+
+def test_batch_rejects_bad_problems(ctx, volumes):
+    g = volumes[0][1]
+    with pytest.raises(ValueError):
+        S.potts_batch(ctx, [(g, [0.5], 30.0)], TAU, TAU, 5)
+    outs, _ = S.potts_batch(ctx, [], TAU, TAU, 5)
+    assert outs == []
