@@ -55,6 +55,13 @@ def lib() -> C.CDLL:
         L.or_support_irls_d.argtypes = [C.c_int, f64p]
         L.or_support_irls_d.restype = C.c_double
         L.or_potts_gap_d.argtypes = [C.POINTER(OrLayout), f32p, C.c_int, f32p, C.c_float, f64p, f64p, f64p, f64p]
+        i8p = C.POINTER(C.c_int8)
+        L.or_adj_loss_d.argtypes = [C.POINTER(OrLayout), f64p, f32p, C.c_int, f64p, C.c_double, C.c_double,
+                                    C.c_double, C.c_int, i8p, C.c_double]
+        L.or_adj_loss_d.restype = C.c_double
+        L.or_adj_grad_d.argtypes = [C.POINTER(OrLayout), f64p, f32p, C.c_int, f64p, C.c_double, C.c_double,
+                                    C.c_double, C.c_int, i8p, C.c_double, C.c_int, f64p, f64p, f64p]
+        L.or_adj_grad_d.restype = C.c_double
         L.or_bf16_round.argtypes = [C.c_float]
         L.or_bf16_round.restype = C.c_float
         L.or_f16_round.argtypes = [C.c_float]
@@ -155,6 +162,42 @@ def potts_gap(layout: Layout, u, c, lam, V, xi) -> tuple[float, float]:
                               _ptr(xi, f64p), C.byref(E), C.byref(D))
     assert rc == 0
     return E.value, D.value
+
+
+def nearest_labels(u, c) -> np.ndarray:
+    """Start labels of the iteration: index of the nearest c (lowest index on ties)."""
+    u = np.asarray(u, np.float64)
+    c = np.asarray(c, np.float64)
+    return np.argmin(np.abs(c[None, :] - u[:, None]), axis=1).astype(np.int8)
+
+
+def _adj_args(layout, u, target, c, init):
+    u = np.ascontiguousarray(u, np.float64)
+    target = np.ascontiguousarray(target, np.float32)
+    c = np.ascontiguousarray(c, np.float64)
+    init = None if init is None else np.ascontiguousarray(init, np.int8)
+    return u, target, c, init
+
+
+def adj_loss(layout: Layout, u, target, c, lam, tau1, tau2, iters, *, init=None, tol=1e-6) -> float:
+    """FP64 loss 1/(N K) sum (V - onehot(target))^2 after iters iterations."""
+    u, target, c, init = _adj_args(layout, u, target, c, init)
+    lay = layout.c()
+    return lib().or_adj_loss_d(C.byref(lay), _ptr(u, f64p), _ptr(target, f32p), c.size, _ptr(c, f64p), lam, tau1,
+                               tau2, iters, None if init is None else init.ctypes.data_as(C.POINTER(C.c_int8)), tol)
+
+
+def adj_grad(layout: Layout, u, target, c, lam, tau1, tau2, iters, *, init=None, tol=1e-6, ck=16):
+    """FP64 adjoint: (loss, dL/dc[K], dL/dlambda, dL/du[span])."""
+    u, target, c, init = _adj_args(layout, u, target, c, init)
+    gc = np.zeros(c.size)
+    gl = C.c_double()
+    gu = np.zeros(layout.span)
+    lay = layout.c()
+    loss = lib().or_adj_grad_d(C.byref(lay), _ptr(u, f64p), _ptr(target, f32p), c.size, _ptr(c, f64p), lam, tau1,
+                               tau2, iters, None if init is None else init.ctypes.data_as(C.POINTER(C.c_int8)), tol,
+                               ck, _ptr(gc, f64p), C.byref(gl), _ptr(gu, f64p))
+    return loss, gc, gl.value, gu
 
 
 def store_round(x, storage: str) -> np.ndarray:

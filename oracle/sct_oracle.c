@@ -461,3 +461,372 @@ int or_potts_gap_d(const or_layout *m, const float *u, int K, const float *c, fl
     *dual = D;
     return 0;
 }
+
+/* ---- FP64 adjoint of the unrolled Potts iteration ---------------------------
+ * Not in the reference.  Loss after `iters` iterations from the start state
+ * (V = V-bar = one-hot of init, Xi = 0):
+ *   L = 1/(N K) sum_x sum_{i<K} (V_i(x) - Y_i(x))^2,  Y one-hot of target,
+ * N in-mask voxels, V_{K-1} = 1 - sum of the others.  Gradients with respect
+ * to c, lambda and u flow through the data term G_i = lambda (u - c_i)^2; the
+ * nearest-label start is piecewise constant (gradient 0 almost everywhere).
+ * Backward through one iteration (s_t -> s_{t+1}):
+ *   V+ = Pi_S(q), q = V + tau1 (div Xi+ - G): Jacobian of Pi_S = diag(a) - a a^T/|a|
+ *        on the active set a (q_i > theta);
+ *   Xi+ = Pi_K(x), x = Xi + tau2 grad V-bar: Dykstra replayed with the same
+ *        sweeps, each pair step back-propagated from its h (excess map
+ *        f(h) = h (1 - 1/|h|) for |h| > 1, Jacobian (1 - 1/|h|) I + h h^T/|h|^3);
+ *   div and grad are each other's negative adjoints on the masked domain.
+ * States are checkpointed every ck iterations and each segment recomputed. */
+typedef struct {
+    const or_layout *m;
+    int K;
+    double c[8], lambda, tau1, tau2, tol;
+    const double *u;
+    size_t span;
+} adj_t;
+
+static void adj_nbrs(const or_layout *m, uint32_t n, uint32_t row, uint32_t col, size_t o, int nx[3], int px[3],
+                     size_t on[3], size_t op[3])
+{
+    nx[0] = n + 1 < m->n_images; nx[1] = has_next_row(m, row, col); nx[2] = has_next_col(m, row, col);
+    px[0] = n > 0; px[1] = has_prev_row(m, row, col); px[2] = has_prev_col(m, row, col);
+    on[0] = o + 1; on[1] = o + m->nstride; on[2] = nx[2] ? line_of(m, row, col + 1) * m->nstride + n : 0;
+    op[0] = o - 1; op[1] = o - m->nstride; op[2] = px[2] ? line_of(m, row, col - 1) * m->nstride + n : 0;
+}
+
+/* Dykstra with the oracle's semantics (20 sweeps, stop when the largest
+ * correction norm of a sweep is below tol), recording h of every pair step. */
+static int adj_dykstra(int K, double q[8][3], double tol, double hist[20][15][3])
+{
+    double r[15][3] = {{0}};
+    int sweep;
+    for (sweep = 0; sweep < 20; ++sweep) {
+        double mx = -DBL_MAX;
+        for (int i1 = 0, p = 0; i1 < K - 1; ++i1)
+            for (int i2 = i1 + 1; i2 < K; ++i2, ++p) {
+                double h[3], e[3], d[3];
+                for (int a = 0; a < 3; ++a) hist[sweep][p][a] = h[a] = (q[i2][a] - q[i1][a]) + r[p][a];
+                double l = sqrt(fma(h[2], h[2], fma(h[1], h[1], h[0] * h[0])));
+                if (l > 1.0) { double s = (l - 1.0) / l; for (int a = 0; a < 3; ++a) e[a] = s * h[a]; }
+                else e[0] = e[1] = e[2] = 0.0;
+                for (int a = 0; a < 3; ++a) d[a] = 0.5 * (e[a] - r[p][a]);
+                double dn = sqrt(fma(d[2], d[2], fma(d[1], d[1], d[0] * d[0])));
+                mx = (dn < mx) ? mx : dn;
+                for (int a = 0; a < 3; ++a) { q[i1][a] += d[a]; q[i2][a] -= d[a]; r[p][a] = e[a]; }
+            }
+        if (tol > mx) { ++sweep; break; }
+    }
+    return sweep;
+}
+
+/* g: adjoint of the output q on entry, of the input q on return. */
+static void adj_dykstra_back(int K, int sweeps, double hist[20][15][3], double g[8][3])
+{
+    double gr[15][3] = {{0}};
+    for (int s = sweeps - 1; s >= 0; --s)
+        for (int i1 = K - 2, p = K * (K - 1) / 2 - 1; i1 >= 0; --i1)
+            for (int i2 = K - 1; i2 > i1; --i2, --p) {
+                const double *h = hist[s][p];
+                double gd[3], ge[3], gh[3] = {0, 0, 0};
+                for (int a = 0; a < 3; ++a) { gd[a] = g[i1][a] - g[i2][a]; ge[a] = 0.5 * gd[a] + gr[p][a]; }
+                double l = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+                if (l > 1.0) {
+                    double hg = h[0] * ge[0] + h[1] * ge[1] + h[2] * ge[2], il = 1.0 / l;
+                    for (int a = 0; a < 3; ++a) gh[a] = (1.0 - il) * ge[a] + hg * il * il * il * h[a];
+                }
+                for (int a = 0; a < 3; ++a) {
+                    g[i1][a] -= gh[a];
+                    g[i2][a] += gh[a];
+                    gr[p][a] = -0.5 * gd[a] + gh[a];
+                }
+            }
+}
+
+/* Pi_S of q in place; act[i] = 1 on the active set. */
+static void adj_simplex(int K, double *q, int *act)
+{
+    double u[8];
+    for (int i = 0; i < K; ++i) u[i] = q[i];
+    sort_desc_d(K, u);
+    double sum = 0, rho_sum = 0;
+    int rho = 0;
+    for (int i = 0; i < K; ++i) {
+        sum += u[i];
+        if (u[i] > (sum - 1.0) / (double)(i + 1)) { rho = i + 1; rho_sum = sum; }
+    }
+    double theta = (rho_sum - 1.0) / (double)rho;
+    for (int i = 0; i < K; ++i) {
+        double a = q[i] - theta;
+        act[i] = a > 0.0;
+        q[i] = a < 0.0 ? 0.0 : a;
+    }
+}
+
+static void adj_x(const adj_t *A, uint32_t n, uint32_t row, uint32_t col, const double *Vb, const double *Xi,
+                  double x[8][3])
+{
+    const or_layout *m = A->m;
+    const int K = A->K;
+    size_t o = line_of(m, row, col) * m->nstride + n, on[3], op[3];
+    int nx[3], px[3];
+    adj_nbrs(m, n, row, col, o, nx, px, on, op);
+    double vb[8], vk = 1.0;
+    for (int i = 0; i < K - 1; ++i) { vb[i] = Vb[i * A->span + o]; vk -= vb[i]; }
+    vb[K - 1] = vk;
+    for (int a = 0; a < 3; ++a) {
+        double p[8], xk = 0.0;
+        if (nx[a]) {
+            double vka = 1.0;
+            for (int i = 0; i < K - 1; ++i) { double t = Vb[i * A->span + on[a]]; p[i] = t; vka -= t; }
+            p[K - 1] = vka;
+            for (int i = 0; i < K; ++i) p[i] -= vb[i];
+        } else {
+            for (int i = 0; i < K; ++i) p[i] = 0.0;
+        }
+        for (int i = 0; i < K - 1; ++i) { double xo = Xi[(3 * i + a) * A->span + o]; xk -= xo; x[i][a] = fma(p[i], A->tau2, xo); }
+        x[K - 1][a] = fma(p[K - 1], A->tau2, xk);
+    }
+}
+
+/* q of the V step at a voxel (K components), from V and Xi+. */
+static void adj_qv(const adj_t *A, uint32_t n, uint32_t row, uint32_t col, const double *V, const double *Xi1,
+                   double *q, double *d_out)
+{
+    const or_layout *m = A->m;
+    const int K = A->K;
+    size_t o = line_of(m, row, col) * m->nstride + n, on[3], op[3];
+    int nx[3], px[3];
+    adj_nbrs(m, n, row, col, o, nx, px, on, op);
+    double xo[8][3], xp[8][3], v[8], vk = 1.0;
+    for (int a = 0; a < 3; ++a) {
+        double xk = 0.0, xkp = 0.0;
+        for (int i = 0; i < K - 1; ++i) {
+            xo[i][a] = Xi1[(3 * i + a) * A->span + o]; xk -= xo[i][a];
+            xp[i][a] = px[a] ? Xi1[(3 * i + a) * A->span + op[a]] : 0.0; xkp -= xp[i][a];
+        }
+        xo[K - 1][a] = xk; xp[K - 1][a] = xkp;
+    }
+    for (int i = 0; i < K - 1; ++i) { v[i] = V[i * A->span + o]; vk -= v[i]; }
+    v[K - 1] = vk;
+    const double uu = A->u[o];
+    for (int i = 0; i < K; ++i) {
+        double div = 0.0;
+        for (int a = 0; a < 3; ++a) { if (px[a]) div -= xp[i][a]; if (nx[a]) div += xo[i][a]; }
+        double d = uu - A->c[i];
+        d_out[i] = d;
+        q[i] = fma(A->tau1, fma(-A->lambda, d * d, div), v[i]);
+    }
+}
+
+#define ADJ_FOR_VOXELS(m)                                                        \
+    _Pragma("omp parallel for schedule(dynamic, 8)")                            \
+    for (uint32_t col = (m)->col_begin; col < (m)->col_end; ++col)              \
+        for (uint32_t row = row_begin((m), col); row < row_end((m), col); ++row) \
+            for (uint32_t n = 0; n < (m)->n_images; ++n)
+
+/* One forward iteration s -> s1 (arrays of 10(K-1)... words: V | Vb | Xi). */
+static void adj_forward(const adj_t *A, const double *s, double *s1)
+{
+    const or_layout *m = A->m;
+    const int K = A->K;
+    const size_t S = A->span;
+    const double *V = s, *Vb = s + (K - 1) * S, *Xi = s + 2 * (K - 1) * S;
+    double *V1 = s1, *Vb1 = s1 + (K - 1) * S, *Xi1 = s1 + 2 * (K - 1) * S;
+    ADJ_FOR_VOXELS(m) {
+        double x[8][3], hist[20][15][3];
+        size_t o = line_of(m, row, col) * m->nstride + n;
+        adj_x(A, n, row, col, Vb, Xi, x);
+        adj_dykstra(K, x, A->tol, hist);
+        for (int i = 0; i < K - 1; ++i) for (int a = 0; a < 3; ++a) Xi1[(3 * i + a) * S + o] = x[i][a];
+    }
+    ADJ_FOR_VOXELS(m) {
+        double q[8], d[8], v[8];
+        int act[8];
+        size_t o = line_of(m, row, col) * m->nstride + n;
+        adj_qv(A, n, row, col, V, Xi1, q, d);
+        adj_simplex(K, q, act);
+        for (int i = 0; i < K - 1; ++i) v[i] = V[i * S + o];
+        for (int i = 0; i < K - 1; ++i) { Vb1[i * S + o] = 2.0 * q[i] - v[i]; V1[i * S + o] = q[i]; }
+    }
+}
+
+static void adj_start(const adj_t *A, const int8_t *init, double *s)
+{
+    const or_layout *m = A->m;
+    const int K = A->K;
+    const size_t S = A->span;
+    memset(s, 0, sizeof(double) * 5 * (K - 1) * S);
+    ADJ_FOR_VOXELS(m) {
+        size_t o = line_of(m, row, col) * m->nstride + n;
+        int cl = 0;
+        if (init) cl = init[o];
+        else {
+            double best = DBL_MAX;
+            for (int k = 0; k < K; ++k) { double dd = fabs(A->c[k] - A->u[o]); if (dd < best) { best = dd; cl = k; } }
+        }
+        for (int i = 0; i < K - 1; ++i) s[i * S + o] = s[(K - 1 + i) * S + o] = (i == cl) ? 1.0 : 0.0;
+    }
+}
+
+static double adj_loss(const adj_t *A, const double *s, const float *target, double *gV)
+{
+    const or_layout *m = A->m;
+    const int K = A->K;
+    const size_t S = A->span;
+    double L = 0.0, N = 0.0;
+    for (uint32_t col = m->col_begin; col < m->col_end; ++col) N += (double)(row_end(m, col) - row_begin(m, col));
+    N *= m->n_images;
+    const double w = 1.0 / (N * K);
+#pragma omp parallel for schedule(dynamic, 8) reduction(+ : L)
+    for (uint32_t col = m->col_begin; col < m->col_end; ++col)
+        for (uint32_t row = row_begin(m, col); row < row_end(m, col); ++row)
+            for (uint32_t n = 0; n < m->n_images; ++n) {
+                size_t o = line_of(m, row, col) * m->nstride + n;
+                int y = (int)target[o];
+                double vk = 1.0, e[8];
+                for (int i = 0; i < K - 1; ++i) { double v = s[i * S + o]; vk -= v; e[i] = v - (i == y ? 1.0 : 0.0); }
+                e[K - 1] = vk - (K - 1 == y ? 1.0 : 0.0);
+                for (int i = 0; i < K; ++i) L += w * e[i] * e[i];
+                if (gV) for (int i = 0; i < K - 1; ++i) gV[i * S + o] = 2.0 * w * (e[i] - e[K - 1]);
+            }
+    return L;
+}
+
+static int adj_setup(adj_t *A, const or_layout *m, const double *u, int K, const double *c, double lambda,
+                     double tau1, double tau2, double tol)
+{
+    if (K < 2 || K > 6) return 1;
+    A->m = m; A->K = K; A->u = u; A->lambda = lambda; A->tau1 = tau1; A->tau2 = tau2; A->tol = tol;
+    for (int i = 0; i < K; ++i) A->c[i] = c[i];
+    A->span = (size_t)m->base[m->width];
+    return 0;
+}
+
+double or_adj_loss_d(const or_layout *m, const double *u, const float *target, int K, const double *c,
+                     double lambda, double tau1, double tau2, int iters, const int8_t *init, double tol)
+{
+    adj_t A;
+    if (adj_setup(&A, m, u, K, c, lambda, tau1, tau2, tol)) return NAN;
+    const size_t W = 5 * (size_t)(K - 1) * A.span;
+    double *s = (double *)malloc(sizeof(double) * W), *s1 = (double *)malloc(sizeof(double) * W);
+    if (!s || !s1) { free(s); free(s1); return NAN; }
+    adj_start(&A, init, s);
+    memset(s1, 0, sizeof(double) * W);
+    for (int it = 0; it < iters; ++it) { adj_forward(&A, s, s1); double *t = s; s = s1; s1 = t; }
+    double L = adj_loss(&A, s, target, NULL);
+    free(s); free(s1);
+    return L;
+}
+
+/* Returns the loss; grad_c[K], *grad_lambda and grad_u[span] (optional). */
+double or_adj_grad_d(const or_layout *m, const double *u, const float *target, int K, const double *c,
+                     double lambda, double tau1, double tau2, int iters, const int8_t *init, double tol,
+                     int ck, double *grad_c, double *grad_lambda, double *grad_u)
+{
+    adj_t A;
+    if (adj_setup(&A, m, u, K, c, lambda, tau1, tau2, tol) || iters < 1 || ck < 1) return NAN;
+    const size_t S = A.span, W = 5 * (size_t)(K - 1) * S;
+    const int nck = (iters + ck - 1) / ck;
+    double *cks = (double *)malloc(sizeof(double) * W * (size_t)nck);
+    double *seg = (double *)malloc(sizeof(double) * W * (size_t)(ck + 1));
+    double *gs = (double *)calloc(W, sizeof(double));          /* gV | gVb | gXi of the current state */
+    double *GQ = (double *)calloc((size_t)K * S, sizeof(double));
+    double *GP = (double *)calloc((size_t)3 * K * S, sizeof(double));
+    double *gu = (double *)calloc(S, sizeof(double));
+    if (!cks || !seg || !gs || !GQ || !GP || !gu) { free(cks); free(seg); free(gs); free(GQ); free(GP); free(gu); return NAN; }
+    /* forward with checkpoints at t = 0, ck, 2 ck, ... */
+    adj_start(&A, init, cks);
+    double *cur = seg, *nxt = seg + W;
+    memcpy(cur, cks, sizeof(double) * W);
+    for (int it = 0; it < iters; ++it) {
+        adj_forward(&A, cur, nxt);
+        double *t = cur; cur = nxt; nxt = t;
+        if ((it + 1) % ck == 0 && (it + 1) / ck < nck) memcpy(cks + W * (size_t)((it + 1) / ck), cur, sizeof(double) * W);
+    }
+    const double loss = adj_loss(&A, cur, target, gs);
+    double gl = 0.0, gc[8] = {0};
+    double *gV = gs, *gVb = gs + (K - 1) * S, *gXi = gs + 2 * (K - 1) * S;
+    for (int k = nck - 1; k >= 0; --k) {
+        const int t0 = k * ck, t1 = (t0 + ck < iters) ? t0 + ck : iters;
+        memcpy(seg, cks + W * (size_t)k, sizeof(double) * W);
+        for (int j = 0; j < t1 - t0; ++j) adj_forward(&A, seg + W * (size_t)j, seg + W * (size_t)(j + 1));
+        for (int t = t1 - 1; t >= t0; --t) {
+            const double *st = seg + W * (size_t)(t - t0), *st1 = st + W;
+            const double *V = st, *Vb = st + (K - 1) * S, *Xi = st + 2 * (K - 1) * S, *Xi1 = st1 + 2 * (K - 1) * S;
+            double gl_t = 0.0, gc_t[8] = {0};
+            /* A: through V+ = Pi_S(q) and V-bar+ = 2 V+ - V */
+#pragma omp parallel for schedule(dynamic, 8) reduction(+ : gl_t)
+            for (uint32_t col = m->col_begin; col < m->col_end; ++col) {
+                double gcl[8] = {0};
+                for (uint32_t row = row_begin(m, col); row < row_end(m, col); ++row)
+                    for (uint32_t n = 0; n < m->n_images; ++n) {
+                        size_t o = line_of(m, row, col) * m->nstride + n;
+                        double q[8], d[8], ge[8], gq[8];
+                        int act[8], na = 0;
+                        adj_qv(&A, n, row, col, V, Xi1, q, d);
+                        adj_simplex(K, q, act);
+                        double mean = 0.0;
+                        for (int i = 0; i < K; ++i) {
+                            ge[i] = (i < K - 1) ? gV[i * S + o] + 2.0 * gVb[i * S + o] : 0.0;
+                            if (act[i]) { mean += ge[i]; ++na; }
+                        }
+                        mean /= (double)na;
+                        for (int i = 0; i < K; ++i) { gq[i] = act[i] ? ge[i] - mean : 0.0; GQ[i * S + o] = gq[i]; }
+                        for (int i = 0; i < K - 1; ++i) gV[i * S + o] = gq[i] - gq[K - 1] - gVb[i * S + o];
+                        for (int i = 0; i < K; ++i) {
+                            const double gG = -A.tau1 * gq[i];
+                            gl_t += gG * d[i] * d[i];
+                            gcl[i] += gG * (-2.0 * A.lambda * d[i]);
+                            gu[o] += gG * 2.0 * A.lambda * d[i];
+                        }
+                    }
+#pragma omp critical
+                for (int i = 0; i < K; ++i) gc_t[i] += gcl[i];
+            }
+            gl += gl_t;
+            for (int i = 0; i < K; ++i) gc[i] += gc_t[i];
+            /* B: through Xi+ = Pi_K(Xi + tau2 grad V-bar) */
+            ADJ_FOR_VOXELS(m) {
+                size_t o = line_of(m, row, col) * m->nstride + n, on[3], op[3];
+                int nx[3], px[3];
+                adj_nbrs(m, n, row, col, o, nx, px, on, op);
+                double g[8][3], full[8][3], x[8][3], hist[20][15][3];
+                for (int i = 0; i < K; ++i)
+                    for (int a = 0; a < 3; ++a)
+                        full[i][a] = nx[a] ? -A.tau1 * (GQ[i * S + on[a]] - GQ[i * S + o]) : 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    for (int i = 0; i < K - 1; ++i) g[i][a] = gXi[(3 * i + a) * S + o] + full[i][a] - full[K - 1][a];
+                    g[K - 1][a] = 0.0;
+                }
+                adj_x(&A, n, row, col, Vb, Xi, x);
+                const int sw = adj_dykstra(K, x, A.tol, hist);
+                adj_dykstra_back(K, sw, hist, g);
+                for (int i = 0; i < K - 1; ++i)
+                    for (int a = 0; a < 3; ++a) gXi[(3 * i + a) * S + o] = g[i][a] - g[K - 1][a];
+                for (int i = 0; i < K; ++i)
+                    for (int a = 0; a < 3; ++a) GP[(3 * i + a) * S + o] = A.tau2 * g[i][a];
+            }
+            /* C: V-bar enters through grad V-bar: adjoint -div */
+            ADJ_FOR_VOXELS(m) {
+                size_t o = line_of(m, row, col) * m->nstride + n, on[3], op[3];
+                int nx[3], px[3];
+                adj_nbrs(m, n, row, col, o, nx, px, on, op);
+                double full[8];
+                for (int i = 0; i < K; ++i) {
+                    double dv = 0.0;
+                    for (int a = 0; a < 3; ++a) {
+                        if (nx[a]) dv += GP[(3 * i + a) * S + o];
+                        if (px[a]) dv -= GP[(3 * i + a) * S + op[a]];
+                    }
+                    full[i] = -dv;
+                }
+                for (int i = 0; i < K - 1; ++i) gVb[i * S + o] = full[i] - full[K - 1];
+            }
+        }
+    }
+    if (grad_c) for (int i = 0; i < K; ++i) grad_c[i] = gc[i];
+    if (grad_lambda) *grad_lambda = gl;
+    if (grad_u) memcpy(grad_u, gu, sizeof(double) * S);
+    free(cks); free(seg); free(gs); free(GQ); free(GP); free(gu);
+    return loss;
+}

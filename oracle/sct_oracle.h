@@ -92,6 +92,16 @@ double or_support_irls_d(int K, const double *p);   /* the K >= 4 method, any K 
 int or_potts_gap_d(const or_layout *m, const float *u, int K, const float *c, float lambda,
                    const double *V, const double *xi, double *primal, double *dual);
 
+/* FP64 adjoint of the unrolled iteration (no counterpart in the reference;
+ * definition in sct_oracle.c).  u: double [span]; target: label index per
+ * voxel [span]; init: start labels [span] or NULL for the nearest label;
+ * tol: Dykstra stopping tolerance (1e-6 in the solver). */
+double or_adj_loss_d(const or_layout *m, const double *u, const float *target, int K, const double *c,
+                     double lambda, double tau1, double tau2, int iters, const int8_t *init, double tol);
+double or_adj_grad_d(const or_layout *m, const double *u, const float *target, int K, const double *c,
+                     double lambda, double tau1, double tau2, int iters, const int8_t *init, double tol,
+                     int ck, double *grad_c, double *grad_lambda, double *grad_u);
+
 float or_bf16_round(float f);
 float or_f16_round(float f);
 
