@@ -208,6 +208,20 @@ typedef struct sct_potts_problem {
 int sct_potts_batch(sct_ctx *ctx, int n, const sct_potts_problem *problems, float tau1, float tau2,
                     int iters, int storage, int flags, sct_field **out, float *elapsed_ms);
 
+/* ---- adjoint (no counterpart in the reference) ---------------------------- */
+/* Gradient of L = 1/(N K) sum_x sum_{i<K} (V_i(x) - [target(x) = i])^2, V after
+ * iters primal-dual iterations from the nearest-label start (N in-mask voxels,
+ * target: float field of label indices with the layout of g), with respect to
+ * c[0..K-1], lambda and u, by reverse-mode differentiation of the unrolled
+ * iteration (FP32).  States are checkpointed every checkpoint_every
+ * iterations (0: round(sqrt(iters))) and recomputed segment by segment.
+ * *grad_u (optional) receives a new field; times_ms[2] (optional) the event
+ * times of the forward pass and of the reverse pass.  Synchronises. */
+int sct_potts_adjoint(sct_ctx *ctx, const sct_field *g, const sct_field *target, int K, const float *c,
+                      float lambda, float tau1, float tau2, int iters, int checkpoint_every,
+                      double *loss, double *grad_c, double *grad_lambda, sct_field **grad_u,
+                      float *times_ms);
+
 /* ---- diagnostics (no counterpart in the reference) ------------------------ */
 /* Checks on the GPU that the Dykstra projection's range-specific correctly
  * rounded square root (x in [1, 2^64)) and quotient (b in [1, 2^64),

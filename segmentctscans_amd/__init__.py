@@ -340,6 +340,32 @@ def potts_batch(ctx: Context, problems, tau1: float, tau2: float, iters: int, *,
     return [DeviceField(ctx, C.c_void_p(outs[i]), problems[i][0].mapping) for i in range(n)], ms.value
 
 
+def potts_adjoint(ctx: Context, g: DeviceField, target: DeviceField, ci, lam: float, tau1: float, tau2: float,
+                  iters: int, *, checkpoint_every: int = 0, want_grad_u: bool = True) -> dict:
+    """Gradient of the mean squared error between V (all K components) and the
+    one-hot target after iters iterations, with respect to c, lambda and u
+    (no counterpart in the reference).  Returns dict(loss, grad_c, grad_lambda,
+    grad_u (host span array or None), forward_ms, backward_ms)."""
+    c = np.ascontiguousarray(ci, np.float32)
+    if not 2 <= c.size <= max_phases:
+        raise ValueError("K must be in [2, 6]")
+    loss, gl = C.c_double(), C.c_double()
+    gc = np.zeros(c.size, np.float64)
+    h = C.c_void_p()
+    t = np.zeros(2, np.float32)
+    ctx.check(N.lib().sct_potts_adjoint(ctx.handle, g.handle, target.handle, c.size, c.ctypes.data_as(N.f32p), lam,
+                                        tau1, tau2, iters, checkpoint_every, C.byref(loss),
+                                        gc.ctypes.data_as(C.POINTER(C.c_double)), C.byref(gl),
+                                        C.byref(h) if want_grad_u else None, t.ctypes.data_as(N.f32p)))
+    gu = None
+    if want_grad_u:
+        f = DeviceField(ctx, h, g.mapping)
+        gu = f.to_host()
+        f.close()
+    return dict(loss=loss.value, grad_c=gc, grad_lambda=gl.value, grad_u=gu, forward_ms=float(t[0]),
+                backward_ms=float(t[1]))
+
+
 class PottsSolver:
     """Stepping interface over the same kernels (single slab)."""
 

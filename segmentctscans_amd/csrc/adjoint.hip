@@ -1,0 +1,481 @@
+// adjoint.hip -- gradient of a segmentation loss through the unrolled Potts
+// iteration (no counterpart in the reference).
+//
+// Loss after `iters` iterations from the nearest-label start:
+//   L = 1/(N K) sum_x sum_{i<K} (V_i(x) - [target(x) = i])^2,
+// N in-mask voxels, V_{K-1} = 1 - sum of the stored components.  Gradients
+// with respect to the labels c, lambda and u flow through the data term
+// G_i = lambda (u - c_i)^2 of every V step; the start (nearest label) is
+// piecewise constant, so its gradient is 0 almost everywhere.
+//
+// Reverse step for one iteration s_t = (V, V-bar, Xi) -> s_{t+1}, two kernels:
+//   A  V+ = Pi_S(q), V-bar+ = 2 V+ - V: the simplex projection's Jacobian is
+//      diag(a) - a a^T / |a| on its active set a; q = V + tau1 (div Xi+ - G)
+//      passes the adjoint to V, to div Xi+ and to G (-> c, lambda, u);
+//   B  div's adjoint is -grad; Xi+ = Pi_K(x), x = Xi + tau2 grad V-bar: the
+//      Dykstra sweeps are replayed with the same arithmetic (so the same sweep
+//      count and the same ball tests) recording each pair's h, and each pair
+//      step is back-propagated: the excess map f(h) = h (1 - 1/|h|) for
+//      |h| > 1 has Jacobian (1 - 1/|h|) I + h h^T / |h|^3;
+//   C  grad's adjoint is -div, into V-bar; evaluated inside the next
+//      step's A (it needs only B's output at the voxel and its -e_a
+//      neighbours), so the V-bar adjoint is never stored.
+// The implicit K-th components turn each full-K adjoint g into g_i - g_{K-1}
+// on the stored components.
+//
+// States are checkpointed every `ck` iterations during the forward pass; the
+// reverse pass recomputes each segment of ck states from its checkpoint
+// (memory ~ (T/ck + ck) states, ck = sqrt(T) by default) with the forward
+// kernels of the solver, so the replayed values are bitwise the forward's.
+#include "potts.hpp"
+#include "twopass.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+using namespace sct;
+using namespace sct::tp;
+
+namespace {
+
+constexpr int kAdjThreads = 256;
+
+// Work layout of the adjoint: state = V (K-1) | V-bar (K-1) | Xi 3(K-1)
+// components of wspan floats each.
+struct AdjLayout {
+    VolumeLines fwd;    // chunks of 128 (the FP32 solver kernels)
+    VolumeLines one;    // chunks of 64, one voxel per lane (adjoint kernels)
+    uint64_t wspan;
+};
+
+template <int K>
+TwoPassArgs<K, StorageF32> state_args(float *st, const float *u, uint64_t wspan)
+{
+    TwoPassArgs<K, StorageF32> a;
+    for (int i = 0; i < K - 1; ++i) {
+        a.V[i] = st + uint64_t(i) * wspan;
+        a.vbar[i] = st + uint64_t(K - 1 + i) * wspan;
+    }
+    for (int j = 0; j < 3 * (K - 1); ++j) a.xi[j] = st + uint64_t(2 * (K - 1) + j) * wspan;
+    a.u = u;
+    return a;
+}
+
+// One voxel per lane: (chunk, lane) of the 64-voxel chunking.
+struct Voxel {
+    bool ok;
+    Chunk ch;
+    uint32_t n, w, uo;
+};
+
+__device__ __forceinline__ Voxel voxel_of(const VolumeLines &L)
+{
+    Voxel v{};
+    const uint64_t gw = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    if (gw >= L.count()) return v;
+    v.ch = L.chunk<1>(gw);
+    const uint32_t lane = threadIdx.x % kWave;
+    v.n = v.ch.n0 + lane;
+    v.ok = v.n < v.ch.len;
+    v.w = v.ch.w + lane;
+    v.uo = v.ch.u + lane;
+    return v;
+}
+
+// Block sum of x[N]; thread k < N adds component k to out[blockIdx.x * N + k]
+// (one slot per workgroup, accumulated over launches with the same grid).
+template <int N>
+__device__ __forceinline__ void block_add(double (&x)[N], double *out)
+{
+    __shared__ double red[N][kAdjThreads / kWave];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double t = x[i];
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, kWave);
+        if (threadIdx.x % kWave == 0) red[i][threadIdx.x / kWave] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < N) {
+        double t = 0.0;
+        for (int k = 0; k < int(blockDim.x / kWave); ++k) t += red[threadIdx.x][k];
+        out[uint64_t(blockIdx.x) * N + threadIdx.x] += t;
+    }
+}
+
+// Loss and its adjoint with respect to the stored components of V_T.
+template <int K>
+__global__ __launch_bounds__(kAdjThreads) void adj_loss_kernel(VolumeLines L, uint64_t wspan, const float *st,
+                                                               const float *target, float wgt, float *gV,
+                                                               double *loss)
+{
+    const Voxel vx = voxel_of(L);
+    double part[1] = {0.0};
+    if (vx.ok) {
+        const int y = int(target[vx.uo]);
+        float e[K], vk = 1.0f;
+        for (int i = 0; i < K - 1; ++i) {
+            const float v = st[uint64_t(i) * wspan + vx.w];
+            vk -= v;
+            e[i] = v - (i == y ? 1.0f : 0.0f);
+        }
+        e[K - 1] = vk - (K - 1 == y ? 1.0f : 0.0f);
+        for (int i = 0; i < K; ++i) part[0] += double(wgt) * double(e[i]) * double(e[i]);
+        for (int i = 0; i < K - 1; ++i) gV[uint64_t(i) * wspan + vx.w] = 2.0f * wgt * (e[i] - e[K - 1]);
+    }
+    block_add<1>(part, loss);
+}
+
+// Adjoint of V-bar+ from the previous reverse step's GP = tau2 x (adjoint of
+// Dykstra's input): grad's adjoint is -div, on all K components, then the
+// implicit K-th component folds into the stored ones.
+template <int K>
+__device__ __forceinline__ void vbar_adjoint(const Voxel &vx, uint64_t wspan, const float *GP, float (&gvb)[K - 1])
+{
+    const Chunk &ch = vx.ch;
+    const uint32_t lane = threadIdx.x % kWave;
+    const bool px[3] = {vx.n > 0, ch.pr != kNone, ch.pc != kNone};
+    const bool nx[3] = {vx.n + 1 < ch.len, ch.nr != kNone, ch.nc != kNone};
+    const uint32_t wp[3] = {vx.w - 1, ch.pr + lane, ch.pc + lane};
+    float full[K];
+    for (int i = 0; i < K; ++i) {
+        float dv = 0.0f;
+        for (int a = 0; a < 3; ++a) {
+            if (nx[a]) dv += GP[uint64_t(3 * i + a) * wspan + vx.w];
+            if (px[a]) dv -= GP[uint64_t(3 * i + a) * wspan + wp[a]];
+        }
+        full[i] = -dv;
+    }
+    for (int i = 0; i < K - 1; ++i) gvb[i] = full[i] - full[K - 1];
+}
+
+// A: through V-bar+ = 2 V+ - V and V+ = Pi_S(q).  gs = gV | gXi.
+template <int K>
+__global__ __launch_bounds__(kAdjThreads) void adj_v_kernel(VolumeLines L, uint64_t wspan, const float *st,
+                                                            const float *st1, const float *u, SolveParams<K> sp,
+                                                            float tau1, const float *GP, float *gs, float *GQ,
+                                                            float *gu, double *acc)
+{
+    constexpr int KM = K - 1;
+    const Voxel vx = voxel_of(L);
+    double part[K + 1] = {};
+    if (vx.ok) {
+        const Chunk &ch = vx.ch;
+        const uint32_t lane = threadIdx.x % kWave;
+        float gvb[KM];
+        vbar_adjoint<K>(vx, wspan, GP, gvb);
+        const float *Xi1 = st1 + uint64_t(2 * KM) * wspan;
+        const bool px[3] = {vx.n > 0, ch.pr != kNone, ch.pc != kNone};
+        const bool nx[3] = {vx.n + 1 < ch.len, ch.nr != kNone, ch.nc != kNone};
+        const uint32_t wp[3] = {vx.w - 1, ch.pr + lane, ch.pc + lane};
+        float xo[KM][3], xp[3][KM], v[KM];
+        for (int i = 0; i < KM; ++i) {
+            v[i] = st[uint64_t(i) * wspan + vx.w];
+            for (int a = 0; a < 3; ++a) {
+                xo[i][a] = Xi1[uint64_t(3 * i + a) * wspan + vx.w];
+                xp[a][i] = px[a] ? Xi1[uint64_t(3 * i + a) * wspan + wp[a]] : 0.0f;
+            }
+        }
+        const float uu = u[vx.uo];
+        float vf[K], q[K];
+        bool act[K];
+        v_input<K>(xo, xp, px, nx, v, uu, sp.c, sp.lambda, tau1, vf, q);
+        project_simplex<K>(q, &act);
+        float ge[K], mean = 0.0f;
+        int na = 0;
+        for (int i = 0; i < K; ++i) {
+            ge[i] = i < KM ? gs[uint64_t(i) * wspan + vx.w] + 2.0f * gvb[i] : 0.0f;
+            if (act[i]) {
+                mean += ge[i];
+                ++na;
+            }
+        }
+        mean /= float(na);
+        float gq[K], gul = 0.0f;
+        for (int i = 0; i < K; ++i) {
+            gq[i] = act[i] ? ge[i] - mean : 0.0f;
+            GQ[uint64_t(i) * wspan + vx.w] = gq[i];
+            const float d = uu - sp.c[i];
+            const float gG = -tau1 * gq[i];
+            gul += gG * 2.0f * sp.lambda * d;
+            part[i] += double(gG) * (-2.0 * double(sp.lambda) * double(d));
+            part[K] += double(gG) * double(d) * double(d);
+        }
+        for (int i = 0; i < KM; ++i) gs[uint64_t(i) * wspan + vx.w] = gq[i] - gq[KM] - gvb[i];
+        gu[vx.w] += gul;
+    }
+    block_add<K + 1>(part, acc);
+}
+
+// Dykstra's sweeps replayed from x with the solver's arithmetic, recording h.
+template <int K>
+__device__ __forceinline__ int dykstra_replay(float (&q)[K][3], float (*hist)[K * (K - 1) / 2][3], DykstraLimits lim)
+{
+    constexpr int NP = K * (K - 1) / 2;
+    float r[NP][3];
+    for (int p = 0; p < NP; ++p) r[p][0] = r[p][1] = r[p][2] = 0.0f;
+    int sw = 0;
+    while (true) {
+        const float m2 = dykstra_sweep<K>(q, r, lim, hist[sw]);
+        ++sw;
+        if (m2 < lim.stol || sw == 20) break;
+    }
+    return sw;
+}
+
+// g: adjoint of Dykstra's output on entry, of its input on return.
+template <int K>
+__device__ __forceinline__ void dykstra_back(int sweeps, const float (*hist)[K * (K - 1) / 2][3], float (&g)[K][3],
+                                             DykstraLimits lim)
+{
+    constexpr int NP = K * (K - 1) / 2;
+    float gr[NP][3];
+    for (int p = 0; p < NP; ++p) gr[p][0] = gr[p][1] = gr[p][2] = 0.0f;
+    for (int s = sweeps - 1; s >= 0; --s)
+        for (int i1 = K - 2, p = NP - 1; i1 >= 0; --i1)
+            for (int i2 = K - 1; i2 > i1; --i2, --p) {
+                const float h0 = hist[s][p][0], h1 = hist[s][p][1], h2 = hist[s][p][2];
+                const float ss = __fmaf_rn(h2, h2, __fmaf_rn(h1, h1, h0 * h0));
+                float gd[3], ge[3], gh[3] = {0.0f, 0.0f, 0.0f};
+                for (int a = 0; a < 3; ++a) {
+                    gd[a] = g[i1][a] - g[i2][a];
+                    ge[a] = 0.5f * gd[a] + gr[p][a];
+                }
+                if (ss > lim.t1) {
+                    const float l = sqrtf(ss), il = 1.0f / l;
+                    const float hg = h0 * ge[0] + h1 * ge[1] + h2 * ge[2];
+                    const float k = hg * il * il * il, s1 = 1.0f - il;
+                    gh[0] = s1 * ge[0] + k * h0;
+                    gh[1] = s1 * ge[1] + k * h1;
+                    gh[2] = s1 * ge[2] + k * h2;
+                }
+                for (int a = 0; a < 3; ++a) {
+                    g[i1][a] -= gh[a];
+                    g[i2][a] += gh[a];
+                    gr[p][a] = -0.5f * gd[a] + gh[a];
+                }
+            }
+}
+
+// B: div's adjoint (-grad of tau1 GQ), then back through Pi_K.
+template <int K>
+__global__ __launch_bounds__(kAdjThreads) void adj_xi_kernel(VolumeLines L, uint64_t wspan, const float *st,
+                                                             float tau1, float tau2, DykstraLimits lim,
+                                                             const float *GQ, float *gs, float *GP)
+{
+    constexpr int KM = K - 1, NP = K * (K - 1) / 2;
+    const Voxel vx = voxel_of(L);
+    if (!vx.ok) return;
+    const Chunk &ch = vx.ch;
+    const uint32_t lane = threadIdx.x % kWave;
+    const bool nx[3] = {vx.n + 1 < ch.len, ch.nr != kNone, ch.nc != kNone};
+    const uint32_t wn[3] = {vx.w + 1, ch.nr + lane, ch.nc + lane};
+    float full[K][3];
+    for (int i = 0; i < K; ++i) {
+        const float own = GQ[uint64_t(i) * wspan + vx.w];
+        for (int a = 0; a < 3; ++a) full[i][a] = nx[a] ? -tau1 * (GQ[uint64_t(i) * wspan + wn[a]] - own) : 0.0f;
+    }
+    float *gXi = gs + uint64_t(KM) * wspan;
+    float g[K][3];
+    for (int a = 0; a < 3; ++a) {
+        for (int i = 0; i < KM; ++i) g[i][a] = gXi[uint64_t(3 * i + a) * wspan + vx.w] + full[i][a] - full[KM][a];
+        g[KM][a] = 0.0f;
+    }
+    // recompute x = Xi + tau2 grad V-bar as the solver does
+    const float *Vb = st + uint64_t(KM) * wspan, *Xi = st + uint64_t(2 * KM) * wspan;
+    float vb[KM], vbn[3][KM], xo[KM][3];
+    for (int i = 0; i < KM; ++i) {
+        vb[i] = Vb[uint64_t(i) * wspan + vx.w];
+        for (int a = 0; a < 3; ++a) {
+            vbn[a][i] = nx[a] ? Vb[uint64_t(i) * wspan + wn[a]] : 0.0f;
+            xo[i][a] = Xi[uint64_t(3 * i + a) * wspan + vx.w];
+        }
+    }
+    float x[K][3];
+    xi_input<K>(vb, vbn, nx, xo, tau2, x);
+    float x1[K][3];
+    for (int i = 0; i < K; ++i)
+        for (int a = 0; a < 3; ++a) x1[i][a] = x[i][a];
+    // A feasible input leaves Dykstra after one sweep without change: identity.
+    if (!dykstra_first_sweep_done<K>(x1, lim)) {
+        float hist[20][NP][3];
+        const int sw = dykstra_replay<K>(x, hist, lim);
+        dykstra_back<K>(sw, hist, g, lim);
+    }
+    for (int i = 0; i < KM; ++i)
+        for (int a = 0; a < 3; ++a) gXi[uint64_t(3 * i + a) * wspan + vx.w] = g[i][a] - g[KM][a];
+    for (int i = 0; i < K; ++i)
+        for (int a = 0; a < 3; ++a) GP[uint64_t(3 * i + a) * wspan + vx.w] = tau2 * g[i][a];
+}
+
+struct Buffers {
+    std::vector<float *> ck, seg;
+    float *gs = nullptr, *GQ = nullptr, *GP = nullptr, *gu = nullptr;
+    double *acc = nullptr;   // dc[K], dlambda, loss
+};
+
+void free_buffers(sct_ctx *ctx, Buffers &b)
+{
+    for (float *p : b.ck) sct_free(ctx, p);
+    for (float *p : b.seg) sct_free(ctx, p);
+    for (void *p : {(void *)b.gs, (void *)b.GQ, (void *)b.GP, (void *)b.gu, (void *)b.acc}) sct_free(ctx, p);
+    b = Buffers{};
+}
+
+template <int K>
+int run_adjoint(sct_ctx *ctx, const sct_field *g, const sct_field *target, const float *c, float lambda, float tau1,
+                float tau2, int iters, int ck, double *loss, double *grad_c, double *grad_lambda, sct_field **grad_u,
+                float *times_ms)
+{
+    const Geometry &geo = g->geo;
+    const uint32_t n = geo.n_images, ws = sct_get_stride(n);
+    AdjLayout Ly;
+    Ly.fwd = VolumeLines{geo.d_nbr, geo.lines, n, (n + 127) / 128, ws, geo.nstride, 0};
+    Ly.one = VolumeLines{geo.d_nbr, geo.lines, n, (n + 63) / 64, ws, geo.nstride, 0};
+    Ly.wspan = uint64_t(geo.lines) * ws;
+    const uint64_t S = std::max<uint64_t>(1, Ly.wspan), state = 5 * uint64_t(K - 1) * S;
+    const int nck = (iters + ck - 1) / ck;
+    const int seg_len = std::min(ck, iters);
+    hipStream_t st = ctx->stream;
+    Buffers B;
+    int rc = SCT_OK;
+    auto alloc = [&](float *&p, uint64_t elems) {
+        if (!rc) rc = sct_alloc(ctx, (void **)&p, sizeof(float) * elems);
+    };
+    B.ck.assign(size_t(nck), nullptr);
+    B.seg.assign(size_t(seg_len) + 1, nullptr);
+    for (auto &p : B.ck) alloc(p, state);
+    for (auto &p : B.seg) alloc(p, state);
+    alloc(B.gs, 4 * uint64_t(K - 1) * S);   // gV | gXi
+    alloc(B.GQ, uint64_t(K) * S);
+    alloc(B.GP, 3 * uint64_t(K) * S);
+    alloc(B.gu, S);
+    const uint32_t agrid = uint32_t((Ly.one.count() * kWave + kAdjThreads - 1) / kAdjThreads);
+    // per-workgroup sums: dc[K], dlambda of every A launch, then the loss
+    const uint64_t nacc = uint64_t(agrid) * (K + 2);
+    if (!rc) rc = sct_alloc(ctx, (void **)&B.acc, sizeof(double) * std::max<uint64_t>(1, nacc));
+    hipEvent_t ev[4] = {};
+    for (auto &e : ev)
+        if (!rc && hipEventCreate(&e) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventCreate failed");
+    if (rc) {
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+        free_buffers(ctx, B);
+        return rc;
+    }
+    const float *u = (const float *)g->data;
+    OneSolve<K> prm;
+    for (int i = 0; i < K; ++i) prm.p.c[i] = c[i];
+    prm.p.lambda = lambda;
+    const DykstraLimits lim = dykstra_limits();
+    const uint32_t fgrid = wave_grid(Ly.fwd.count()), xgrid = xi_grid<StorageF32>(Ly.fwd.count());
+    auto forward = [&](float *s) {
+        const auto a = state_args<K>(s, u, S);
+        potts_xi_kernel<K, StorageF32><<<xgrid, 256, 0, st>>>(Ly.fwd, a, tau2, lim);
+        potts_v_kernel<K, StorageF32, false><<<fgrid, 256, 0, st>>>(Ly.fwd, a, prm, tau1, nullptr);
+    };
+    auto chk = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess && !rc) rc = sct_hip_fail(ctx, e, what);
+    };
+    auto copy = [&](float *dst, const float *src) {
+        chk(hipMemcpyAsync(dst, src, sizeof(float) * state, hipMemcpyDeviceToDevice, st), "adjoint state copy");
+    };
+    // forward pass with checkpoints
+    float *cur = B.seg[0];
+    chk(hipMemsetAsync(cur, 0, sizeof(float) * state, st), "memset");
+    chk(hipEventRecord(ev[0], st), "hipEventRecord");
+    potts_init_kernel<K, StorageF32><<<fgrid, 256, 0, st>>>(Ly.fwd, state_args<K>(cur, u, S), prm);
+    copy(B.ck[0], cur);
+    for (int it = 0; it < iters && !rc; ++it) {
+        forward(cur);
+        if ((it + 1) % ck == 0 && (it + 1) / ck < nck) copy(B.ck[(it + 1) / ck], cur);
+    }
+    chk(hipEventRecord(ev[1], st), "hipEventRecord");
+    // loss and the adjoint of the final state
+    chk(hipMemsetAsync(B.gs, 0, sizeof(float) * 4 * (K - 1) * S, st), "memset");
+    chk(hipMemsetAsync(B.gu, 0, sizeof(float) * S, st), "memset");
+    chk(hipMemsetAsync(B.GP, 0, sizeof(float) * 3 * K * S, st), "memset");
+    chk(hipMemsetAsync(B.acc, 0, sizeof(double) * std::max<uint64_t>(1, nacc), st), "memset");
+    const float wgt = float(1.0 / (double(std::max<uint64_t>(1, geo.data_size)) * K));
+    adj_loss_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, cur, (const float *)target->data, wgt, B.gs,
+                                                       B.acc + uint64_t(agrid) * (K + 1));
+    // reverse pass, one checkpoint segment at a time
+    chk(hipEventRecord(ev[2], st), "hipEventRecord");
+    for (int k = nck - 1; k >= 0 && !rc; --k) {
+        const int t0 = k * ck, t1 = std::min(iters, t0 + ck);
+        copy(B.seg[0], B.ck[size_t(k)]);
+        for (int j = 0; j < t1 - t0; ++j) {
+            copy(B.seg[size_t(j) + 1], B.seg[size_t(j)]);
+            forward(B.seg[size_t(j) + 1]);
+        }
+        for (int t = t1 - 1; t >= t0; --t) {
+            const float *s0 = B.seg[size_t(t - t0)], *s1 = B.seg[size_t(t - t0) + 1];
+            adj_v_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, s1, u, prm.p, tau1, B.GP, B.gs, B.GQ, B.gu,
+                                                            B.acc);
+            adj_xi_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, tau1, tau2, lim, B.GQ, B.gs, B.GP);
+        }
+    }
+    chk(hipEventRecord(ev[3], st), "hipEventRecord");
+    chk(hipGetLastError(), "adjoint launch");
+    std::vector<double> acc(nacc);
+    (void)hipStreamSynchronize(st);
+    if (!rc && nacc) {
+        hipError_t e = hipMemcpyAsync(acc.data(), B.acc, sizeof(double) * nacc, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) rc = sct_hip_fail(ctx, e, "adjoint");
+    }
+    if (!rc && grad_u) {
+        sct_layout l{geo.n_images, geo.height, geo.width, geo.nstride, geo.col_begin, geo.col_end,
+                     geo.start_row.data(), geo.base.data()};
+        rc = sct_field_create(ctx, &l, SCT_F32, grad_u);
+        if (!rc) {
+            hipError_t e = hipMemsetAsync((*grad_u)->data, 0, sizeof(float) * geo.span(), st);
+            if (e == hipSuccess && geo.lines)
+                e = hipMemcpy2DAsync((*grad_u)->data, sizeof(float) * geo.nstride, B.gu, sizeof(float) * ws,
+                                     sizeof(float) * n, geo.lines, hipMemcpyDeviceToDevice, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) rc = sct_hip_fail(ctx, e, "adjoint grad_u");
+        }
+    }
+    if (!rc && times_ms) {
+        if (hipEventElapsedTime(&times_ms[0], ev[0], ev[1]) != hipSuccess ||
+            hipEventElapsedTime(&times_ms[1], ev[2], ev[3]) != hipSuccess)
+            rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventElapsedTime failed");
+    }
+    for (auto e : ev) (void)hipEventDestroy(e);
+    free_buffers(ctx, B);
+    if (rc) return rc;
+    double sums[8] = {};
+    for (uint32_t b = 0; b < agrid; ++b) {
+        for (int i = 0; i <= K; ++i) sums[i] += acc[size_t(b) * (K + 1) + size_t(i)];
+        sums[K + 1] += acc[size_t(agrid) * (K + 1) + b];
+    }
+    for (int i = 0; i < K; ++i) grad_c[i] = sums[i];
+    *grad_lambda = sums[K];
+    *loss = sums[K + 1];
+    return SCT_OK;
+}
+
+}  // namespace
+
+extern "C" int sct_potts_adjoint(sct_ctx *ctx, const sct_field *g, const sct_field *target, int K, const float *c,
+                                 float lambda, float tau1, float tau2, int iters, int checkpoint_every,
+                                 double *loss, double *grad_c, double *grad_lambda, sct_field **grad_u,
+                                 float *times_ms)
+{
+    if (!ctx) return SCT_ERR_INVALID;
+    if (grad_u) *grad_u = nullptr;
+    if (!g || !target || !c || !loss || !grad_c || !grad_lambda || K < 2 || K > 6 || iters < 1 || checkpoint_every < 0)
+        return sct_fail(ctx, SCT_ERR_INVALID, "invalid adjoint arguments");
+    if (g->dtype != SCT_F32 || target->dtype != SCT_F32 || target->geo.span() != g->geo.span() ||
+        target->geo.base != g->geo.base || target->geo.start_row != g->geo.start_row)
+        return sct_fail(ctx, SCT_ERR_INVALID, "target must be a float field with the layout of g");
+    const int ck = checkpoint_every > 0 ? checkpoint_every : std::max(1, int(std::lround(std::sqrt(double(iters)))));
+    switch (K) {
+    case 2: return run_adjoint<2>(ctx, g, target, c, lambda, tau1, tau2, iters, ck, loss, grad_c, grad_lambda, grad_u, times_ms);
+    case 3: return run_adjoint<3>(ctx, g, target, c, lambda, tau1, tau2, iters, ck, loss, grad_c, grad_lambda, grad_u, times_ms);
+    case 4: return run_adjoint<4>(ctx, g, target, c, lambda, tau1, tau2, iters, ck, loss, grad_c, grad_lambda, grad_u, times_ms);
+    case 5: return run_adjoint<5>(ctx, g, target, c, lambda, tau1, tau2, iters, ck, loss, grad_c, grad_lambda, grad_u, times_ms);
+    default: return run_adjoint<6>(ctx, g, target, c, lambda, tau1, tau2, iters, ck, loss, grad_c, grad_lambda, grad_u, times_ms);
+    }
+}

@@ -76,8 +76,9 @@ __device__ __forceinline__ void sort_desc(float (&u)[K])
 }
 
 // Euclidean projection onto the probability simplex, potts-min-partition.cpp:292-347.
+// act (optional) receives the active set {i : q_i > theta}.
 template <int K>
-__device__ __forceinline__ void project_simplex(float (&q)[K])
+__device__ __forceinline__ void project_simplex(float (&q)[K], bool (*act)[K] = nullptr)
 {
     float u[K];
 #pragma unroll
@@ -96,14 +97,17 @@ __device__ __forceinline__ void project_simplex(float (&q)[K])
     for (int i = 0; i < K; ++i) {
         const float a = q[i] - theta;
         q[i] = (a < 0.0f) ? 0.0f : a;    // cuda::std::max(a, 0.0f)
+        if (act) (*act)[i] = a > 0.0f;
     }
 }
 
 // One sweep of Dykstra's alternating projection onto {|q_i1 - q_i2| <= 1 for
 // all pairs}, potts-min-partition.cpp:204-234.  r holds the correction of
 // each pair.  Returns the largest squared correction norm of the sweep.
+// hist (optional) receives the pre-projection difference h of every pair.
 template <int K>
-__device__ __forceinline__ float dykstra_sweep(float (&q)[K][3], float (&r)[K * (K - 1) / 2][3], DykstraLimits lim)
+__device__ __forceinline__ float dykstra_sweep(float (&q)[K][3], float (&r)[K * (K - 1) / 2][3], DykstraLimits lim,
+                                               float (*hist)[3] = nullptr)
 {
     float m2 = 0.0f;
 #pragma unroll
@@ -115,6 +119,11 @@ __device__ __forceinline__ float dykstra_sweep(float (&q)[K][3], float (&r)[K * 
             const float h1 = (q[i2][1] - q[i1][1]) + r[p][1];
             const float h2 = (q[i2][2] - q[i1][2]) + r[p][2];
             const float ss = __fmaf_rn(h2, h2, __fmaf_rn(h1, h1, h0 * h0));
+            if (hist) {
+                hist[p][0] = h0;
+                hist[p][1] = h1;
+                hist[p][2] = h2;
+            }
             float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
             if (ss > lim.t1) {
                 // ss > t1 gives l > 1, so l - 1 >= 2^-23 and l - 1 < l
@@ -244,15 +253,14 @@ __device__ __forceinline__ int xi_update(const float (&vb)[K - 1], const float (
     return sweeps;
 }
 
-// V update of one voxel, potts-min-partition.cpp:397-480.
-//   xo[i][a]   Xi at x;  xp[a][i] Xi at x - e_a (used when px[a])
-//   v[i]       in: V at x, out: projected V;  vbar[i] out: 2 V+ - V
-// Returns sum_k (V+_k - V_k)^2 over all K components.
+// Input of the simplex projection of the V step, potts-min-partition.cpp:397-467:
+// q = V + tau1 (div Xi - lambda (u - c)^2) over all K labels; vf = V with its
+// implicit K-th component.
 template <int K>
-__device__ __forceinline__ float v_update(const float (&xo)[K - 1][3], const float (&xp)[3][K - 1],
-                                          const bool (&px)[3], const bool (&nx)[3], float (&v)[K - 1],
-                                          float (&vbar)[K - 1], float u, const float (&c)[K],
-                                          float lambda, float tau1)
+__device__ __forceinline__ void v_input(const float (&xo)[K - 1][3], const float (&xp)[3][K - 1],
+                                        const bool (&px)[3], const bool (&nx)[3], const float (&v)[K - 1],
+                                        float u, const float (&c)[K], float lambda, float tau1, float (&vf)[K],
+                                        float (&q)[K])
 {
     float xof[K][3], xpf[3][K];
 #pragma unroll
@@ -268,7 +276,6 @@ __device__ __forceinline__ float v_update(const float (&xo)[K - 1][3], const flo
         xof[K - 1][a] = xk;
         xpf[a][K - 1] = xkp;
     }
-    float vf[K];
     float vk = 1.0f;
 #pragma unroll
     for (int i = 0; i < K - 1; ++i) {
@@ -276,7 +283,6 @@ __device__ __forceinline__ float v_update(const float (&xo)[K - 1][3], const flo
         vk = vk - v[i];
     }
     vf[K - 1] = vk;
-    float q[K];
 #pragma unroll
     for (int i = 0; i < K; ++i) {
         float div = 0.0f;
@@ -289,6 +295,20 @@ __device__ __forceinline__ float v_update(const float (&xo)[K - 1][3], const flo
         const float d = u - c[i];
         q[i] = __fmaf_rn(tau1, __fmaf_rn(-lambda, d * d, div), vf[i]);
     }
+}
+
+// V update of one voxel, potts-min-partition.cpp:397-480.
+//   xo[i][a]   Xi at x;  xp[a][i] Xi at x - e_a (used when px[a])
+//   v[i]       in: V at x, out: projected V;  vbar[i] out: 2 V+ - V
+// Returns sum_k (V+_k - V_k)^2 over all K components.
+template <int K>
+__device__ __forceinline__ float v_update(const float (&xo)[K - 1][3], const float (&xp)[3][K - 1],
+                                          const bool (&px)[3], const bool (&nx)[3], float (&v)[K - 1],
+                                          float (&vbar)[K - 1], float u, const float (&c)[K],
+                                          float lambda, float tau1)
+{
+    float vf[K], q[K];
+    v_input<K>(xo, xp, px, nx, v, u, c, lambda, tau1, vf, q);
     project_simplex<K>(q);
 #pragma unroll
     for (int i = 0; i < K - 1; ++i) {
