@@ -84,6 +84,27 @@ def cpu_baseline(ph, c, tau, iters: int, size: int) -> dict:
                       f"FP32 host mirror of the reference update ({dt:.1f} s)"}
 
 
+def adjoint_ratio(S) -> dict:
+    """Reverse pass / forward pass of the adjoint on the 128^3 workload (K = 3,
+    200 iterations, checkpoints every 16; median of 3 runs)."""
+    from segmentctscans_amd import Field
+    from segmentctscans_amd.phantom import make_phantom
+    tau = float(np.float32(1.0 / np.sqrt(12.0)))
+    ph = make_phantom(128, 128, 4.0, 12.0, seed=5)
+    ctx = S.make_context(0)
+    g = S.upload(ctx, ph.field)
+    t = S.upload(ctx, Field(ph.field.mapping, ph.labels.astype(np.float32)))
+    c = ph.normalized_labels()
+    runs = [S.potts_adjoint(ctx, g, t, c, 30.0, tau, tau, 200, checkpoint_every=16, want_grad_u=False)
+            for _ in range(4)][1:]
+    fw = float(np.median([r["forward_ms"] for r in runs]))
+    rv = float(np.median([r["backward_ms"] for r in runs]))
+    g.close()
+    t.close()
+    return {"workload": "128^3 cylinder (1,334,784 voxels), K=3, 200 iterations, checkpoints every 16",
+            "forward_ms": fw, "reverse_ms": rv, "ratio": rv / fw, "ratio_total": (fw + rv) / fw}
+
+
 def copy_calibration(mib: int) -> int:
     """A known-size copy: FETCH_SIZE and WRITE_SIZE of its kernel should both read mib MiB."""
     import ctypes as C
@@ -176,6 +197,7 @@ def main() -> int:
     ap.add_argument("--cpu-size", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-storage comparison run")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing run")
     ap.add_argument("--bf16-steps", type=int, default=30)
     ap.add_argument("--batch", action="store_true",
                     help="run the batched workload (576 solves) instead of the single-volume one")
@@ -296,6 +318,10 @@ def main() -> int:
                                   "steps": args.bf16_steps}
         log(f"BF16 storage: {ms_bf16:.3f} ms/step, speedup {ms_step / ms_bf16:.3f}")
     g.close()
+    if not args.no_adjoint:
+        result["adjoint"] = adjoint_ratio(S)
+        log(f"adjoint: forward {result['adjoint']['forward_ms']:.2f} ms, reverse "
+            f"{result['adjoint']['reverse_ms']:.2f} ms, ratio {result['adjoint']['ratio']:.2f}")
     if not args.no_cpu_baseline:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(ph, c, tau, args.cpu_iters, args.cpu_size)

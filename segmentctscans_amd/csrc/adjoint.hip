@@ -40,6 +40,7 @@ using namespace sct::tp;
 namespace {
 
 constexpr int kAdjThreads = 256;
+constexpr int kQueues = 64;   // sub-queues of the Dykstra replay work list
 
 // Work layout of the adjoint: state = V (K-1) | V-bar (K-1) | Xi 3(K-1)
 // components of wspan floats each.
@@ -150,13 +151,15 @@ __device__ __forceinline__ void vbar_adjoint(const Voxel &vx, uint64_t wspan, co
 }
 
 // A: through V-bar+ = 2 V+ - V and V+ = Pi_S(q).  gs = gV | gXi.
+// Also empties the replay queue of the following B1 launch.
 template <int K>
 __global__ __launch_bounds__(kAdjThreads) void adj_v_kernel(VolumeLines L, uint64_t wspan, const float *st,
                                                             const float *st1, const float *u, SolveParams<K> sp,
                                                             float tau1, const float *GP, float *gs, float *GQ,
-                                                            float *gu, double *acc)
+                                                            float *gu, double *acc, uint32_t *queue_count)
 {
     constexpr int KM = K - 1;
+    if (blockIdx.x == 0 && threadIdx.x < kQueues) queue_count[threadIdx.x] = 0;
     const Voxel vx = voxel_of(L);
     double part[K + 1] = {};
     if (vx.ok) {
@@ -257,60 +260,224 @@ __device__ __forceinline__ void dykstra_back(int sweeps, const float (*hist)[K *
             }
 }
 
-// B: div's adjoint (-grad of tau1 GQ), then back through Pi_K.
+// dykstra_back with the sweep loop unrolled (constant history indices) and
+// the history of sweep s - 1 loaded while sweep s is back-propagated, so the
+// replay waits for one load latency per sweep, not one per pair step.
+template <int K>
+__device__ __forceinline__ void dykstra_back_unrolled(int sweeps, const float (&hist)[20][K * (K - 1) / 2][3],
+                                                      float (&g)[K][3], DykstraLimits lim)
+{
+    constexpr int NP = K * (K - 1) / 2;
+    float gr[NP][3], hn[NP][3];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) gr[p][0] = gr[p][1] = gr[p][2] = 0.0f;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) hn[p][0] = hn[p][1] = hn[p][2] = 0.0f;
+    if (sweeps >= 20) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) hn[p][a] = hist[19][p][a];
+    }
+#pragma unroll
+    for (int s = 19; s >= 0; --s) {
+        if (s >= sweeps) {
+            if (s - 1 >= 0 && s - 1 < sweeps) {
+#pragma unroll
+                for (int p = 0; p < NP; ++p)
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) hn[p][a] = hist[s - 1][p][a];
+            }
+            continue;
+        }
+        float hc[NP][3];
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) hc[p][a] = hn[p][a];
+        if (s - 1 >= 0) {
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) hn[p][a] = hist[s - 1][p][a];
+        }
+#pragma unroll
+        for (int i1 = K - 2; i1 >= 0; --i1)
+#pragma unroll
+            for (int i2 = K - 1; i2 > i1; --i2) {
+                const int p = (2 * K - 1 - i1) * i1 / 2 + i2 - i1 - 1;
+                const float h0 = hc[p][0], h1 = hc[p][1], h2 = hc[p][2];
+                const float ss = __fmaf_rn(h2, h2, __fmaf_rn(h1, h1, h0 * h0));
+                float gd[3], ge[3], gh[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    gd[a] = g[i1][a] - g[i2][a];
+                    ge[a] = 0.5f * gd[a] + gr[p][a];
+                }
+                if (ss > lim.t1) {
+                    const float l = sqrtf(ss), il = 1.0f / l;
+                    const float hg = h0 * ge[0] + h1 * ge[1] + h2 * ge[2];
+                    const float k = hg * il * il * il, s1 = 1.0f - il;
+                    gh[0] = s1 * ge[0] + k * h0;
+                    gh[1] = s1 * ge[1] + k * h1;
+                    gh[2] = s1 * ge[2] + k * h2;
+                }
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    g[i1][a] -= gh[a];
+                    g[i2][a] += gh[a];
+                    gr[p][a] = -0.5f * gd[a] + gh[a];
+                }
+            }
+    }
+}
+
+// Writes the Xi-step adjoints of a voxel: g = adjoint of Dykstra's input x.
+template <int K>
+__device__ __forceinline__ void xi_outputs(uint64_t wspan, uint32_t w, float tau2, const float (&g)[K][3],
+                                           float *gXi, float *GP)
+{
+    for (int i = 0; i < K - 1; ++i)
+        for (int a = 0; a < 3; ++a) gXi[uint64_t(3 * i + a) * wspan + w] = g[i][a] - g[K - 1][a];
+    for (int i = 0; i < K; ++i)
+        for (int a = 0; a < 3; ++a) GP[uint64_t(3 * i + a) * wspan + w] = tau2 * g[i][a];
+}
+
+// Voxels whose Dykstra replay is needed, appended by workgroup b to sub-queue
+// b % kQueues (one atomic per workgroup); an item is x (K x 3), the output
+// adjoint g (K x 3) and the voxel's work offset.
+struct XiQueue {
+    float *items;
+    uint32_t *count;   // [kQueues]
+    uint32_t cap;      // items per sub-queue
+};
+template <int K>
+constexpr int item_floats() { return 6 * K + 1; }
+
+// B1: div's adjoint (-grad of tau1 GQ); voxels whose Dykstra input is feasible
+// (one sweep, no change) pass the adjoint through Pi_K, the others are queued.
 template <int K>
 __global__ __launch_bounds__(kAdjThreads) void adj_xi_kernel(VolumeLines L, uint64_t wspan, const float *st,
                                                              float tau1, float tau2, DykstraLimits lim,
-                                                             const float *GQ, float *gs, float *GP)
+                                                             const float *GQ, float *gs, float *GP, XiQueue Q)
 {
-    constexpr int KM = K - 1, NP = K * (K - 1) / 2;
+    constexpr int KM = K - 1, IF = item_floats<K>();
+    __shared__ int nq;
+    __shared__ uint32_t base;
+    if (threadIdx.x == 0) nq = 0;
+    __syncthreads();
     const Voxel vx = voxel_of(L);
-    if (!vx.ok) return;
-    const Chunk &ch = vx.ch;
-    const uint32_t lane = threadIdx.x % kWave;
-    const bool nx[3] = {vx.n + 1 < ch.len, ch.nr != kNone, ch.nc != kNone};
-    const uint32_t wn[3] = {vx.w + 1, ch.nr + lane, ch.nc + lane};
-    float full[K][3];
-    for (int i = 0; i < K; ++i) {
-        const float own = GQ[uint64_t(i) * wspan + vx.w];
-        for (int a = 0; a < 3; ++a) full[i][a] = nx[a] ? -tau1 * (GQ[uint64_t(i) * wspan + wn[a]] - own) : 0.0f;
-    }
     float *gXi = gs + uint64_t(KM) * wspan;
-    float g[K][3];
-    for (int a = 0; a < 3; ++a) {
-        for (int i = 0; i < KM; ++i) g[i][a] = gXi[uint64_t(3 * i + a) * wspan + vx.w] + full[i][a] - full[KM][a];
-        g[KM][a] = 0.0f;
-    }
-    // recompute x = Xi + tau2 grad V-bar as the solver does
-    const float *Vb = st + uint64_t(KM) * wspan, *Xi = st + uint64_t(2 * KM) * wspan;
-    float vb[KM], vbn[3][KM], xo[KM][3];
-    for (int i = 0; i < KM; ++i) {
-        vb[i] = Vb[uint64_t(i) * wspan + vx.w];
-        for (int a = 0; a < 3; ++a) {
-            vbn[a][i] = nx[a] ? Vb[uint64_t(i) * wspan + wn[a]] : 0.0f;
-            xo[i][a] = Xi[uint64_t(3 * i + a) * wspan + vx.w];
+    bool queued = false;
+    float x[K][3], g[K][3];
+    if (vx.ok) {
+        const Chunk &ch = vx.ch;
+        const uint32_t lane = threadIdx.x % kWave;
+        const bool nx[3] = {vx.n + 1 < ch.len, ch.nr != kNone, ch.nc != kNone};
+        const uint32_t wn[3] = {vx.w + 1, ch.nr + lane, ch.nc + lane};
+        float full[K][3];
+        for (int i = 0; i < K; ++i) {
+            const float own = GQ[uint64_t(i) * wspan + vx.w];
+            for (int a = 0; a < 3; ++a) full[i][a] = nx[a] ? -tau1 * (GQ[uint64_t(i) * wspan + wn[a]] - own) : 0.0f;
         }
+        for (int a = 0; a < 3; ++a) {
+            for (int i = 0; i < KM; ++i) g[i][a] = gXi[uint64_t(3 * i + a) * wspan + vx.w] + full[i][a] - full[KM][a];
+            g[KM][a] = 0.0f;
+        }
+        // recompute x = Xi + tau2 grad V-bar as the solver does
+        const float *Vb = st + uint64_t(KM) * wspan, *Xi = st + uint64_t(2 * KM) * wspan;
+        float vb[KM], vbn[3][KM], xo[KM][3];
+        for (int i = 0; i < KM; ++i) {
+            vb[i] = Vb[uint64_t(i) * wspan + vx.w];
+            for (int a = 0; a < 3; ++a) {
+                vbn[a][i] = nx[a] ? Vb[uint64_t(i) * wspan + wn[a]] : 0.0f;
+                xo[i][a] = Xi[uint64_t(3 * i + a) * wspan + vx.w];
+            }
+        }
+        xi_input<K>(vb, vbn, nx, xo, tau2, x);
+        float x1[K][3];
+        for (int i = 0; i < K; ++i)
+            for (int a = 0; a < 3; ++a) x1[i][a] = x[i][a];
+        queued = !dykstra_first_sweep_done<K>(x1, lim);
+        if (!queued) xi_outputs<K>(wspan, vx.w, tau2, g, gXi, GP);
     }
-    float x[K][3];
-    xi_input<K>(vb, vbn, nx, xo, tau2, x);
-    float x1[K][3];
-    for (int i = 0; i < K; ++i)
-        for (int a = 0; a < 3; ++a) x1[i][a] = x[i][a];
-    // A feasible input leaves Dykstra after one sweep without change: identity.
-    if (!dykstra_first_sweep_done<K>(x1, lim)) {
-        float hist[20][NP][3];
-        const int sw = dykstra_replay<K>(x, hist, lim);
-        dykstra_back<K>(sw, hist, g, lim);
+    const int lane = threadIdx.x % kWave;
+    const uint64_t mask = __ballot(queued);
+    int p = 0;
+    if (mask) {
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        if (lane == leader) p = atomicAdd(&nq, __popcll(mask));
+        p = __shfl(p, leader, kWave) + __popcll(mask & ((1ull << lane) - 1ull));
     }
-    for (int i = 0; i < KM; ++i)
-        for (int a = 0; a < 3; ++a) gXi[uint64_t(3 * i + a) * wspan + vx.w] = g[i][a] - g[KM][a];
-    for (int i = 0; i < K; ++i)
-        for (int a = 0; a < 3; ++a) GP[uint64_t(3 * i + a) * wspan + vx.w] = tau2 * g[i][a];
+    __syncthreads();   // block count known
+    const uint32_t qi = blockIdx.x % kQueues;
+    if (threadIdx.x == 0) base = nq ? atomicAdd(&Q.count[qi], uint32_t(nq)) : 0u;
+    __syncthreads();
+    if (queued) {
+        float *it = Q.items + (uint64_t(qi) * Q.cap + base + uint32_t(p)) * IF;
+        for (int i = 0; i < K; ++i)
+            for (int a = 0; a < 3; ++a) {
+                it[3 * i + a] = x[i][a];
+                it[3 * K + 3 * i + a] = g[i][a];
+            }
+        it[6 * K] = __uint_as_float(vx.w);
+    }
+}
+
+// B2: the queued voxels, one per lane: Dykstra replayed from x recording each
+// pair's h, then back-propagated.
+template <int K>
+__global__ __launch_bounds__(kAdjThreads) void adj_xi_replay_kernel(uint64_t wspan, float tau2, DykstraLimits lim,
+                                                                    float *gs, float *GP, XiQueue Q)
+{
+    constexpr int KM = K - 1, NP = K * (K - 1) / 2, IF = item_floats<K>();
+    __shared__ uint32_t start[kQueues + 1];
+    if (threadIdx.x == 0) {
+        start[0] = 0;
+        for (int q = 0; q < kQueues; ++q) start[q + 1] = start[q] + min(Q.count[q], Q.cap);
+    }
+    __syncthreads();
+    const uint32_t total = start[kQueues];
+    float *gXi = gs + uint64_t(KM) * wspan;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        int q = 0;
+        while (start[q + 1] <= t) ++q;
+        const float *it = Q.items + (uint64_t(q) * Q.cap + (t - start[q])) * IF;
+        float x[K][3], g[K][3];
+        for (int i = 0; i < K; ++i)
+            for (int a = 0; a < 3; ++a) {
+                x[i][a] = it[3 * i + a];
+                g[i][a] = it[3 * K + 3 * i + a];
+            }
+        if constexpr (NP <= 3) {
+            // whole history in registers: both sweep loops fully unrolled
+            float hist[20][NP][3], r[NP][3];
+#pragma unroll
+            for (int pp = 0; pp < NP; ++pp) r[pp][0] = r[pp][1] = r[pp][2] = 0.0f;
+            int sw = 0;
+            bool run = true;
+#pragma unroll
+            for (int s = 0; s < 20; ++s) {
+                if (run) {
+                    const float m2 = dykstra_sweep<K>(x, r, lim, hist[s]);
+                    sw = s + 1;
+                    run = !(m2 < lim.stol);
+                }
+            }
+            dykstra_back_unrolled<K>(sw, hist, g, lim);
+        } else {
+            float hist[20][NP][3];
+            const int sw = dykstra_replay<K>(x, hist, lim);
+            dykstra_back<K>(sw, hist, g, lim);
+        }
+        xi_outputs<K>(wspan, __float_as_uint(it[6 * K]), tau2, g, gXi, GP);
+    }
 }
 
 struct Buffers {
     std::vector<float *> ck, seg;
-    float *gs = nullptr, *GQ = nullptr, *GP = nullptr, *gu = nullptr;
+    float *gs = nullptr, *GQ = nullptr, *GP = nullptr, *gu = nullptr, *items = nullptr;
+    uint32_t *count = nullptr;
     double *acc = nullptr;   // dc[K], dlambda, loss
 };
 
@@ -318,7 +485,9 @@ void free_buffers(sct_ctx *ctx, Buffers &b)
 {
     for (float *p : b.ck) sct_free(ctx, p);
     for (float *p : b.seg) sct_free(ctx, p);
-    for (void *p : {(void *)b.gs, (void *)b.GQ, (void *)b.GP, (void *)b.gu, (void *)b.acc}) sct_free(ctx, p);
+    for (void *p : {(void *)b.gs, (void *)b.GQ, (void *)b.GP, (void *)b.gu, (void *)b.items, (void *)b.count,
+                    (void *)b.acc})
+        sct_free(ctx, p);
     b = Buffers{};
 }
 
@@ -354,6 +523,13 @@ int run_adjoint(sct_ctx *ctx, const sct_field *g, const sct_field *target, const
     // per-workgroup sums: dc[K], dlambda of every A launch, then the loss
     const uint64_t nacc = uint64_t(agrid) * (K + 2);
     if (!rc) rc = sct_alloc(ctx, (void **)&B.acc, sizeof(double) * std::max<uint64_t>(1, nacc));
+    // replay queue: a sub-queue serves every kQueues-th workgroup of <= 256 voxels
+    XiQueue Q{nullptr, nullptr, ((agrid + kQueues - 1) / kQueues) * uint32_t(kAdjThreads)};
+    alloc(B.items, uint64_t(kQueues) * Q.cap * item_floats<K>());
+    if (!rc) rc = sct_alloc(ctx, (void **)&B.count, sizeof(uint32_t) * kQueues);
+    Q.items = B.items;
+    Q.count = B.count;
+    const uint32_t rgrid = std::max<uint32_t>(1, ctx->cu_count * 4);
     hipEvent_t ev[4] = {};
     for (auto &e : ev)
         if (!rc && hipEventCreate(&e) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventCreate failed");
@@ -396,6 +572,7 @@ int run_adjoint(sct_ctx *ctx, const sct_field *g, const sct_field *target, const
     chk(hipMemsetAsync(B.gu, 0, sizeof(float) * S, st), "memset");
     chk(hipMemsetAsync(B.GP, 0, sizeof(float) * 3 * K * S, st), "memset");
     chk(hipMemsetAsync(B.acc, 0, sizeof(double) * std::max<uint64_t>(1, nacc), st), "memset");
+    chk(hipMemsetAsync(B.count, 0, sizeof(uint32_t) * kQueues, st), "memset");
     const float wgt = float(1.0 / (double(std::max<uint64_t>(1, geo.data_size)) * K));
     adj_loss_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, cur, (const float *)target->data, wgt, B.gs,
                                                        B.acc + uint64_t(agrid) * (K + 1));
@@ -411,8 +588,9 @@ int run_adjoint(sct_ctx *ctx, const sct_field *g, const sct_field *target, const
         for (int t = t1 - 1; t >= t0; --t) {
             const float *s0 = B.seg[size_t(t - t0)], *s1 = B.seg[size_t(t - t0) + 1];
             adj_v_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, s1, u, prm.p, tau1, B.GP, B.gs, B.GQ, B.gu,
-                                                            B.acc);
-            adj_xi_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, tau1, tau2, lim, B.GQ, B.gs, B.GP);
+                                                            B.acc, B.count);
+            adj_xi_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, tau1, tau2, lim, B.GQ, B.gs, B.GP, Q);
+            adj_xi_replay_kernel<K><<<rgrid, kAdjThreads, 0, st>>>(S, tau2, lim, B.gs, B.GP, Q);
         }
     }
     chk(hipEventRecord(ev[3], st), "hipEventRecord");
