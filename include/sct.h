@@ -18,6 +18,8 @@
  *   include/min_max.h:7                      min_max
  *   include/grad_magn.h:5-6                  grad_magn
  *   uv-hist/uv-hist.h:18-44                  uv_hist (both overloads)
+ *   transpose/convert_to_fp.h:8-10           convert_to_fp<uint16_t | float>
+ *   transpose/find_intensity_range.h:8-9     find_intensity_range
  *
  * Threading: one HIP stream per context; a context is not thread-safe.
  * Every call that returns host values synchronises the stream, as the
@@ -188,6 +190,18 @@ int sct_uv_hist(sct_ctx *ctx, const sct_field *u, const sct_field *v,
  * ranges[4] = {u_start, u_end, v_start, v_end}. */
 int sct_uv_hist_auto(sct_ctx *ctx, const sct_field *u, const sct_field *v, uint32_t nbins,
                      uint32_t *counts, uint32_t *stride, float *ranges);
+
+/* ---- ingest: transpose/ ------------------------------------------------- */
+/* convert_to_fp<ElementType>(ctx, in, lo, hi), convert_to_fp.h:8-10 with
+ * ElementType uint16_t (SCT_U16) or float (SCT_F32): *out receives a new float
+ * field, v <= lo -> 0, v >= hi -> 1, else (v - lo) / (hi - lo); padding 0.
+ * Asynchronous, as the reference. */
+int sct_convert_to_fp(sct_ctx *ctx, const sct_field *in, float lo, float hi, sct_field **out);
+/* find_intensity_range(ctx, field, tail_thr), find_intensity_range.h:8-9: the
+ * 65536-bin histogram of a u16 field and its tail cut; hist (optional)
+ * receives the 65536 counts.  Synchronises. */
+int sct_find_intensity_range(sct_ctx *ctx, const sct_field *field, double tail_thr, uint16_t *lo,
+                             uint16_t *hi, uint64_t *hist);
 
 /* ---- batched Potts (no counterpart in the reference, whose driver solves one
  * volume per call, potts-model.cpp:164) ------------------------------------ */

@@ -12,6 +12,8 @@
 //   include/min_max.h:7                    min_max
 //   include/grad_magn.h:5-6                grad_magn
 //   uv-hist/uv-hist.h:11-44                uv_hist, UVHistReturn
+//   transpose/convert_to_fp.h:8-10         convert_to_fp<uint16_t | float>
+//   transpose/find_intensity_range.h:8-9   find_intensity_range
 //
 // Differences a caller sees: the context type is HipContext; Field<T>::view()
 // returns a CylinderView (element access view(n, row, col)) in place of a
@@ -264,6 +266,27 @@ inline device_field_ptr<float> grad_magn(HipContext &ctx, DeviceField<float> &u)
     sct_field *v = nullptr;
     sct_detail::check(ctx.get(), sct_grad_magn(ctx.get(), u.handle(), &v), "grad_magn");
     return device_field_ptr<float>{new DeviceField<float>{v, u.mapping()}};
+}
+
+// convert_to_fp.h:8-10 (ElementType uint16_t or float).
+template <typename ElementType>
+device_field_ptr<float> convert_to_fp(HipContext &ctx, const DeviceField<ElementType> &in, float lo, float hi)
+{
+    static_assert(std::is_same_v<ElementType, uint16_t> || std::is_same_v<ElementType, float>,
+                  "convert_to_fp takes uint16_t or float fields");
+    sct_field *out = nullptr;
+    sct_detail::check(ctx.get(), sct_convert_to_fp(ctx.get(), in.handle(), lo, hi, &out), "convert_to_fp");
+    return device_field_ptr<float>{new DeviceField<float>{out, in.mapping()}};
+}
+
+// find_intensity_range.h:8-9.
+inline std::tuple<uint16_t, uint16_t> find_intensity_range(HipContext &ctx, DeviceField<uint16_t> &field,
+                                                           double tail_thr)
+{
+    uint16_t lo = 0, hi = 0;
+    sct_detail::check(ctx.get(), sct_find_intensity_range(ctx.get(), field.handle(), tail_thr, &lo, &hi, nullptr),
+                      "find_intensity_range");
+    return {lo, hi};
 }
 
 // uv-hist.h:11-16: histogram rows of `stride` elements.

@@ -62,6 +62,10 @@ def lib() -> C.CDLL:
         L.or_adj_grad_d.argtypes = [C.POINTER(OrLayout), f64p, f32p, C.c_int, f64p, C.c_double, C.c_double,
                                     C.c_double, C.c_int, i8p, C.c_double, C.c_int, f64p, f64p, f64p]
         L.or_adj_grad_d.restype = C.c_double
+        u16p = C.POINTER(C.c_uint16)
+        L.or_u16_hist.argtypes = [C.POINTER(OrLayout), u16p, C.POINTER(C.c_uint64)]
+        L.or_find_intensity_range.argtypes = [C.POINTER(OrLayout), u16p, C.c_double, u16p, u16p]
+        L.or_convert_to_fp.argtypes = [C.POINTER(OrLayout), C.c_void_p, C.c_int, C.c_float, C.c_float, f32p]
         L.or_bf16_round.argtypes = [C.c_float]
         L.or_bf16_round.restype = C.c_float
         L.or_f16_round.argtypes = [C.c_float]
@@ -198,6 +202,32 @@ def adj_grad(layout: Layout, u, target, c, lam, tau1, tau2, iters, *, init=None,
                                tau2, iters, None if init is None else init.ctypes.data_as(C.POINTER(C.c_int8)), tol,
                                ck, _ptr(gc, f64p), C.byref(gl), _ptr(gu, f64p))
     return loss, gc, gl.value, gu
+
+
+def u16_hist(layout: Layout, u) -> np.ndarray:
+    u = np.ascontiguousarray(u, np.uint16)
+    h = np.zeros(65536, np.uint64)
+    lay = layout.c()
+    lib().or_u16_hist(C.byref(lay), u.ctypes.data_as(C.POINTER(C.c_uint16)), h.ctypes.data_as(C.POINTER(C.c_uint64)))
+    return h
+
+
+def find_intensity_range(layout: Layout, u, tail: float) -> tuple[int, int]:
+    u = np.ascontiguousarray(u, np.uint16)
+    lo, hi = C.c_uint16(), C.c_uint16()
+    lay = layout.c()
+    lib().or_find_intensity_range(C.byref(lay), u.ctypes.data_as(C.POINTER(C.c_uint16)), tail, C.byref(lo),
+                                  C.byref(hi))
+    return lo.value, hi.value
+
+
+def convert_to_fp(layout: Layout, x, lo: float, hi: float) -> np.ndarray:
+    x = np.ascontiguousarray(x)
+    assert x.dtype in (np.uint16, np.float32)
+    out = np.zeros(layout.span, np.float32)
+    lay = layout.c()
+    lib().or_convert_to_fp(C.byref(lay), x.ctypes.data, int(x.dtype == np.uint16), lo, hi, _ptr(out, f32p))
+    return out
 
 
 def store_round(x, storage: str) -> np.ndarray:

@@ -830,3 +830,40 @@ double or_adj_grad_d(const or_layout *m, const double *u, const float *target, i
     free(cks); free(seg); free(gs); free(GQ); free(GP); free(gu);
     return loss;
 }
+
+/* ---- ingest: transpose/convert_to_fp.cpp:11-46, find_intensity_range.cpp:22-110 */
+void or_u16_hist(const or_layout *m, const uint16_t *u, uint64_t *hist)
+{
+    memset(hist, 0, sizeof(uint64_t) * 65536);
+    for (uint32_t col = m->col_begin; col < m->col_end; ++col)
+        for (uint32_t row = row_begin(m, col); row < row_end(m, col); ++row)
+            for (uint32_t n = 0; n < m->n_images; ++n) ++hist[u[line_of(m, row, col) * m->nstride + n]];
+}
+
+void or_find_intensity_range(const or_layout *m, const uint16_t *u, double tail, uint16_t *lo, uint16_t *hi)
+{
+    uint64_t *h = (uint64_t *)malloc(sizeof(uint64_t) * 65536), total = 0, cum = 0;
+    or_u16_hist(m, u, h);
+    for (int i = 0; i < 65536; ++i) total += h[i];
+    const double pl = tail * (double)total, ph = (1.0 - tail) * (double)total;
+    size_t l = 0, r = 65536;
+    for (size_t i = 0; i < 65536; ++i) {
+        cum += h[i];
+        if (pl >= (double)cum) l = i;
+        if (ph <= (double)cum) { r = i; break; }
+    }
+    *lo = (uint16_t)l;
+    *hi = (uint16_t)r;
+    free(h);
+}
+
+void or_convert_to_fp(const or_layout *m, const void *in, int is_u16, float lo, float hi, float *out)
+{
+    for (uint32_t col = m->col_begin; col < m->col_end; ++col)
+        for (uint32_t row = row_begin(m, col); row < row_end(m, col); ++row)
+            for (uint32_t n = 0; n < m->n_images; ++n) {
+                size_t o = line_of(m, row, col) * m->nstride + n;
+                float v = is_u16 ? (float)((const uint16_t *)in)[o] : ((const float *)in)[o];
+                out[o] = v <= lo ? 0.0f : (v >= hi ? 1.0f : (v - lo) / (hi - lo));
+            }
+}

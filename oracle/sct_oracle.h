@@ -102,6 +102,11 @@ double or_adj_grad_d(const or_layout *m, const double *u, const float *target, i
                      double lambda, double tau1, double tau2, int iters, const int8_t *init, double tol,
                      int ck, double *grad_c, double *grad_lambda, double *grad_u);
 
+/* Ingest (transpose/): u16 histogram [65536], tail cut, conversion to float. */
+void or_u16_hist(const or_layout *m, const uint16_t *u, uint64_t *hist);
+void or_find_intensity_range(const or_layout *m, const uint16_t *u, double tail, uint16_t *lo, uint16_t *hi);
+void or_convert_to_fp(const or_layout *m, const void *in, int is_u16, float lo, float hi, float *out);
+
 float or_bf16_round(float f);
 float or_f16_round(float f);
 

@@ -314,6 +314,23 @@ def potts_min_partition(ctx: Context, g: DeviceField, ci, lam: float, tau1: floa
     return (res, *extra) if extra else res
 
 
+def convert_to_fp(ctx: Context, f: DeviceField, lo: float, hi: float) -> DeviceField:
+    """convert_to_fp<ElementType>(ctx, in, lo, hi), convert_to_fp.cpp:48-71."""
+    h = C.c_void_p()
+    ctx.check(N.lib().sct_convert_to_fp(ctx.handle, f.handle, lo, hi, C.byref(h)))
+    return DeviceField(ctx, h, f.mapping)
+
+
+def find_intensity_range(ctx: Context, f: DeviceField, tail_thr: float, *, hist: bool = False):
+    """find_intensity_range(ctx, field, tail_thr), find_intensity_range.cpp:63-110.
+    Returns (lo, hi), or (lo, hi, counts[65536]) with hist=True."""
+    lo, hi = C.c_uint16(), C.c_uint16()
+    h = np.zeros(65536, np.uint64) if hist else None
+    ctx.check(N.lib().sct_find_intensity_range(ctx.handle, f.handle, tail_thr, C.byref(lo), C.byref(hi),
+                                               h.ctypes.data_as(C.POINTER(C.c_uint64)) if hist else None))
+    return (lo.value, hi.value, h) if hist else (lo.value, hi.value)
+
+
 def potts_batch(ctx: Context, problems, tau1: float, tau2: float, iters: int, *, storage="fp32",
                 graph=True) -> tuple[list, float]:
     """Many Potts solves in shared launches (no counterpart in the reference).

@@ -71,6 +71,9 @@ SIGNATURES = {
                                           C.c_float, C.POINTER(SctPottsOpts), C.POINTER(C.c_void_p)]),
     "sct_potts_solver_iterate": (C.c_int, [C.c_void_p, C.c_int]),
     "sct_potts_solver_kernels": (C.c_int, [C.c_void_p]),
+    "sct_convert_to_fp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.POINTER(C.c_void_p)]),
+    "sct_find_intensity_range": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_uint16),
+                                           C.POINTER(C.c_uint16), C.POINTER(C.c_uint64)]),
     "sct_potts_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, f32p, C.c_float, C.c_float,
                                     C.c_float, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double), C.POINTER(C.c_void_p), f32p]),

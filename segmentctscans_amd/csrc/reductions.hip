@@ -415,3 +415,154 @@ extern "C" int sct_uv_hist_auto(sct_ctx *ctx, const sct_field *u, const sct_fiel
     }
     return sct_uv_hist(ctx, u, v, u0, u1, v0, v1, nbins, counts, stride);
 }
+
+// ---------------------------------------------------------------------------
+// Ingest: the steps before the path on scanner data (transpose/).
+// ---------------------------------------------------------------------------
+namespace {
+
+// convert_to_fp_kernel, convert_to_fp.cpp:11-46: v <= lo -> 0, v >= hi -> 1,
+// else (v - lo) / (hi - lo) in float.
+template <typename T>
+__global__ __launch_bounds__(256) void convert_kernel(VoxGeom g, const T *__restrict__ in, float *__restrict__ out,
+                                                      float lo, float hi)
+{
+    const uint64_t total = uint64_t(g.lines) * g.chunks;
+    const uint64_t nwaves = uint64_t(gridDim.x) * (blockDim.x / kWave);
+    const uint32_t lane = threadIdx.x % kWave;
+    for (uint64_t w = uint64_t(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; w < total; w += nwaves) {
+        const uint32_t line = uint32_t(w / g.chunks), n = uint32_t(w % g.chunks) * kWave + lane;
+        if (n >= g.len) continue;
+        const uint64_t o = uint64_t(line) * g.stride + n;
+        const float v = float(in[o]);
+        out[o] = v <= lo ? 0.0f : (v >= hi ? 1.0f : (v - lo) / (hi - lo));
+    }
+}
+
+// histogram_kernel of find_intensity_range.cpp:22-46: 65536 bins, one per
+// u16 value.  Two 16-bit counters per LDS word; a workgroup counts at most
+// 65535 voxels (lines_per_block lines) so no half-word carries, then adds its
+// non-zero counters to the device histogram.  PACKED = false: device atomics.
+template <bool PACKED>
+__global__ __launch_bounds__(256) void u16_hist_kernel(VoxGeom g, const uint16_t *__restrict__ in,
+                                                       uint32_t lines_per_block, uint32_t *__restrict__ counts)
+{
+    extern __shared__ uint32_t sh[];
+    constexpr uint32_t nwords = 32768;
+    if constexpr (PACKED) {
+        for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) sh[i] = 0;
+        __syncthreads();
+    }
+    const uint32_t l0 = blockIdx.x * lines_per_block, l1 = min(g.lines, l0 + lines_per_block);
+    const uint32_t lane = threadIdx.x % kWave, nw = blockDim.x / kWave;
+    const uint32_t pairs = l0 < l1 ? (l1 - l0) * g.chunks : 0;
+    for (uint32_t p = threadIdx.x / kWave; p < pairs; p += nw) {
+        const uint32_t line = l0 + p / g.chunks, n = (p % g.chunks) * kWave + lane;
+        if (n >= g.len) continue;
+        const uint32_t b = in[uint64_t(line) * g.stride + n];
+        if constexpr (PACKED) atomicAdd(&sh[b >> 1], 1u << (16 * (b & 1)));
+        else atomicAdd(&counts[b], 1u);
+    }
+    if constexpr (PACKED) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) {
+            const uint32_t c = sh[i];
+            if (c & 0xffffu) atomicAdd(&counts[2 * i], c & 0xffffu);
+            if (c >> 16) atomicAdd(&counts[2 * i + 1], c >> 16);
+        }
+    }
+}
+
+}  // namespace
+
+// convert_to_fp<ElementType>(ctx, in, lo, hi), convert_to_fp.cpp:48-71
+// (ElementType uint16_t or float).  *out: new float field, padding zero.
+extern "C" int sct_convert_to_fp(sct_ctx *ctx, const sct_field *in, float lo, float hi, sct_field **out)
+{
+    if (!ctx || !in || !out) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const Geometry &geo = in->geo;
+    sct_layout l{geo.n_images, geo.height, geo.width, geo.nstride, geo.col_begin, geo.col_end,
+                 geo.start_row.data(), geo.base.data()};
+    sct_field *f = nullptr;
+    int rc = sct_field_create(ctx, &l, SCT_F32, &f);
+    if (rc) return rc;
+    const VoxGeom g = vox_geom(in);
+    hipError_t e = hipMemsetAsync(f->data, 0, sizeof(float) * geo.span(), ctx->stream);
+    if (e == hipSuccess && g.lines && g.len) {
+        if (in->dtype == SCT_U16)
+            convert_kernel<uint16_t><<<stream_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const uint16_t *)in->data,
+                                                                                (float *)f->data, lo, hi);
+        else
+            convert_kernel<float><<<stream_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const float *)in->data,
+                                                                             (float *)f->data, lo, hi);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+        sct_field_destroy(f);
+        return sct_hip_fail(ctx, e, "convert_to_fp");
+    }
+    *out = f;
+    return SCT_OK;
+}
+
+// find_intensity_range(ctx, field, tail_thr), find_intensity_range.cpp:63-110:
+// the 65536-bin histogram of a u16 field and the tail cut.  hist (optional)
+// receives the 65536 counts.  Synchronises.
+extern "C" int sct_find_intensity_range(sct_ctx *ctx, const sct_field *f, double tail_thr, uint16_t *lo,
+                                        uint16_t *hi, uint64_t *hist)
+{
+    if (!ctx || !f || !lo || !hi) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
+    if (f->dtype != SCT_U16) return sct_fail(ctx, SCT_ERR_INVALID, "u16 field required");
+    constexpr uint32_t nbins = 65536;
+    uint32_t *d = nullptr;
+    int rc = sct_alloc(ctx, (void **)&d, sizeof(uint32_t) * nbins);
+    if (rc) return rc;
+    const VoxGeom g = vox_geom(f);
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t) * nbins, ctx->stream);
+    if (e == hipSuccess && g.lines && g.len) {
+        const size_t lds = sizeof(uint32_t) * nbins / 2;
+        bool packed = g.len <= 65535 &&
+                      hipFuncSetAttribute((const void *)u16_hist_kernel<true>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) == hipSuccess;
+        if (packed) {
+            const uint32_t lpb = std::max<uint32_t>(1, 65535u / g.len);
+            u16_hist_kernel<true><<<(g.lines + lpb - 1) / lpb, 256, lds, ctx->stream>>>(g, (const uint16_t *)f->data,
+                                                                                         lpb, d);
+            if (hipGetLastError() != hipSuccess) packed = false;
+        }
+        if (!packed) {
+            (void)hipGetLastError();
+            const uint32_t blocks = std::max<uint32_t>(1, ctx->cu_count * 4);
+            const uint32_t lpb = std::max<uint32_t>(1, (g.lines + blocks - 1) / blocks);
+            e = hipMemsetAsync(d, 0, sizeof(uint32_t) * nbins, ctx->stream);
+            if (e == hipSuccess)
+                u16_hist_kernel<false><<<(g.lines + lpb - 1) / lpb, 256, 0, ctx->stream>>>(
+                    g, (const uint16_t *)f->data, lpb, d);
+            if (e == hipSuccess) e = hipGetLastError();
+        }
+    }
+    std::vector<uint32_t> h(nbins);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, sizeof(uint32_t) * nbins, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    sct_free(ctx, d);
+    if (e != hipSuccess) return sct_hip_fail(ctx, e, "find_intensity_range");
+    uint64_t total = 0;
+    for (uint32_t c : h) total += c;
+    const double pct_lo = tail_thr * double(total), pct_hi = (1.0 - tail_thr) * double(total);
+    size_t l = 0, u = nbins;
+    uint64_t cum = 0;
+    for (size_t i = 0; i < nbins; ++i) {
+        cum += h[i];
+        if (pct_lo >= double(cum)) l = i;
+        if (pct_hi <= double(cum)) {
+            u = i;
+            break;
+        }
+    }
+    *lo = uint16_t(l);
+    *hi = uint16_t(u);   // 65536 wraps to 0 as in the reference's uint16_t(hi)
+    if (hist)
+        for (uint32_t i = 0; i < nbins; ++i) hist[i] = h[i];
+    return SCT_OK;
+}
