@@ -20,6 +20,7 @@
  *   uv-hist/uv-hist.h:18-44                  uv_hist (both overloads)
  *   transpose/convert_to_fp.h:8-10           convert_to_fp<uint16_t | float>
  *   transpose/find_intensity_range.h:8-9     find_intensity_range
+ *   include/field.h:92-153                   Field<T> .bin archive save / load
  *
  * Threading: one HIP stream per context; a context is not thread-safe.
  * Every call that returns host values synchronises the stream, as the
@@ -190,6 +191,36 @@ int sct_uv_hist(sct_ctx *ctx, const sct_field *u, const sct_field *v,
  * ranges[4] = {u_start, u_end, v_start, v_end}. */
 int sct_uv_hist_auto(sct_ctx *ctx, const sct_field *u, const sct_field *v, uint32_t nbins,
                      uint32_t *counts, uint32_t *stride, float *ranges);
+
+/* ---- .bin archive (host only): Field<T>::save / load, field.h:92-153 -------
+ * One MSB-first bit stream: the mapping header of layout_cylinder.h:233-318
+ * (Exp-Golomb ue / se, io/writeprimitives.h:55-66, io/readprimitives.h:53-75),
+ * lo and hi as 32-bit patterns for float fields, then the in-mask voxels in
+ * (col, row, n) order as 16- or 32-bit values; the file is the stream's bytes,
+ * the last zero-padded (transpose/transpose.cpp:303-308).  dtype: SCT_F32
+ * (Field<float>) or SCT_U16 (Field<uint16_t>, no lo / hi). */
+typedef struct sct_bin_info {
+    uint32_t n_images, height, width, nstride, col_begin, col_end;
+    uint64_t span;          /* elements of the field (base[width])             */
+    uint64_t data_size;     /* in-mask voxels                                   */
+    float    lo, hi;        /* float fields; 0 for u16                          */
+    uint64_t header_bits;   /* stream bits before the first voxel               */
+} sct_bin_info;
+
+/* Parses the header of an archive in buf[len]; start_row[width] and
+ * base[width + 1] (optional) receive the mapping tables.  SCT_ERR_SIZE when
+ * buf is shorter than the archive it describes, SCT_ERR_INVALID when the
+ * header is not a cylinder mapping. */
+int sct_bin_parse(const void *buf, size_t len, int dtype, sct_bin_info *info, uint32_t *start_row,
+                  uint32_t *base);
+/* Reads the voxels of a parsed archive into data[span] (padding zero). */
+int sct_bin_read(const void *buf, size_t len, int dtype, const sct_bin_info *info, void *data);
+/* Upper bound of the bytes sct_bin_write needs. */
+size_t sct_bin_bound(const sct_layout *layout, int dtype);
+/* Writes the archive of data[span] (layout's mapping; lo, hi for float) into
+ * buf[cap]; *written receives its bytes. */
+int sct_bin_write(const sct_layout *layout, int dtype, float lo, float hi, const void *data, void *buf,
+                  size_t cap, size_t *written);
 
 /* ---- ingest: transpose/ ------------------------------------------------- */
 /* convert_to_fp<ElementType>(ctx, in, lo, hi), convert_to_fp.h:8-10 with

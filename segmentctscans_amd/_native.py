@@ -28,6 +28,13 @@ class SctLayout(C.Structure):
     ]
 
 
+class SctBinInfo(C.Structure):
+    _fields_ = [("n_images", C.c_uint32), ("height", C.c_uint32), ("width", C.c_uint32),
+                ("nstride", C.c_uint32), ("col_begin", C.c_uint32), ("col_end", C.c_uint32),
+                ("span", C.c_uint64), ("data_size", C.c_uint64), ("lo", C.c_float), ("hi", C.c_float),
+                ("header_bits", C.c_uint64)]
+
+
 class SctPottsProblem(C.Structure):
     _fields_ = [("g", C.c_void_p), ("K", C.c_int32), ("c", C.c_float * 6), ("lam", C.c_float)]
 
@@ -71,6 +78,11 @@ SIGNATURES = {
                                           C.c_float, C.POINTER(SctPottsOpts), C.POINTER(C.c_void_p)]),
     "sct_potts_solver_iterate": (C.c_int, [C.c_void_p, C.c_int]),
     "sct_potts_solver_kernels": (C.c_int, [C.c_void_p]),
+    "sct_bin_parse": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, u32p, u32p]),
+    "sct_bin_read": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_void_p]),
+    "sct_bin_bound": (C.c_size_t, [C.c_void_p, C.c_int]),
+    "sct_bin_write": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_size_t,
+                                C.POINTER(C.c_size_t)]),
     "sct_convert_to_fp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.POINTER(C.c_void_p)]),
     "sct_find_intensity_range": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_uint16),
                                            C.POINTER(C.c_uint16), C.POINTER(C.c_uint64)]),
