@@ -105,6 +105,91 @@ def adjoint_ratio(S) -> dict:
             "forward_ms": fw, "reverse_ms": rv, "ratio": rv / fw, "ratio_total": (fw + rv) / fw}
 
 
+TV_BYTES = 28        # R xi 3, R g, W xi 3 (FP32) per voxel and iteration
+TV_UPDATE_BYTES = 20  # final g - lambda div xi: R xi 3, R+W g
+
+
+def tv_workload(S, iters: int, cpu_iters: int) -> dict:
+    """Chambolle TV then the 65536-bin histogram on the 512^3 phantom (cylinder
+    radius 230.4, spheres U[4, 20]): for lambda = 0.04, 0.05, ..., 0.12, TV
+    from the same input (tau = 0.99 / 6, iters iterations), then calc_hist of
+    the result over the input's [min, max] with 65536 bins and no dither.
+    Each call is timed with HIP events on the library's stream; a call
+    includes zeroing both xi buffers (24 B per stored element) and the final
+    update, so the time of one iteration is (call - call with 0 iterations)
+    / iters, and its achieved bandwidth is 28 B per voxel over that time.
+    voxel_iters_per_s is over the whole call."""
+    import oracle as O
+    from segmentctscans_amd.hiptime import EventTimeline
+    from segmentctscans_amd.phantom import make_phantom
+
+    tau = float(np.float32(1.0 / 6.0) * np.float32(0.99))
+    ph = make_phantom(512, 512, 4.0, 20.0, seed=7, r=230.4)
+    m = ph.field.mapping
+    vox = m.data_size
+    ctx = S.make_context(0)
+    g0 = S.upload(ctx, ph.field)
+    mn, mx = S.min_max(ctx, g0)
+    g0.close()
+    g = S.upload(ctx, ph.field)
+    S.chambolle(ctx, g, 0.04, tau, 5)   # warm-up
+    g.close()
+    tv_ms, hist_ms, in_range = [], [], []
+    lams = [round(0.04 + 0.01 * k, 2) for k in range(9)]
+    for lam in lams:
+        g = S.upload(ctx, ph.field)
+        ctx.synchronize()
+        tl = EventTimeline(ctx.stream, 3)
+        tl.mark()
+        S.chambolle(ctx, g, lam, tau, iters)
+        tl.mark()
+        cnt = S.calc_hist(ctx, g, mn, mx, 65536, 0.0, counts=True)
+        tl.mark()
+        d = tl.durations()
+        tl.close()
+        g.close()
+        tv_ms.append(float(d[0]))
+        hist_ms.append(float(d[1]))
+        in_range.append(int(cnt.sum()))
+    # the same call with no iteration: set-up, zeroing and the final update
+    base_ms = []
+    for _ in range(3):
+        g = S.upload(ctx, ph.field)
+        ctx.synchronize()
+        tl = EventTimeline(ctx.stream, 2)
+        tl.mark()
+        S.chambolle(ctx, g, 0.08, tau, 0)
+        tl.mark()
+        base_ms.append(float(tl.durations()[0]))
+        tl.close()
+        g.close()
+    t = float(np.median(tv_ms))
+    t0 = float(np.median(base_ms))
+    it_ms = (t - t0) / iters
+    achieved = vox * TV_BYTES / (it_ms / 1e3) / 1e9
+    out = {"workload": f"512^3 cylinder r=230.4 ({vox} voxels), spheres U[4,20], lambda 0.04..0.12 x "
+                       f"{iters} iterations, tau=0.99/6; then 65536-bin calc_hist over [min, max]",
+           "voxel_iters_per_s": vox * iters / (t / 1e3),
+           "ms_per_call_median": t, "ms_per_iteration": it_ms, "ms_call_without_iterations": t0,
+           "calls_ms": tv_ms, "hist_ms": hist_ms, "hist_in_range": in_range,
+           "achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS,
+           "bytes_per_voxel_iter": TV_BYTES}
+    traffic, src = committed_traffic("tv_step_kernel", "tv_512")
+    out["traffic"], out["traffic_source"] = traffic, src
+    if cpu_iters:
+        lay = O.Layout.from_mapping(m)
+        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        cores = min(cores, 16)
+        t0 = time.perf_counter()
+        O.chambolle(lay, ph.field.data, 0.08, tau, cpu_iters, precision="f32", nthreads=cores)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": vox * cpu_iters / dt, "unit": "voxel-iters/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"the same 512^3 input, lambda 0.08, {cpu_iters} iterations, FP32 "
+                                         f"host mirror of the reference's kernels ({dt:.1f} s)"}
+    return out
+
+
 def copy_calibration(mib: int) -> int:
     """A known-size copy: FETCH_SIZE and WRITE_SIZE of its kernel should both read mib MiB."""
     import ctypes as C
@@ -199,6 +284,11 @@ def main() -> int:
     ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-storage comparison run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing run")
     ap.add_argument("--bf16-steps", type=int, default=30)
+    ap.add_argument("--no-tv", action="store_true", help="skip the Chambolle TV + histogram workload")
+    ap.add_argument("--tv-iters", type=int, default=200)
+    ap.add_argument("--tv-cpu-iters", type=int, default=10)
+    ap.add_argument("--tv-only", action="store_true",
+                    help="run only the Chambolle TV + histogram workload (512^3) and print its JSON")
     ap.add_argument("--batch", action="store_true",
                     help="run the batched workload (576 solves) instead of the single-volume one")
     ap.add_argument("--batch-volumes", type=int, default=64)
@@ -211,6 +301,12 @@ def main() -> int:
         return batch_workload(args)
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    if args.tv_only:
+        import segmentctscans_amd as S
+        r = tv_workload(S, args.tv_iters, 0 if args.no_cpu_baseline else args.tv_cpu_iters)
+        log(f"TV: {r['ms_per_iteration']:.3f} ms/iteration, {r['frac']:.3f} of HBM peak")
+        print(json.dumps(r, default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the solver has no multi-GPU path in this build "
@@ -322,6 +418,11 @@ def main() -> int:
         result["adjoint"] = adjoint_ratio(S)
         log(f"adjoint: forward {result['adjoint']['forward_ms']:.2f} ms, reverse "
             f"{result['adjoint']['reverse_ms']:.2f} ms, ratio {result['adjoint']['ratio']:.2f}")
+    if not args.no_tv:
+        result["tv"] = tv_workload(S, args.tv_iters, 0 if args.no_cpu_baseline else args.tv_cpu_iters)
+        log(f"TV: {result['tv']['ms_per_iteration']:.3f} ms/iteration, "
+            f"{result['tv']['voxel_iters_per_s']:.3e} voxel-iters/s, {result['tv']['frac']:.3f} of HBM peak; "
+            f"65536-bin histogram {np.median(result['tv']['hist_ms']):.3f} ms")
     if not args.no_cpu_baseline:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(ph, c, tau, args.cpu_iters, args.cpu_size)

@@ -145,12 +145,44 @@ int sct_geometry_build(sct_ctx *ctx, const sct_layout *l, Geometry &g)
         }
     }
     g.data_size = rows * g.n_images;
+    std::vector<uint32_t> order;
+    order.reserve(g.lines);
+    const uint32_t bands = kBandRows ? (g.height + kBandRows - 1) / kBandRows : 1;
+    for (uint32_t band = 0; band < bands; ++band) {
+        const uint32_t r0 = kBandRows ? band * kBandRows : 0, r1 = kBandRows ? r0 + kBandRows : g.height;
+        for (uint32_t col = g.col_begin; col < g.col_end; ++col) {
+            const uint32_t rb = std::max(r0, g.start_row[col]), re = std::min(r1, g.row_end(col));
+            for (uint32_t row = rb; row < re; ++row) order.push_back(uint32_t(line_of(row, col)));
+        }
+    }
+    if (order.size() != g.lines) return sct_fail(ctx, SCT_ERR_INVALID, "layout lines do not tile the mask");
+    std::vector<OrderedLine> oline(g.lines);
+    for (uint32_t k = 0; k < g.lines; ++k) {
+        const uint32_t L = order[k];
+        const LineNbr &nb = g.h_nbr[L];
+        OrderedLine &o = oline[k];
+        o = OrderedLine{int32_t(L), nb.prow, nb.nrow, nb.pcol, nb.ncol, -1, -1, 0u};
+        if (nb.nrow >= 0) {
+            const LineNbr &r = g.h_nbr[size_t(nb.nrow)];
+            o.nrow_pcol = r.pcol;
+            o.flags |= (r.nrow >= 0 ? kNrowHasNrow : 0u) | (r.ncol >= 0 ? kNrowHasNcol : 0u);
+        }
+        if (nb.ncol >= 0) {
+            const LineNbr &c = g.h_nbr[size_t(nb.ncol)];
+            o.ncol_prow = c.prow;
+            o.flags |= (c.nrow >= 0 ? kNcolHasNrow : 0u) | (c.ncol >= 0 ? kNcolHasNcol : 0u);
+        }
+    }
     int rc;
     if ((rc = sct_alloc(ctx, (void **)&g.d_nbr, sizeof(LineNbr) * std::max<size_t>(1, g.lines)))) return rc;
+    if ((rc = sct_alloc(ctx, (void **)&g.d_oline, sizeof(OrderedLine) * std::max<size_t>(1, g.lines)))) return rc;
     if ((rc = sct_alloc(ctx, (void **)&g.d_start_row, sizeof(uint32_t) * g.width))) return rc;
     if ((rc = sct_alloc(ctx, (void **)&g.d_base, sizeof(uint32_t) * (g.width + 1)))) return rc;
-    if (g.lines)
+    if (g.lines) {
         SCT_HIP(ctx, hipMemcpyAsync(g.d_nbr, g.h_nbr.data(), sizeof(LineNbr) * g.lines, hipMemcpyHostToDevice, ctx->stream));
+        SCT_HIP(ctx, hipMemcpyAsync(g.d_oline, oline.data(), sizeof(OrderedLine) * g.lines, hipMemcpyHostToDevice,
+                                    ctx->stream));
+    }
     SCT_HIP(ctx, hipMemcpyAsync(g.d_start_row, g.start_row.data(), sizeof(uint32_t) * g.width, hipMemcpyHostToDevice, ctx->stream));
     SCT_HIP(ctx, hipMemcpyAsync(g.d_base, g.base.data(), sizeof(uint32_t) * (g.width + 1), hipMemcpyHostToDevice, ctx->stream));
     SCT_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -160,9 +192,11 @@ int sct_geometry_build(sct_ctx *ctx, const sct_layout *l, Geometry &g)
 void sct_geometry_release(sct_ctx *ctx, Geometry &g)
 {
     sct_free(ctx, g.d_nbr);
+    sct_free(ctx, g.d_oline);
     sct_free(ctx, g.d_start_row);
     sct_free(ctx, g.d_base);
     g.d_nbr = nullptr;
+    g.d_oline = nullptr;
     g.d_start_row = g.d_base = nullptr;
 }
 

@@ -24,6 +24,26 @@ struct LineNbr {
     int32_t prow, nrow, pcol, ncol;
 };
 
+// Band order of the lines: the (row, col) plane is cut into bands of
+// kBandRows rows; band by band, column by column, the band's rows of that
+// column.  A stencil that walks the lines in this order finds its +col / -col
+// neighbour kBandRows lines away instead of a whole column (about 2r lines
+// for a cylinder of radius r), so the neighbour lines are still in the L2
+// that fetched them.  Only the lines at band edges have a row neighbour in
+// another band.
+constexpr uint32_t kBandRows = 32;
+
+// One line of the band order with everything a stencil reaching one line in
+// each direction, and the diagonal lines a divergence at the +row / +col line
+// needs, reads about it: one 32-byte scalar load per wave.
+struct OrderedLine {
+    int32_t line, prow, nrow, pcol, ncol;
+    int32_t nrow_pcol;   // -col line of the +row line, or -1
+    int32_t ncol_prow;   // -row line of the +col line, or -1
+    uint32_t flags;      // kNrowHasNrow | kNrowHasNcol | kNcolHasNrow | kNcolHasNcol
+};
+constexpr uint32_t kNrowHasNrow = 1, kNrowHasNcol = 2, kNcolHasNrow = 4, kNcolHasNcol = 8;
+
 struct Geometry {
     uint32_t n_images = 0, height = 0, width = 0, nstride = 0, col_begin = 0, col_end = 0;
     std::vector<uint32_t> start_row, base;   // host copies of the mapping tables
@@ -32,6 +52,7 @@ struct Geometry {
     std::vector<LineNbr> h_nbr;
     std::vector<uint32_t> line_col, line_row;
     LineNbr *d_nbr = nullptr;                // device [lines]
+    OrderedLine *d_oline = nullptr;          // device [lines], in band order
     uint32_t *d_start_row = nullptr, *d_base = nullptr;   // device mapping tables
 
     uint32_t row_end(uint32_t col) const { return start_row[col] + (base[col + 1] - base[col]) / nstride; }

@@ -40,6 +40,24 @@ def test_chambolle_bitwise_vs_fp32_mirror(ctx, ph24, lam):
     assert np.array_equal(_bits(dq), _bits(dq32))
 
 
+@pytest.mark.parametrize("n_images,nstride", [(200, None), (129, None), (37, 37), (130, 131), (256, None)])
+def test_chambolle_chunk_edges_bitwise(ctx, n_images, nstride):
+    """Lines longer than one chunk, partial chunks, odd lengths and odd
+    strides (one voxel per lane): bitwise vs the FP32 mirror, dq trace too."""
+    mp = S.Mapping.cylinder(n_images, 21, 19, 9.3, 8.7, 8.9, nstride)
+    rng = np.random.default_rng(n_images)
+    data = np.zeros(mp.span, np.float32)
+    valid = mp.valid_mask()
+    data[valid] = rng.random(int(valid.sum()), dtype=np.float32)
+    f = S.Field(mp, data)
+    g = S.upload(ctx, f)
+    dq = S.chambolle(ctx, g, 0.08, TAU_TV, 12, dq_trace=True)
+    lay = O.Layout.from_mapping(mp)
+    want, _, dq32 = O.chambolle(lay, data, 0.08, TAU_TV, 12, precision="f32")
+    assert np.array_equal(_bits(g.to_host()[valid]), _bits(want[valid]))
+    assert np.array_equal(_bits(dq), _bits(dq32))
+
+
 def test_tv_then_65536_bin_histogram_bit_exact(ctx, ph128):
     """Config 2 pipeline on a 128^3 phantom: TV at the nine weights
     0.04..0.12 (200 iterations each, from the same input), then a 65536-bin
