@@ -14,6 +14,7 @@
 //   uv-hist/uv-hist.h:11-44                uv_hist, UVHistReturn
 //   transpose/convert_to_fp.h:8-10         convert_to_fp<uint16_t | float>
 //   transpose/find_intensity_range.h:8-9   find_intensity_range
+//   include/field.h:92-153                 Field<T>::save / load as save_bin / load_bin
 //
 // Differences a caller sees: the context type is HipContext; Field<T>::view()
 // returns a CylinderView (element access view(n, row, col)) in place of a
@@ -25,11 +26,13 @@
 
 #include <array>
 #include <cstdint>
+#include <cstdio>
 #include <memory>
 #include <stdexcept>
 #include <string>
 #include <tuple>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "sct.h"
@@ -51,6 +54,16 @@ public:
             throw std::invalid_argument("invalid cylinder layout");
         if (col_begin_ == ~0u) col_begin_ = col_end_ = 0;
         data_size_ = ds;
+    }
+    // A mapping from its tables (a deserialised layout_cylinder::mapping_storage,
+    // layout_cylinder.h:233-318): start_row[width], base[width + 1].
+    CylinderMapping(uint32_t n_images, uint32_t height, uint32_t width, uint32_t nstride, uint32_t col_begin,
+                    uint32_t col_end, std::vector<uint32_t> start_row, std::vector<uint32_t> base, uint64_t data_size)
+        : n_(n_images), h_(height), w_(width), nstride_(nstride), col_begin_(col_begin), col_end_(col_end),
+          data_size_(data_size), start_row_(std::move(start_row)), base_(std::move(base))
+    {
+        if (start_row_.size() != w_ || base_.size() != size_t(w_) + 1)
+            throw std::invalid_argument("mapping tables do not match the width");
     }
 
     uint32_t operator()(uint32_t n, uint32_t row, uint32_t col) const
@@ -110,6 +123,8 @@ public:
     const CylinderMapping &mapping() const { return map_; }
     T *data() { return storage_.data(); }
     const T *data() const { return storage_.data(); }
+
+    std::vector<T> &storage() { return storage_; }
 
     void lo(float x) { lo_ = x; }
     void hi(float x) { hi_ = x; }
@@ -324,4 +339,61 @@ inline UVHistReturn uv_hist(HipContext &ctx, DeviceField<float> &u, DeviceField<
     auto [u0, u1] = min_max(ctx, u);
     auto [hist, hm] = uv_hist(ctx, u, v, u0, u1, v0, v1, nbins);
     return {hist, hm, u0, u1, v0, v1};
+}
+
+// ---- the .bin archive: Field<T>::save / load, field.h:92-153 ------------------------
+// The reference serialises with Boost archives over its bit streams; the same
+// stream is produced and read here through the C ABI, on host memory.
+template <typename T>
+std::vector<unsigned char> save_bin(const Field<T> &f)
+{
+    const sct_layout l = f.mapping().c_layout();
+    std::vector<unsigned char> buf(sct_bin_bound(&l, sct_detail::dtype<T>()));
+    size_t written = 0;
+    sct_detail::check(nullptr,
+                      sct_bin_write(&l, sct_detail::dtype<T>(), f.lo(), f.hi(), f.data(), buf.data(), buf.size(),
+                                    &written),
+                      "save");
+    buf.resize(written);
+    return buf;
+}
+
+template <typename T>
+void save_bin(const Field<T> &f, const std::string &path)
+{
+    const auto buf = save_bin(f);
+    std::FILE *fp = std::fopen(path.c_str(), "wb");
+    if (!fp) throw std::runtime_error("cannot open " + path);
+    const size_t n = std::fwrite(buf.data(), 1, buf.size(), fp);
+    const bool ok = n == buf.size() && std::fclose(fp) == 0;
+    if (!ok) throw std::runtime_error("cannot write " + path);
+}
+
+template <typename T>
+Field<T> load_bin(const void *buf, size_t len)
+{
+    sct_bin_info info{};
+    sct_detail::check(nullptr, sct_bin_parse(buf, len, sct_detail::dtype<T>(), &info, nullptr, nullptr), "load");
+    std::vector<uint32_t> start_row(info.width), base(size_t(info.width) + 1);
+    sct_detail::check(nullptr, sct_bin_parse(buf, len, sct_detail::dtype<T>(), &info, start_row.data(), base.data()),
+                      "load");
+    Field<T> f(CylinderMapping(info.n_images, info.height, info.width, info.nstride, info.col_begin, info.col_end,
+                               std::move(start_row), std::move(base), info.data_size));
+    sct_detail::check(nullptr, sct_bin_read(buf, len, sct_detail::dtype<T>(), &info, f.data()), "load");
+    f.lo(info.lo);
+    f.hi(info.hi);
+    return f;
+}
+
+template <typename T>
+Field<T> load_bin(const std::string &path)
+{
+    std::FILE *fp = std::fopen(path.c_str(), "rb");
+    if (!fp) throw std::runtime_error("cannot open " + path);
+    std::vector<unsigned char> buf;
+    unsigned char chunk[1 << 16];
+    size_t n;
+    while ((n = std::fread(chunk, 1, sizeof chunk, fp)) > 0) buf.insert(buf.end(), chunk, chunk + n);
+    std::fclose(fp);
+    return load_bin<T>(buf.data(), buf.size());
 }

@@ -82,13 +82,15 @@ def build_oracle() -> Path:
 
 
 def build_cpp_example() -> Path:
-    """tests/cpp/api_example: the C++ API (include/segmentctscans.hpp) over libsct.so."""
+    """tests/cpp/api_example (device) and archive_example (host only): the C++
+    API (include/segmentctscans.hpp) over libsct.so."""
     out_dir = ROOT / "tests" / "cpp" / "_build"
     out_dir.mkdir(exist_ok=True)
-    target = out_dir / "api_example"
-    _run(["g++", "-O2", "-std=c++17", "-I", str(ROOT / "include"), str(ROOT / "tests" / "cpp" / "api_example.cpp"),
-          "-L", str(LIB), "-lsct", "-Wl,-rpath,$ORIGIN/../../../segmentctscans_amd/_lib", "-o", str(target)])
-    return target
+    for name in ("api_example", "archive_example"):
+        _run(["g++", "-O2", "-std=c++17", "-I", str(ROOT / "include"), str(ROOT / "tests" / "cpp" / f"{name}.cpp"),
+              "-L", str(LIB), "-lsct", "-Wl,-rpath,$ORIGIN/../../../segmentctscans_amd/_lib", "-o",
+              str(out_dir / name)])
+    return out_dir / "api_example"
 
 
 def build_all() -> None:

@@ -89,3 +89,27 @@ def test_truncated_and_invalid():
         A.load_bin(b[:-3])
     with pytest.raises(SctError):
         A.load_bin(b"\x00\x00\x00\x00")
+
+
+def test_cpp_save_load_bin_matches_oracle_bytes(tmp_path):
+    """save_bin / load_bin of the C++ header (tests/cpp/archive_example.cpp):
+    the files it writes are the oracle's bytes for the same fields."""
+    import subprocess
+    from pathlib import Path
+    exe = Path(__file__).resolve().parent / "cpp" / "_build" / "archive_example"
+    assert exe.exists(), "build first (__graft_entry__.build())"
+    p = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 0 and "archive_example ok" in p.stdout, p.stderr
+    for dtype, name, lo, hi in ((np.float32, "f32.bin", -0.25, 1.5), (np.uint16, "u16.bin", 0.0, 0.0)):
+        f = Field.cylinder(37, 29, 23, 14.0, 11.0, 11.2)
+        m = f.mapping
+        data = np.zeros(m.span, dtype)
+        for col in range(m.col_begin, m.col_end):
+            for row in range(m.row_begin(col), m.row_end(col)):
+                o = int(m.base[col]) + (row - m.row_begin(col)) * m.nstride
+                n = np.arange(m.n_images)
+                if dtype == np.float32:
+                    data[o:o + m.n_images] = (n + 3 * row + 7 * col).astype(np.float32) / np.float32(1000.0)
+                else:
+                    data[o:o + m.n_images] = (n * 31 + row * 17 + col * 13) & 0xFFFF
+        assert (tmp_path / name).read_bytes() == OA.save(m, data, lo, hi)
