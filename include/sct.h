@@ -267,6 +267,23 @@ int sct_potts_adjoint(sct_ctx *ctx, const sct_field *g, const sct_field *target,
                       double *loss, double *grad_c, double *grad_lambda, sct_field **grad_u,
                       float *times_ms);
 
+/* The same differentiation split into a forward and a backward call, for a
+ * caller-defined loss.  sct_potts_tape_forward runs iters iterations from the
+ * nearest-label start, keeps the checkpoints in *tape and, with V non-null,
+ * stores K new float fields V[i] = V_i after the last iteration (V_{K-1} = 1
+ * minus the stored components; the layout of g).  sct_potts_tape_backward
+ * takes dL/dV_i (K float fields with the layout of g) and returns dL/dc[K],
+ * dL/dlambda and (optional) a new field dL/du; it may be called any number of
+ * times on one tape.  g must stay alive and unchanged while the tape lives.
+ * *forward_ms / *backward_ms (optional): event times.  Both synchronise. */
+typedef struct sct_potts_tape sct_potts_tape;
+int sct_potts_tape_forward(sct_ctx *ctx, const sct_field *g, int K, const float *c, float lambda, float tau1,
+                           float tau2, int iters, int checkpoint_every, sct_field **V, sct_potts_tape **tape,
+                           float *forward_ms);
+int sct_potts_tape_backward(sct_potts_tape *tape, const sct_field *const *dV, double *grad_c, double *grad_lambda,
+                            sct_field **grad_u, float *backward_ms);
+void sct_potts_tape_destroy(sct_potts_tape *tape);
+
 /* ---- diagnostics (no counterpart in the reference) ------------------------ */
 /* Checks on the GPU that the Dykstra projection's range-specific correctly
  * rounded square root (x in [1, 2^64)) and quotient (b in [1, 2^64),

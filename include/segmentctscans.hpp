@@ -15,6 +15,8 @@
 //   transpose/convert_to_fp.h:8-10         convert_to_fp<uint16_t | float>
 //   transpose/find_intensity_range.h:8-9   find_intensity_range
 //   include/field.h:92-153                 Field<T>::save / load as save_bin / load_bin
+// and, with no counterpart in the reference, PottsFunction<K> (forward /
+// backward through the unrolled iteration).
 //
 // Differences a caller sees: the context type is HipContext; Field<T>::view()
 // returns a CylinderView (element access view(n, row, col)) in place of a
@@ -340,6 +342,72 @@ inline UVHistReturn uv_hist(HipContext &ctx, DeviceField<float> &u, DeviceField<
     auto [hist, hm] = uv_hist(ctx, u, v, u0, u1, v0, v1, nbins);
     return {hist, hm, u0, u1, v0, v1};
 }
+
+// ---- differentiable Potts (no counterpart in the reference) -------------------------
+// A forward / backward Function object over the unrolled primal-dual iteration:
+// forward(g, c, lambda) runs iters iterations from the nearest-label start and
+// returns V_0..V_{K-1} (the soft segmentation), keeping checkpoints every
+// checkpoint_every iterations (0: sqrt(iters)); backward(dV) takes dL/dV_i and
+// returns dL/dc, dL/dlambda and dL/du, and may be called repeatedly.  g must
+// outlive the forward pass's tape.
+template <size_t K>
+class PottsFunction {
+    static_assert(K >= 2 && K <= max_phases, "K must be in [2, 6]");
+
+public:
+    struct Gradients {
+        std::array<double, K> c{};
+        double lambda = 0;
+        device_field_ptr<float> u;
+    };
+
+    PottsFunction(HipContext &ctx, float tau1, float tau2, int iters, int checkpoint_every = 0)
+        : ctx_(ctx), tau1_(tau1), tau2_(tau2), iters_(iters), ck_(checkpoint_every)
+    {}
+    ~PottsFunction() { sct_potts_tape_destroy(tape_); }
+    PottsFunction(const PottsFunction &) = delete;
+    PottsFunction &operator=(const PottsFunction &) = delete;
+
+    std::array<device_field_ptr<float>, K> forward(DeviceField<float> &g, std::array<float, K> c, float lambda)
+    {
+        sct_potts_tape_destroy(tape_);
+        tape_ = nullptr;
+        std::array<sct_field *, K> v{};
+        sct_detail::check(ctx_.get(),
+                          sct_potts_tape_forward(ctx_.get(), g.handle(), int(K), c.data(), lambda, tau1_, tau2_, iters_,
+                                                 ck_, v.data(), &tape_, &forward_ms_),
+                          "PottsFunction::forward");
+        map_ = g.mapping();
+        std::array<device_field_ptr<float>, K> out;
+        for (size_t i = 0; i < K; ++i) out[i] = device_field_ptr<float>{new DeviceField<float>{v[i], map_}};
+        return out;
+    }
+
+    Gradients backward(const std::array<const DeviceField<float> *, K> &dV)
+    {
+        if (!tape_) throw std::logic_error("PottsFunction::backward before forward");
+        std::array<const sct_field *, K> d{};
+        for (size_t i = 0; i < K; ++i) d[i] = dV[i]->handle();
+        Gradients gr;
+        sct_field *gu = nullptr;
+        sct_detail::check(ctx_.get(),
+                          sct_potts_tape_backward(tape_, d.data(), gr.c.data(), &gr.lambda, &gu, &backward_ms_),
+                          "PottsFunction::backward");
+        gr.u = device_field_ptr<float>{new DeviceField<float>{gu, map_}};
+        return gr;
+    }
+
+    float forward_ms() const { return forward_ms_; }
+    float backward_ms() const { return backward_ms_; }
+
+private:
+    HipContext &ctx_;
+    float tau1_, tau2_;
+    int iters_, ck_;
+    sct_potts_tape *tape_ = nullptr;
+    CylinderMapping map_;
+    float forward_ms_ = 0, backward_ms_ = 0;
+};
 
 // ---- the .bin archive: Field<T>::save / load, field.h:92-153 ------------------------
 // The reference serialises with Boost archives over its bit streams; the same

@@ -28,7 +28,7 @@ from . import _native as N
 __all__ = [
     "SctError", "Mapping", "Field", "Context", "DeviceField", "PottsSolver",
     "make_context", "upload", "download", "device_memory", "potts_min_partition",
-    "chambolle", "calc_hist", "min_max", "grad_magn", "grad_min_max", "uv_hist",
+    "PottsTape", "chambolle", "calc_hist", "min_max", "grad_magn", "grad_min_max", "uv_hist",
     "UVHistReturn", "get_stride", "prev_stride", "max_phases",
 ]
 
@@ -381,6 +381,62 @@ def potts_adjoint(ctx: Context, g: DeviceField, target: DeviceField, ci, lam: fl
         f.close()
     return dict(loss=loss.value, grad_c=gc, grad_lambda=gl.value, grad_u=gu, forward_ms=float(t[0]),
                 backward_ms=float(t[1]))
+
+
+class PottsTape:
+    """Forward / backward over the unrolled Potts iteration (no counterpart in
+    the reference).  forward() runs the iterations from the nearest-label start
+    and keeps the checkpoints; backward(dV) maps dL/dV_i (K device fields) to
+    dL/dc, dL/dlambda and dL/du, any number of times."""
+
+    def __init__(self, ctx: Context, g: DeviceField, ci, lam: float, tau1: float, tau2: float, iters: int, *,
+                 checkpoint_every: int = 0, want_v: bool = True):
+        self.ctx, self.g = ctx, g
+        c = np.ascontiguousarray(ci, np.float32)
+        if not 2 <= c.size <= max_phases:
+            raise ValueError("K must be in [2, 6]")
+        self.K = c.size
+        vh = (C.c_void_p * self.K)()
+        h = C.c_void_p()
+        t = C.c_float()
+        ctx.check(N.lib().sct_potts_tape_forward(ctx.handle, g.handle, self.K, c.ctypes.data_as(N.f32p), lam, tau1,
+                                                 tau2, iters, checkpoint_every, vh if want_v else None, C.byref(h),
+                                                 C.byref(t)))
+        self.handle = h
+        self.forward_ms = t.value
+        self.V = [DeviceField(ctx, C.c_void_p(vh[i]), g.mapping) for i in range(self.K)] if want_v else None
+
+    def backward(self, dV, *, want_grad_u: bool = True) -> dict:
+        if len(dV) != self.K:
+            raise ValueError("backward needs K fields")
+        arr = (C.c_void_p * self.K)(*[d.handle.value if isinstance(d.handle, C.c_void_p) else d.handle for d in dV])
+        gc = np.zeros(self.K, np.float64)
+        gl = C.c_double()
+        h = C.c_void_p()
+        t = C.c_float()
+        self.ctx.check(N.lib().sct_potts_tape_backward(self.handle, arr, gc.ctypes.data_as(C.POINTER(C.c_double)),
+                                                       C.byref(gl), C.byref(h) if want_grad_u else None, C.byref(t)))
+        gu = None
+        if want_grad_u:
+            f = DeviceField(self.ctx, h, self.g.mapping)
+            gu = f.to_host()
+            f.close()
+        return dict(grad_c=gc, grad_lambda=gl.value, grad_u=gu, backward_ms=t.value)
+
+    def close(self) -> None:
+        if self.V:
+            for f in self.V:
+                f.close()
+            self.V = None
+        if self.handle:
+            N.lib().sct_potts_tape_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class PottsSolver:

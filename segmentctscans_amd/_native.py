@@ -89,6 +89,11 @@ SIGNATURES = {
     "sct_potts_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, f32p, C.c_float, C.c_float,
                                     C.c_float, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                     C.POINTER(C.c_double), C.POINTER(C.c_void_p), f32p]),
+    "sct_potts_tape_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, f32p, C.c_float, C.c_float, C.c_float,
+                                         C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), f32p]),
+    "sct_potts_tape_backward": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_void_p), f32p]),
+    "sct_potts_tape_destroy": (None, [C.c_void_p]),
     "sct_potts_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
                                   C.POINTER(C.c_void_p), C.POINTER(C.c_float)]),
     "sct_potts_solver_gap": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

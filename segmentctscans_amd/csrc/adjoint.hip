@@ -491,150 +491,368 @@ void free_buffers(sct_ctx *ctx, Buffers &b)
     b = Buffers{};
 }
 
+// Seed of the reverse pass from dL/dV_i (i < K, fields with the layout of g):
+// the stored components receive dV_i - dV_{K-1}.
 template <int K>
-int run_adjoint(sct_ctx *ctx, const sct_field *g, const sct_field *target, const float *c, float lambda, float tau1,
-                float tau2, int iters, int ck, double *loss, double *grad_c, double *grad_lambda, sct_field **grad_u,
-                float *times_ms)
+struct DvFields {
+    const float *p[K];
+};
+
+template <int K>
+__global__ __launch_bounds__(kAdjThreads) void adj_seed_kernel(VolumeLines L, uint64_t wspan, DvFields<K> dv, float *gV)
 {
-    const Geometry &geo = g->geo;
-    const uint32_t n = geo.n_images, ws = sct_get_stride(n);
+    const Voxel vx = voxel_of(L);
+    if (!vx.ok) return;
+    const float last = dv.p[K - 1][vx.uo];
+    for (int i = 0; i < K - 1; ++i) gV[uint64_t(i) * wspan + vx.w] = dv.p[i][vx.uo] - last;
+}
+
+// V_i (i < K) of a state into fields with the layout of g; V_{K-1} = 1 - sum.
+template <int K>
+struct VFields {
+    float *p[K];
+};
+
+template <int K>
+__global__ __launch_bounds__(kAdjThreads) void adj_export_v_kernel(VolumeLines L, uint64_t wspan, const float *st,
+                                                                   VFields<K> out)
+{
+    const Voxel vx = voxel_of(L);
+    if (!vx.ok) return;
+    float last = 1.0f;
+    for (int i = 0; i < K - 1; ++i) {
+        const float v = st[uint64_t(i) * wspan + vx.w];
+        last -= v;
+        out.p[i][vx.uo] = v;
+    }
+    out.p[K - 1][vx.uo] = last;
+}
+
+struct TapeBase {
+    virtual ~TapeBase() = default;
+    virtual const sct_field *input() const = 0;
+    // seeds: target (MSE loss) or dV[K]
+    virtual int backward(const sct_field *target, const sct_field *const *dV, double *loss, double *grad_c,
+                         double *grad_lambda, sct_field **grad_u, float *ms) = 0;
+};
+
+// The forward pass with its checkpoints, and any number of reverse passes
+// from them.
+template <int K>
+struct Tape : TapeBase {
+    sct_ctx *ctx;
+    const sct_field *g;
+    int iters, ck, nck = 0, seg_len = 0;
     AdjLayout Ly;
-    Ly.fwd = VolumeLines{geo.d_nbr, geo.lines, n, (n + 127) / 128, ws, geo.nstride, 0};
-    Ly.one = VolumeLines{geo.d_nbr, geo.lines, n, (n + 63) / 64, ws, geo.nstride, 0};
-    Ly.wspan = uint64_t(geo.lines) * ws;
-    const uint64_t S = std::max<uint64_t>(1, Ly.wspan), state = 5 * uint64_t(K - 1) * S;
-    const int nck = (iters + ck - 1) / ck;
-    const int seg_len = std::min(ck, iters);
-    hipStream_t st = ctx->stream;
+    uint64_t S = 1, state = 0, nacc = 0;
     Buffers B;
+    XiQueue Q{};
+    OneSolve<K> prm;
+    DykstraLimits lim;
+    float tau1, tau2;
+    uint32_t fgrid = 0, xgrid = 0, agrid = 0, rgrid = 0;
+    float *final = nullptr;   // the state after the last iteration (a recompute buffer)
     int rc = SCT_OK;
-    auto alloc = [&](float *&p, uint64_t elems) {
-        if (!rc) rc = sct_alloc(ctx, (void **)&p, sizeof(float) * elems);
-    };
-    B.ck.assign(size_t(nck), nullptr);
-    B.seg.assign(size_t(seg_len) + 1, nullptr);
-    for (auto &p : B.ck) alloc(p, state);
-    for (auto &p : B.seg) alloc(p, state);
-    alloc(B.gs, 4 * uint64_t(K - 1) * S);   // gV | gXi
-    alloc(B.GQ, uint64_t(K) * S);
-    alloc(B.GP, 3 * uint64_t(K) * S);
-    alloc(B.gu, S);
-    const uint32_t agrid = uint32_t((Ly.one.count() * kWave + kAdjThreads - 1) / kAdjThreads);
-    // per-workgroup sums: dc[K], dlambda of every A launch, then the loss
-    const uint64_t nacc = uint64_t(agrid) * (K + 2);
-    if (!rc) rc = sct_alloc(ctx, (void **)&B.acc, sizeof(double) * std::max<uint64_t>(1, nacc));
-    // replay queue: a sub-queue serves every kQueues-th workgroup of <= 256 voxels
-    XiQueue Q{nullptr, nullptr, ((agrid + kQueues - 1) / kQueues) * uint32_t(kAdjThreads)};
-    alloc(B.items, uint64_t(kQueues) * Q.cap * item_floats<K>());
-    if (!rc) rc = sct_alloc(ctx, (void **)&B.count, sizeof(uint32_t) * kQueues);
-    Q.items = B.items;
-    Q.count = B.count;
-    const uint32_t rgrid = std::max<uint32_t>(1, ctx->cu_count * 4);
-    hipEvent_t ev[4] = {};
-    for (auto &e : ev)
-        if (!rc && hipEventCreate(&e) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventCreate failed");
-    if (rc) {
-        for (auto e : ev)
-            if (e) (void)hipEventDestroy(e);
-        free_buffers(ctx, B);
+
+    Tape(sct_ctx *c, const sct_field *gf, const float *cv, float lambda, float t1, float t2, int it, int ckp)
+        : ctx(c), g(gf), iters(it), ck(ckp), lim(dykstra_limits()), tau1(t1), tau2(t2)
+    {
+        for (int i = 0; i < K; ++i) prm.p.c[i] = cv[i];
+        prm.p.lambda = lambda;
+    }
+    ~Tape() override { free_buffers(ctx, B); }
+    const sct_field *input() const override { return g; }
+
+    void chk(hipError_t e, const char *what)
+    {
+        if (e != hipSuccess && !rc) rc = sct_hip_fail(ctx, e, what);
+    }
+    void copy(float *dst, const float *src)
+    {
+        chk(hipMemcpyAsync(dst, src, sizeof(float) * state, hipMemcpyDeviceToDevice, ctx->stream), "adjoint state copy");
+    }
+    const float *u() const { return (const float *)g->data; }
+    void step(float *s)
+    {
+        hipStream_t st = ctx->stream;
+        const auto a = state_args<K>(s, u(), S);
+        potts_xi_kernel<K, StorageF32><<<xgrid, kXiThreads, 0, st>>>(Ly.fwd, a, tau2, lim);
+        potts_v_kernel<K, StorageF32, false><<<fgrid, 256, 0, st>>>(Ly.fwd, a, prm, tau1, nullptr);
+    }
+
+    int allocate()
+    {
+        const Geometry &geo = g->geo;
+        const uint32_t n = geo.n_images, ws = sct_get_stride(n);
+        Ly.fwd = VolumeLines{geo.d_nbr, geo.lines, n, (n + 127) / 128, ws, geo.nstride, 0};
+        Ly.one = VolumeLines{geo.d_nbr, geo.lines, n, (n + 63) / 64, ws, geo.nstride, 0};
+        Ly.wspan = uint64_t(geo.lines) * ws;
+        S = std::max<uint64_t>(1, Ly.wspan);
+        state = 5 * uint64_t(K - 1) * S;
+        nck = (iters + ck - 1) / ck;
+        seg_len = std::min(ck, iters);
+        auto alloc = [&](float *&p, uint64_t elems) {
+            if (!rc) rc = sct_alloc(ctx, (void **)&p, sizeof(float) * elems);
+        };
+        B.ck.assign(size_t(nck), nullptr);
+        B.seg.assign(size_t(seg_len) + 1, nullptr);
+        for (auto &p : B.ck) alloc(p, state);
+        for (auto &p : B.seg) alloc(p, state);
+        alloc(B.gs, 4 * uint64_t(K - 1) * S);   // gV | gXi
+        alloc(B.GQ, uint64_t(K) * S);
+        alloc(B.GP, 3 * uint64_t(K) * S);
+        alloc(B.gu, S);
+        agrid = uint32_t((Ly.one.count() * kWave + kAdjThreads - 1) / kAdjThreads);
+        // per-workgroup sums: dc[K], dlambda of every A launch, then the loss
+        nacc = uint64_t(agrid) * (K + 2);
+        if (!rc) rc = sct_alloc(ctx, (void **)&B.acc, sizeof(double) * std::max<uint64_t>(1, nacc));
+        // replay queue: a sub-queue serves every kQueues-th workgroup of <= 256 voxels
+        Q = XiQueue{nullptr, nullptr, ((agrid + kQueues - 1) / kQueues) * uint32_t(kAdjThreads)};
+        alloc(B.items, uint64_t(kQueues) * Q.cap * item_floats<K>());
+        if (!rc) rc = sct_alloc(ctx, (void **)&B.count, sizeof(uint32_t) * kQueues);
+        Q.items = B.items;
+        Q.count = B.count;
+        rgrid = std::max<uint32_t>(1, ctx->cu_count * 4);
+        fgrid = wave_grid(Ly.fwd.count());
+        xgrid = xi_grid<StorageF32>(Ly.fwd.count());
         return rc;
     }
-    const float *u = (const float *)g->data;
-    OneSolve<K> prm;
-    for (int i = 0; i < K; ++i) prm.p.c[i] = c[i];
-    prm.p.lambda = lambda;
-    const DykstraLimits lim = dykstra_limits();
-    const uint32_t fgrid = wave_grid(Ly.fwd.count()), xgrid = xi_grid<StorageF32>(Ly.fwd.count());
-    auto forward = [&](float *s) {
-        const auto a = state_args<K>(s, u, S);
-        potts_xi_kernel<K, StorageF32><<<xgrid, 256, 0, st>>>(Ly.fwd, a, tau2, lim);
-        potts_v_kernel<K, StorageF32, false><<<fgrid, 256, 0, st>>>(Ly.fwd, a, prm, tau1, nullptr);
-    };
-    auto chk = [&](hipError_t e, const char *what) {
-        if (e != hipSuccess && !rc) rc = sct_hip_fail(ctx, e, what);
-    };
-    auto copy = [&](float *dst, const float *src) {
-        chk(hipMemcpyAsync(dst, src, sizeof(float) * state, hipMemcpyDeviceToDevice, st), "adjoint state copy");
-    };
-    // forward pass with checkpoints
-    float *cur = B.seg[0];
-    chk(hipMemsetAsync(cur, 0, sizeof(float) * state, st), "memset");
-    chk(hipEventRecord(ev[0], st), "hipEventRecord");
-    potts_init_kernel<K, StorageF32><<<fgrid, 256, 0, st>>>(Ly.fwd, state_args<K>(cur, u, S), prm);
-    copy(B.ck[0], cur);
-    for (int it = 0; it < iters && !rc; ++it) {
-        forward(cur);
-        if ((it + 1) % ck == 0 && (it + 1) / ck < nck) copy(B.ck[(it + 1) / ck], cur);
-    }
-    chk(hipEventRecord(ev[1], st), "hipEventRecord");
-    // loss and the adjoint of the final state
-    chk(hipMemsetAsync(B.gs, 0, sizeof(float) * 4 * (K - 1) * S, st), "memset");
-    chk(hipMemsetAsync(B.gu, 0, sizeof(float) * S, st), "memset");
-    chk(hipMemsetAsync(B.GP, 0, sizeof(float) * 3 * K * S, st), "memset");
-    chk(hipMemsetAsync(B.acc, 0, sizeof(double) * std::max<uint64_t>(1, nacc), st), "memset");
-    chk(hipMemsetAsync(B.count, 0, sizeof(uint32_t) * kQueues, st), "memset");
-    const float wgt = float(1.0 / (double(std::max<uint64_t>(1, geo.data_size)) * K));
-    adj_loss_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, cur, (const float *)target->data, wgt, B.gs,
-                                                       B.acc + uint64_t(agrid) * (K + 1));
-    // reverse pass, one checkpoint segment at a time
-    chk(hipEventRecord(ev[2], st), "hipEventRecord");
-    for (int k = nck - 1; k >= 0 && !rc; --k) {
-        const int t0 = k * ck, t1 = std::min(iters, t0 + ck);
-        copy(B.seg[0], B.ck[size_t(k)]);
-        for (int j = 0; j < t1 - t0; ++j) {
-            copy(B.seg[size_t(j) + 1], B.seg[size_t(j)]);
-            forward(B.seg[size_t(j) + 1]);
+
+    // iters iterations from the nearest-label start, checkpointing every ck.
+    int forward(float *ms)
+    {
+        hipStream_t st = ctx->stream;
+        hipEvent_t ev[2] = {};
+        for (auto &e : ev)
+            if (!rc && hipEventCreate(&e) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventCreate failed");
+        float *cur = B.seg.empty() ? nullptr : B.seg[0];
+        if (!rc) {
+            chk(hipMemsetAsync(cur, 0, sizeof(float) * state, st), "memset");
+            chk(hipEventRecord(ev[0], st), "hipEventRecord");
+            potts_init_kernel<K, StorageF32><<<fgrid, 256, 0, st>>>(Ly.fwd, state_args<K>(cur, u(), S), prm);
+            copy(B.ck[0], cur);
+            for (int it = 0; it < iters && !rc; ++it) {
+                step(cur);
+                if ((it + 1) % ck == 0 && (it + 1) / ck < nck) copy(B.ck[(it + 1) / ck], cur);
+            }
+            chk(hipEventRecord(ev[1], st), "hipEventRecord");
+            chk(hipGetLastError(), "adjoint forward launch");
+            chk(hipStreamSynchronize(st), "adjoint forward");
         }
-        for (int t = t1 - 1; t >= t0; --t) {
-            const float *s0 = B.seg[size_t(t - t0)], *s1 = B.seg[size_t(t - t0) + 1];
-            adj_v_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, s1, u, prm.p, tau1, B.GP, B.gs, B.GQ, B.gu,
-                                                            B.acc, B.count);
-            adj_xi_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, tau1, tau2, lim, B.GQ, B.gs, B.GP, Q);
-            adj_xi_replay_kernel<K><<<rgrid, kAdjThreads, 0, st>>>(S, tau2, lim, B.gs, B.GP, Q);
-        }
+        if (!rc && ms && hipEventElapsedTime(ms, ev[0], ev[1]) != hipSuccess)
+            rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventElapsedTime failed");
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+        final = cur;
+        return rc;
     }
-    chk(hipEventRecord(ev[3], st), "hipEventRecord");
-    chk(hipGetLastError(), "adjoint launch");
-    std::vector<double> acc(nacc);
-    (void)hipStreamSynchronize(st);
-    if (!rc && nacc) {
-        hipError_t e = hipMemcpyAsync(acc.data(), B.acc, sizeof(double) * nacc, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) rc = sct_hip_fail(ctx, e, "adjoint");
-    }
-    if (!rc && grad_u) {
+
+    int export_v(sct_field **V)
+    {
+        const Geometry &geo = g->geo;
         sct_layout l{geo.n_images, geo.height, geo.width, geo.nstride, geo.col_begin, geo.col_end,
                      geo.start_row.data(), geo.base.data()};
-        rc = sct_field_create(ctx, &l, SCT_F32, grad_u);
-        if (!rc) {
-            hipError_t e = hipMemsetAsync((*grad_u)->data, 0, sizeof(float) * geo.span(), st);
-            if (e == hipSuccess && geo.lines)
-                e = hipMemcpy2DAsync((*grad_u)->data, sizeof(float) * geo.nstride, B.gu, sizeof(float) * ws,
-                                     sizeof(float) * n, geo.lines, hipMemcpyDeviceToDevice, st);
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
-            if (e != hipSuccess) rc = sct_hip_fail(ctx, e, "adjoint grad_u");
+        VFields<K> out{};
+        for (int i = 0; i < K && !rc; ++i) {
+            rc = sct_field_create(ctx, &l, SCT_F32, &V[i]);
+            if (!rc) {
+                out.p[i] = (float *)V[i]->data;
+                chk(hipMemsetAsync(out.p[i], 0, sizeof(float) * geo.span(), ctx->stream), "memset");
+            }
         }
+        if (!rc) {
+            adj_export_v_kernel<K><<<agrid, kAdjThreads, 0, ctx->stream>>>(Ly.one, S, final, out);
+            chk(hipGetLastError(), "adjoint V export");
+            chk(hipStreamSynchronize(ctx->stream), "adjoint V export");
+        }
+        return rc;
     }
-    if (!rc && times_ms) {
-        if (hipEventElapsedTime(&times_ms[0], ev[0], ev[1]) != hipSuccess ||
-            hipEventElapsedTime(&times_ms[1], ev[2], ev[3]) != hipSuccess)
+
+    int backward(const sct_field *target, const sct_field *const *dV, double *loss, double *grad_c,
+                 double *grad_lambda, sct_field **grad_u, float *ms) override
+    {
+        const Geometry &geo = g->geo;
+        hipStream_t st = ctx->stream;
+        const uint32_t n = geo.n_images, ws = sct_get_stride(n);
+        if (target && !final) return sct_fail(ctx, SCT_ERR_INVALID, "the final state was overwritten");
+        hipEvent_t ev[2] = {};
+        for (auto &e : ev)
+            if (!rc && hipEventCreate(&e) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventCreate failed");
+        if (rc) {
+            for (auto e : ev)
+                if (e) (void)hipEventDestroy(e);
+            return rc;
+        }
+        chk(hipMemsetAsync(B.gs, 0, sizeof(float) * 4 * (K - 1) * S, st), "memset");
+        chk(hipMemsetAsync(B.gu, 0, sizeof(float) * S, st), "memset");
+        chk(hipMemsetAsync(B.GP, 0, sizeof(float) * 3 * K * S, st), "memset");
+        chk(hipMemsetAsync(B.acc, 0, sizeof(double) * std::max<uint64_t>(1, nacc), st), "memset");
+        chk(hipMemsetAsync(B.count, 0, sizeof(uint32_t) * kQueues, st), "memset");
+        if (target) {
+            const float wgt = float(1.0 / (double(std::max<uint64_t>(1, geo.data_size)) * K));
+            adj_loss_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, final, (const float *)target->data, wgt,
+                                                               B.gs, B.acc + uint64_t(agrid) * (K + 1));
+        } else {
+            DvFields<K> d{};
+            for (int i = 0; i < K; ++i) d.p[i] = (const float *)dV[i]->data;
+            adj_seed_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, d, B.gs);
+        }
+        // reverse pass, one checkpoint segment at a time
+        chk(hipEventRecord(ev[0], st), "hipEventRecord");
+        for (int k = nck - 1; k >= 0 && !rc; --k) {
+            const int t0 = k * ck, t1 = std::min(iters, t0 + ck);
+            copy(B.seg[0], B.ck[size_t(k)]);
+            for (int j = 0; j < t1 - t0; ++j) {
+                copy(B.seg[size_t(j) + 1], B.seg[size_t(j)]);
+                step(B.seg[size_t(j) + 1]);
+            }
+            for (int t = t1 - 1; t >= t0; --t) {
+                const float *s0 = B.seg[size_t(t - t0)], *s1 = B.seg[size_t(t - t0) + 1];
+                adj_v_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, s1, u(), prm.p, tau1, B.GP, B.gs, B.GQ,
+                                                                B.gu, B.acc, B.count);
+                adj_xi_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, tau1, tau2, lim, B.GQ, B.gs, B.GP, Q);
+                adj_xi_replay_kernel<K><<<rgrid, kAdjThreads, 0, st>>>(S, tau2, lim, B.gs, B.GP, Q);
+            }
+        }
+        chk(hipEventRecord(ev[1], st), "hipEventRecord");
+        chk(hipGetLastError(), "adjoint launch");
+        final = nullptr;   // the recompute buffers no longer hold the final state
+        std::vector<double> acc(nacc);
+        (void)hipStreamSynchronize(st);
+        if (!rc && nacc) {
+            hipError_t e = hipMemcpyAsync(acc.data(), B.acc, sizeof(double) * nacc, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) rc = sct_hip_fail(ctx, e, "adjoint");
+        }
+        if (!rc && grad_u) {
+            sct_layout l{geo.n_images, geo.height, geo.width, geo.nstride, geo.col_begin, geo.col_end,
+                         geo.start_row.data(), geo.base.data()};
+            rc = sct_field_create(ctx, &l, SCT_F32, grad_u);
+            if (!rc) {
+                hipError_t e = hipMemsetAsync((*grad_u)->data, 0, sizeof(float) * geo.span(), st);
+                if (e == hipSuccess && geo.lines)
+                    e = hipMemcpy2DAsync((*grad_u)->data, sizeof(float) * geo.nstride, B.gu, sizeof(float) * ws,
+                                         sizeof(float) * n, geo.lines, hipMemcpyDeviceToDevice, st);
+                if (e == hipSuccess) e = hipStreamSynchronize(st);
+                if (e != hipSuccess) rc = sct_hip_fail(ctx, e, "adjoint grad_u");
+            }
+        }
+        if (!rc && ms && hipEventElapsedTime(ms, ev[0], ev[1]) != hipSuccess)
             rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventElapsedTime failed");
+        for (auto e : ev) (void)hipEventDestroy(e);
+        if (rc) return rc;
+        double sums[8] = {};
+        for (uint32_t b = 0; b < agrid; ++b) {
+            for (int i = 0; i <= K; ++i) sums[i] += acc[size_t(b) * (K + 1) + size_t(i)];
+            sums[K + 1] += acc[size_t(agrid) * (K + 1) + b];
+        }
+        for (int i = 0; i < K; ++i) grad_c[i] = sums[i];
+        *grad_lambda = sums[K];
+        if (loss) *loss = sums[K + 1];
+        return SCT_OK;
     }
-    for (auto e : ev) (void)hipEventDestroy(e);
-    free_buffers(ctx, B);
-    if (rc) return rc;
-    double sums[8] = {};
-    for (uint32_t b = 0; b < agrid; ++b) {
-        for (int i = 0; i <= K; ++i) sums[i] += acc[size_t(b) * (K + 1) + size_t(i)];
-        sums[K + 1] += acc[size_t(agrid) * (K + 1) + b];
+};
+
+template <int K>
+TapeBase *new_tape(sct_ctx *ctx, const sct_field *g, const float *c, float lambda, float tau1, float tau2, int iters,
+                   int ck, sct_field **V, float *forward_ms, int &rc)
+{
+    auto *t = new Tape<K>(ctx, g, c, lambda, tau1, tau2, iters, ck);
+    rc = t->allocate();
+    if (!rc) rc = t->forward(forward_ms);
+    if (!rc && V) rc = t->export_v(V);
+    if (rc) {
+        delete t;
+        return nullptr;
     }
-    for (int i = 0; i < K; ++i) grad_c[i] = sums[i];
-    *grad_lambda = sums[K];
-    *loss = sums[K + 1];
+    return t;
+}
+
+TapeBase *make_tape(int K, sct_ctx *ctx, const sct_field *g, const float *c, float lambda, float tau1, float tau2,
+                    int iters, int ck, sct_field **V, float *forward_ms, int &rc)
+{
+    switch (K) {
+    case 2: return new_tape<2>(ctx, g, c, lambda, tau1, tau2, iters, ck, V, forward_ms, rc);
+    case 3: return new_tape<3>(ctx, g, c, lambda, tau1, tau2, iters, ck, V, forward_ms, rc);
+    case 4: return new_tape<4>(ctx, g, c, lambda, tau1, tau2, iters, ck, V, forward_ms, rc);
+    case 5: return new_tape<5>(ctx, g, c, lambda, tau1, tau2, iters, ck, V, forward_ms, rc);
+    default: return new_tape<6>(ctx, g, c, lambda, tau1, tau2, iters, ck, V, forward_ms, rc);
+    }
+}
+
+bool same_layout(const sct_field *a, const sct_field *b)
+{
+    const Geometry &x = a->geo, &y = b->geo;
+    return x.n_images == y.n_images && x.height == y.height && x.width == y.width && x.nstride == y.nstride &&
+           x.col_begin == y.col_begin && x.col_end == y.col_end && x.base == y.base && x.start_row == y.start_row;
+}
+
+int check_forward_args(sct_ctx *ctx, const sct_field *g, int K, const float *c, int iters, int checkpoint_every)
+{
+    if (!g || !c || K < 2 || K > 6 || iters < 1 || checkpoint_every < 0)
+        return sct_fail(ctx, SCT_ERR_INVALID, "invalid adjoint arguments");
+    if (g->dtype != SCT_F32) return sct_fail(ctx, SCT_ERR_INVALID, "the adjoint needs a float field");
     return SCT_OK;
 }
 
+int checkpoint_interval(int iters, int checkpoint_every)
+{
+    return checkpoint_every > 0 ? checkpoint_every : std::max(1, int(std::lround(std::sqrt(double(iters)))));
+}
+
 }  // namespace
+
+struct sct_potts_tape {
+    sct_ctx *ctx;
+    int K;
+    TapeBase *t;
+};
+
+extern "C" int sct_potts_tape_forward(sct_ctx *ctx, const sct_field *g, int K, const float *c, float lambda,
+                                      float tau1, float tau2, int iters, int checkpoint_every, sct_field **V,
+                                      sct_potts_tape **tape, float *forward_ms)
+{
+    if (!ctx) return SCT_ERR_INVALID;
+    if (!tape) return sct_fail(ctx, SCT_ERR_INVALID, "null tape");
+    *tape = nullptr;
+    if (V)
+        for (int i = 0; i < K && i < 6; ++i) V[i] = nullptr;
+    int rc = check_forward_args(ctx, g, K, c, iters, checkpoint_every);
+    if (rc) return rc;
+    TapeBase *t = make_tape(K, ctx, g, c, lambda, tau1, tau2, iters, checkpoint_interval(iters, checkpoint_every), V,
+                            forward_ms, rc);
+    if (!t) {
+        if (V)
+            for (int i = 0; i < K; ++i) {
+                sct_field_destroy(V[i]);
+                V[i] = nullptr;
+            }
+        return rc;
+    }
+    *tape = new sct_potts_tape{ctx, K, t};
+    return SCT_OK;
+}
+
+extern "C" int sct_potts_tape_backward(sct_potts_tape *tape, const sct_field *const *dV, double *grad_c,
+                                       double *grad_lambda, sct_field **grad_u, float *backward_ms)
+{
+    if (!tape) return SCT_ERR_INVALID;
+    sct_ctx *ctx = tape->ctx;
+    if (grad_u) *grad_u = nullptr;
+    if (!dV || !grad_c || !grad_lambda) return sct_fail(ctx, SCT_ERR_INVALID, "invalid adjoint arguments");
+    for (int i = 0; i < tape->K; ++i)
+        if (!dV[i] || dV[i]->dtype != SCT_F32 || !same_layout(dV[i], tape->t->input()))
+            return sct_fail(ctx, SCT_ERR_INVALID, "dV must be K float fields with the layout of g");
+    return tape->t->backward(nullptr, dV, nullptr, grad_c, grad_lambda, grad_u, backward_ms);
+}
+
+extern "C" void sct_potts_tape_destroy(sct_potts_tape *tape)
+{
+    if (!tape) return;
+    (void)hipStreamSynchronize(tape->ctx->stream);
+    delete tape->t;
+    delete tape;
+}
 
 extern "C" int sct_potts_adjoint(sct_ctx *ctx, const sct_field *g, const sct_field *target, int K, const float *c,
                                  float lambda, float tau1, float tau2, int iters, int checkpoint_every,
@@ -643,17 +861,15 @@ extern "C" int sct_potts_adjoint(sct_ctx *ctx, const sct_field *g, const sct_fie
 {
     if (!ctx) return SCT_ERR_INVALID;
     if (grad_u) *grad_u = nullptr;
-    if (!g || !target || !c || !loss || !grad_c || !grad_lambda || K < 2 || K > 6 || iters < 1 || checkpoint_every < 0)
-        return sct_fail(ctx, SCT_ERR_INVALID, "invalid adjoint arguments");
-    if (g->dtype != SCT_F32 || target->dtype != SCT_F32 || target->geo.span() != g->geo.span() ||
-        target->geo.base != g->geo.base || target->geo.start_row != g->geo.start_row)
+    if (!target || !loss || !grad_c || !grad_lambda) return sct_fail(ctx, SCT_ERR_INVALID, "invalid adjoint arguments");
+    int rc = check_forward_args(ctx, g, K, c, iters, checkpoint_every);
+    if (rc) return rc;
+    if (target->dtype != SCT_F32 || !same_layout(target, g))
         return sct_fail(ctx, SCT_ERR_INVALID, "target must be a float field with the layout of g");
-    const int ck = checkpoint_every > 0 ? checkpoint_every : std::max(1, int(std::lround(std::sqrt(double(iters)))));
-    switch (K) {
-    case 2: return run_adjoint<2>(ctx, g, target, c, lambda, tau1, tau2, iters, ck, loss, grad_c, grad_lambda, grad_u, times_ms);
-    case 3: return run_adjoint<3>(ctx, g, target, c, lambda, tau1, tau2, iters, ck, loss, grad_c, grad_lambda, grad_u, times_ms);
-    case 4: return run_adjoint<4>(ctx, g, target, c, lambda, tau1, tau2, iters, ck, loss, grad_c, grad_lambda, grad_u, times_ms);
-    case 5: return run_adjoint<5>(ctx, g, target, c, lambda, tau1, tau2, iters, ck, loss, grad_c, grad_lambda, grad_u, times_ms);
-    default: return run_adjoint<6>(ctx, g, target, c, lambda, tau1, tau2, iters, ck, loss, grad_c, grad_lambda, grad_u, times_ms);
-    }
+    TapeBase *t = make_tape(K, ctx, g, c, lambda, tau1, tau2, iters, checkpoint_interval(iters, checkpoint_every),
+                            nullptr, times_ms, rc);
+    if (!t) return rc;
+    rc = t->backward(target, nullptr, loss, grad_c, grad_lambda, grad_u, times_ms ? times_ms + 1 : nullptr);
+    delete t;
+    return rc;
 }

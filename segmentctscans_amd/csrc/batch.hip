@@ -148,7 +148,7 @@ int group_phase(sct_ctx *ctx, const sct_potts_problem *pr, Group &G, int phase, 
         potts_init_kernel<K, St><<<wave_grid(L.count()), 256, 0, st>>>(L, a, prm);
         potts_xi_zero_kernel<K, St><<<wave_grid(L.count()), 256, 0, st>>>(L, a);
     } else if (phase == 2) {
-        potts_xi_kernel<K, St><<<xi_grid<St>(L.count()), 256, 0, st>>>(L, a, tau2, dykstra_limits());
+        potts_xi_kernel<K, St><<<xi_grid<St>(L.count()), kXiThreads, 0, st>>>(L, a, tau2, dykstra_limits());
         potts_v_kernel<K, St, false><<<wave_grid(L.count()), 256, 0, st>>>(L, a, prm, tau1, nullptr);
     } else {
         potts_labels_kernel<K, St><<<wave_grid(L.count()), 256, 0, st>>>(L, a, prm, G.R, 0u, 0xffffffffu);

@@ -120,7 +120,7 @@ int twopass_iterate(sct_potts_solver *s, int iters, float *dq_dev)
     const uint32_t xg = xi_grid<St>(L.count()), vg = wave_grid(L.count());
     for (int i = 0; i < iters; ++i) {
         potts_mark(s);
-        potts_xi_kernel<K, St><<<xg, 256, 0, s->ctx->stream>>>(L, a, s->tau2, lim);
+        potts_xi_kernel<K, St><<<xg, kXiThreads, 0, s->ctx->stream>>>(L, a, s->tau2, lim);
         potts_mark(s);
         if (dq_dev)
             potts_v_kernel<K, St, true><<<vg, 256, 0, s->ctx->stream>>>(L, a, prm, s->tau1, dq_dev + i);

@@ -135,9 +135,20 @@ struct TwoPassArgs {
     const float *u;
 };
 
-constexpr int kXiVoxelsPerLane = 4;  // per lane and workgroup: kXiVoxelsPerLane / VPL chunks per wave
-constexpr int kXiItems = 512;        // work-list capacity per workgroup
-constexpr int kXiWorkWaves = 4;      // waves that drain the work list
+#ifndef SCT_XI_VOX
+#define SCT_XI_VOX 4
+#endif
+#ifndef SCT_XI_WAVES
+#define SCT_XI_WAVES 4
+#endif
+#ifndef SCT_XI_BLOCK_WAVES
+#define SCT_XI_BLOCK_WAVES 4
+#endif
+constexpr int kXiVoxelsPerLane = SCT_XI_VOX;  // per lane and workgroup: kXiVoxelsPerLane / VPL chunks per wave
+constexpr int kXiBlockWaves = SCT_XI_BLOCK_WAVES;
+constexpr int kXiThreads = kXiBlockWaves * 64;
+constexpr int kXiItems = 128 * kXiBlockWaves;        // work-list capacity per workgroup
+constexpr int kXiWorkWaves = SCT_XI_WAVES;      // waves that drain the work list
 
 // Workgroups of the one-chunk-per-wave kernels, in XCD runs of 64 / VPL
 // (about 11 lines of 1426 voxels).
@@ -210,7 +221,7 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<
     }
     __syncthreads();
     const uint64_t total = lines.count();
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, 16);
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, 64 / kXiBlockWaves);
     float res[CPW][VPL][KM][3];
     int pos[CPW][VPL];
     uint32_t wofs[CPW];
@@ -466,7 +477,7 @@ template <typename St>
 inline uint32_t xi_grid(uint64_t chunks)
 {
     constexpr uint64_t cpw = kXiVoxelsPerLane / vpl_of<St>();
-    return uint32_t(((chunks + cpw - 1) / cpw + 3) / 4);
+    return uint32_t(((chunks + cpw - 1) / cpw + kXiBlockWaves - 1) / kXiBlockWaves);
 }
 inline uint32_t wave_grid(uint64_t chunks) { return uint32_t((chunks + 3) / 4); }
 

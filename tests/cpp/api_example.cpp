@@ -67,6 +67,36 @@ int main()
     auto uv = uv_hist(*ctx, *d_img, *d_v, 64u);
     CHECK(uv.mapping.stride == 64 && uv.v_start > 0);
 
+    // forward / backward through the unrolled iteration: L = sum of V_0 over
+    // the left half (dL/dV_0 = 1 there); twice the upstream gradient gives
+    // exactly twice the gradients
+    {
+        auto fn = PottsFunction<2>(*ctx, tau, tau, 20, 5);
+        auto d_in = upload(*ctx, img);
+        auto V = fn.forward(*d_in, ci, 30.0f);
+        auto hv0 = Field<float>(img.mapping()), hv1 = Field<float>(img.mapping());
+        download(*ctx, *V[0], hv0);
+        download(*ctx, *V[1], hv1);
+        auto w = Field<float>(img.mapping()), w2 = Field<float>(img.mapping()), z = Field<float>(img.mapping());
+        auto wv = w.view(), w2v = w2.view();
+        for (auto col = m.col_begin(); col < m.col_end(); ++col)
+            for (auto row = m.row_begin(col); row < m.row_end(col); ++row)
+                for (uint32_t n = 0; n < m.n_images(); ++n) {
+                    CHECK(hv0.view()(n, row, col) + hv1.view()(n, row, col) == 1.0f);
+                    wv(n, row, col) = col < 20 ? 1.0f : 0.0f;
+                    w2v(n, row, col) = col < 20 ? 2.0f : 0.0f;
+                }
+        auto d_w = upload(*ctx, w), d_w2 = upload(*ctx, w2), d_z = upload(*ctx, z);
+        auto g1 = fn.backward({d_w.get(), d_z.get()});
+        auto g2 = fn.backward({d_w2.get(), d_z.get()});
+        CHECK(std::isfinite(g1.c[0]) && std::isfinite(g1.lambda));
+        CHECK(g2.c[0] == 2 * g1.c[0] && g2.c[1] == 2 * g1.c[1] && g2.lambda == 2 * g1.lambda);
+        auto hu1 = Field<float>(img.mapping()), hu2 = Field<float>(img.mapping());
+        download(*ctx, *g1.u, hu1);
+        download(*ctx, *g2.u, hu2);
+        for (uint32_t i = 0; i < m.required_span_size(); ++i) CHECK(hu2.data()[i] == 2.0f * hu1.data()[i]);
+    }
+
     // download into a field of another size throws (cuda-context.cpp:69-71)
     bool threw = false;
     try {

@@ -98,3 +98,64 @@ def test_adjoint_128_cube_200_iterations(ctx):
         print(f"128^3 central differences h={h:g}: dc {np.array(fd_c)}, dlambda {fd_l:.6e}")
         assert np.allclose(gc, fd_c, rtol=rel)
         assert gl == pytest.approx(fd_l, rel=rel)
+
+
+def test_tape_forward_backward(ctx):
+    """The split forward / backward (PottsTape): V of the forward pass is bitwise
+    the solver's V after the same iterations; backward of the MSE loss's
+    dL/dV equals potts_adjoint (which the FP64 host adjoint pins); backward is
+    repeatable and exactly linear in dV (2 dV -> exactly twice every gradient)."""
+    K, iters, lam = 3, 12, 30.0
+    ph = make_phantom(18, 18, 2.0, 4.0, seed=101)
+    c, tgt, g, t, _ = _setup(ctx, ph, K)
+    mp = ph.field.mapping
+    m = mp.valid_mask()
+    tape = S.PottsTape(ctx, g, c, lam, TAU, TAU, iters, checkpoint_every=5)
+    V = [f.to_host() for f in tape.V]
+    sol = S.PottsSolver(ctx, g, c, lam, TAU, TAU)
+    sol.iterate(iters)
+    Vs, _ = sol.state()
+    sol.close()
+    last = np.ones(mp.span, np.float32)
+    for i in range(K - 1):
+        assert np.array_equal(V[i][m].view(np.uint32), Vs[i][m].view(np.uint32))
+        last = last - Vs[i]
+    assert np.array_equal(V[K - 1][m].view(np.uint32), last[m].view(np.uint32))
+    # dL/dV of L = 1/(N K) sum (V_i - [target = i])^2
+    wgt = np.float32(1.0 / (m.sum() * K))
+    dv = []
+    for i in range(K):
+        d = np.zeros(mp.span, np.float32)
+        d[m] = np.float32(2) * wgt * (V[i][m] - (tgt[m] == i).astype(np.float32))
+        dv.append(d)
+    dev = [S.upload(ctx, Field(mp, d)) for d in dv]
+    r = tape.backward(dev)
+    ref = S.potts_adjoint(ctx, g, t, c, lam, TAU, TAU, iters, checkpoint_every=5)
+    print(f"tape dc {r['grad_c']} vs {ref['grad_c']}; dlambda {r['grad_lambda']:.8e} vs {ref['grad_lambda']:.8e}")
+    assert np.allclose(r["grad_c"], ref["grad_c"], rtol=1e-5, atol=1e-6 * np.abs(ref["grad_c"]).max())
+    assert r["grad_lambda"] == pytest.approx(ref["grad_lambda"], rel=1e-5)
+    assert _rel_l2(r["grad_u"][m], ref["grad_u"][m]) < 1e-5
+    again = tape.backward(dev)
+    assert np.array_equal(again["grad_c"], r["grad_c"]) and again["grad_lambda"] == r["grad_lambda"]
+    assert np.array_equal(again["grad_u"], r["grad_u"])
+    dev2 = [S.upload(ctx, Field(mp, np.float32(2) * d)) for d in dv]
+    r2 = tape.backward(dev2)
+    assert np.array_equal(r2["grad_c"], 2 * r["grad_c"]) and r2["grad_lambda"] == 2 * r["grad_lambda"]
+    assert np.array_equal(r2["grad_u"], np.float32(2) * r["grad_u"])
+    for f in dev + dev2:
+        f.close()
+    tape.close()
+
+
+def test_tape_rejects_mismatched_gradients(ctx):
+    ph = make_phantom(12, 12, 2.0, 3.0, seed=102)
+    c, _, g, _, _ = _setup(ctx, ph, 3)
+    tape = S.PottsTape(ctx, g, c, 30.0, TAU, TAU, 4, want_v=False)
+    for n, side in ((14, 12), (12, 14)):   # other n_images (same span), other cylinder
+        other = S.upload(ctx, make_phantom(n, side, 2.0, 3.0, seed=103).field)
+        with pytest.raises(S.SctError):
+            tape.backward([g, g, other])
+        other.close()
+    with pytest.raises(ValueError):
+        tape.backward([g, g])
+    tape.close()

@@ -61,7 +61,8 @@ def test_checkpoint_interval_does_not_change_gradients(prob):
     ref = O.adj_grad(lay, u, target, c, 30.0, TAU, TAU, 10, ck=1)
     for ck in (3, 4, 10, 16):
         got = O.adj_grad(lay, u, target, c, 30.0, TAU, TAU, 10, ck=ck)
-        assert got[0] == ref[0]
+        # the loss is an OpenMP reduction: its order may follow the thread count
+        assert got[0] == pytest.approx(ref[0], rel=1e-13)
         assert np.allclose(got[1], ref[1], rtol=1e-12, atol=0)
         assert got[2] == pytest.approx(ref[2], rel=1e-12)
         assert np.array_equal(got[3], ref[3])
