@@ -284,6 +284,24 @@ int sct_potts_tape_backward(sct_potts_tape *tape, const sct_field *const *dV, do
                             sct_field **grad_u, float *backward_ms);
 void sct_potts_tape_destroy(sct_potts_tape *tape);
 
+/* ---- gmm-peaks: passes of the 1-D Gaussian mixture fit, gmm-peaks/gmm.h ----
+ * Samples (doubles) live on the device; component k has weight pi[k], mean
+ * mu[k] and variance var[k], density pi_k (2 pi)^-1/2 exp(-(x - mu_k)^2 /
+ * (2 var_k)) / sqrt(|var_k|), K in [1, 16]. */
+typedef struct sct_samples sct_samples;
+int sct_samples_create(sct_ctx *ctx, const double *x, uint64_t n, sct_samples **out);
+void sct_samples_destroy(sct_samples *s);
+/* The fused E+M pass, gmm.h:145-201: responsibilities g_ik = density_k /
+ * sum_j density_j; w[k] = sum_i g_ik, s1[k] = sum_i g_ik x_i, s2[k] =
+ * sum_i g_ik x_i^2, *nll = -sum_i log sum_k density_k (a sample whose
+ * densities all underflow counts 1/K to each component and -log DBL_MIN).
+ * Deterministic: workgroup partial sums are added in a fixed order. */
+int sct_gmm1d_pass(const sct_samples *s, int K, const double *pi, const double *mu, const double *var, double *w,
+                   double *s1, double *s2, double *nll);
+/* gmm.h:204-219: clusters[i] = the component of largest density (first on ties). */
+int sct_gmm1d_assign(const sct_samples *s, int K, const double *pi, const double *mu, const double *var,
+                     int32_t *clusters);
+
 /* ---- diagnostics (no counterpart in the reference) ------------------------ */
 /* Checks on the GPU that the Dykstra projection's range-specific correctly
  * rounded square root (x in [1, 2^64)) and quotient (b in [1, 2^64),

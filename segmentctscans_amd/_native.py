@@ -94,6 +94,10 @@ SIGNATURES = {
     "sct_potts_tape_backward": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_double),
                                           C.POINTER(C.c_double), C.POINTER(C.c_void_p), f32p]),
     "sct_potts_tape_destroy": (None, [C.c_void_p]),
+    "sct_samples_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_void_p)]),
+    "sct_samples_destroy": (None, [C.c_void_p]),
+    "sct_gmm1d_pass": (C.c_int, [C.c_void_p, C.c_int] + [C.POINTER(C.c_double)] * 7),
+    "sct_gmm1d_assign": (C.c_int, [C.c_void_p, C.c_int] + [C.POINTER(C.c_double)] * 3 + [C.POINTER(C.c_int32)]),
     "sct_potts_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
                                   C.POINTER(C.c_void_p), C.POINTER(C.c_float)]),
     "sct_potts_solver_gap": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

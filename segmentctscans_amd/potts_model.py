@@ -46,6 +46,19 @@ def png_bytes(img: np.ndarray) -> bytes:
             + chunk(b"IDAT", zlib.compress(raw, 6)) + chunk(b"IEND", b""))
 
 
+def png_bytes_rgb(img: np.ndarray) -> bytes:
+    """8-bit RGB PNG of an (h, w, 3) array (filter type 0 on every row)."""
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w, _ = img.shape
+
+    def chunk(tag: bytes, data: bytes) -> bytes:
+        return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+
+    raw = b"".join(b"\x00" + img[r].tobytes() for r in range(h))
+    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0))
+            + chunk(b"IDAT", zlib.compress(raw, 6)) + chunk(b"IEND", b""))
+
+
 def slice_image(field: Field, col: int) -> np.ndarray:
     """write_png (:34-49): rows x n of column col, v^(1/2.2) * 255 rounded to
     nearest (ties to even) and saturated to [0, 255].  A picture for the eye:
