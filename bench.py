@@ -240,10 +240,19 @@ def batch_workload(args) -> int:
     vox = sum(g.mapping.data_size for (g, _, _) in probs)
     log(f"{len(probs)} solves, {vox} voxels in all, {iters} iterations")
     S.potts_batch(ctx, probs[:4], tau, tau, 2)   # warm-up (code objects)
+    # the default tile, then the autotuned one (its trial iterations are
+    # outside the timed region; the results are the same)
     t = time.perf_counter()
-    outs, ms = S.potts_batch(ctx, probs, tau, tau, iters, graph=True)
+    outs0, ms_default = S.potts_batch(ctx, probs, tau, tau, iters, graph=True)
+    wall_default = time.perf_counter() - t
+    for o in outs0:
+        o.close()
+    log(f"batched, default tile: iterations {ms_default:.1f} ms (events), call {wall_default:.2f} s")
+    t = time.perf_counter()
+    outs, ms = S.potts_batch(ctx, probs, tau, tau, iters, graph=True, autotune=not args.no_autotune)
     wall_batch = time.perf_counter() - t
-    log(f"batched: iterations {ms:.1f} ms (events), call {wall_batch:.2f} s")
+    tiles = S.potts_batch_tiles(ctx)
+    log(f"batched, autotuned tiles {tiles}: iterations {ms:.1f} ms (events), call {wall_batch:.2f} s")
     t = time.perf_counter()
     refs = [S.potts_min_partition(ctx, g, c, lam, tau, tau, iters) for (g, c, lam) in probs]
     ctx.synchronize()
@@ -261,7 +270,9 @@ def batch_workload(args) -> int:
         "config": {"workload": "potts_batch_576", "volumes": args.batch_volumes, "lambdas": lams,
                    "iterations": iters, "solves": len(probs), "voxels": vox,
                    "shapes": [[n, h, w, "box" if box else "cylinder", int(c.size)] for (_, c, n, h, w, box) in vols]},
-        "iterations_ms": ms, "batched_call_s": wall_batch, "looped_calls_s": wall_loop,
+        "iterations_ms": ms, "iterations_ms_default_tile": ms_default,
+        "xi_tile_waves": {str(k): v for k, v in tiles.items()},
+        "batched_call_s": wall_batch, "looped_calls_s": wall_loop,
         "identical_labels": agree}), flush=True)
     return 0
 
@@ -293,6 +304,7 @@ def main() -> int:
                     help="run the batched workload (576 solves) instead of the single-volume one")
     ap.add_argument("--batch-volumes", type=int, default=64)
     ap.add_argument("--batch-iters", type=int, default=300)
+    ap.add_argument("--no-autotune", action="store_true", help="batched workload without the tile autotuner")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB mebibytes (hipMemcpy's copy "
                          "kernel), to calibrate the FETCH_SIZE / WRITE_SIZE counters")

@@ -244,6 +244,7 @@ typedef struct sct_potts_problem {
 } sct_potts_problem;
 
 #define SCT_BATCH_GRAPH 1   /* capture all iterations into one hipGraph launch */
+#define SCT_BATCH_AUTOTUNE 2 /* pick each K group's Xi workgroup tile by timing (cached per context) */
 
 /* Solves n problems, iters primal-dual iterations each, with common tau1 and
  * tau2 and storage (SCT_STORAGE_*): out[p] receives a new field of c[argmax]
@@ -252,6 +253,15 @@ typedef struct sct_potts_problem {
  * (optional) receives the event time of the iterations.  Synchronises. */
 int sct_potts_batch(sct_ctx *ctx, int n, const sct_potts_problem *problems, float tau1, float tau2,
                     int iters, int storage, int flags, sct_field **out, float *elapsed_ms);
+/* The Xi workgroup tiles (waves per workgroup: 4, 2 or 1; 0 for a K with no
+ * problem) the last sct_potts_batch call on ctx used, for K = 2..6.  With
+ * SCT_BATCH_AUTOTUNE the call times 3 iterations of each tile per K group
+ * before the timed iterations and restarts the group from its initial state,
+ * so the results do not depend on the choice. */
+int sct_potts_batch_tiles(sct_ctx *ctx, int *waves);
+/* Fixes the tile of each K (waves[K - 2] in {1, 2, 4}; 0 leaves K to the
+ * default or the autotuner) for the following sct_potts_batch calls on ctx. */
+int sct_potts_batch_set_tiles(sct_ctx *ctx, const int *waves);
 
 /* ---- adjoint (no counterpart in the reference) ---------------------------- */
 /* Gradient of L = 1/(N K) sum_x sum_{i<K} (V_i(x) - [target(x) = i])^2, V after

@@ -332,13 +332,14 @@ def find_intensity_range(ctx: Context, f: DeviceField, tail_thr: float, *, hist:
 
 
 def potts_batch(ctx: Context, problems, tau1: float, tau2: float, iters: int, *, storage="fp32",
-                graph=True) -> tuple[list, float]:
+                graph=True, autotune=False) -> tuple[list, float]:
     """Many Potts solves in shared launches (no counterpart in the reference).
 
     problems: sequence of (DeviceField g, labels c, lambda).  Returns the
     label fields (one per problem, as potts_min_partition would return them)
     and the event time of the iterations in ms; graph=True runs all
-    iterations as one captured hipGraph launch."""
+    iterations as one captured hipGraph launch; autotune=True first picks
+    each K group's Xi workgroup tile by timing (potts_batch_tiles reports it)."""
     n = len(problems)
     arr = (N.SctPottsProblem * max(n, 1))()
     for i, (g, c, lam) in enumerate(problems):
@@ -353,8 +354,23 @@ def potts_batch(ctx: Context, problems, tau1: float, tau2: float, iters: int, *,
     outs = (C.c_void_p * max(n, 1))()
     ms = C.c_float()
     ctx.check(N.lib().sct_potts_batch(ctx.handle, n, arr, tau1, tau2, iters, N.STORAGE[storage],
-                                      N.SCT_BATCH_GRAPH if graph else 0, outs, C.byref(ms)))
+                                      (N.SCT_BATCH_GRAPH if graph else 0) | (N.SCT_BATCH_AUTOTUNE if autotune else 0),
+                                      outs, C.byref(ms)))
     return [DeviceField(ctx, C.c_void_p(outs[i]), problems[i][0].mapping) for i in range(n)], ms.value
+
+
+def potts_batch_set_tiles(ctx: Context, tiles: dict) -> None:
+    """Fix the Xi workgroup tile ({K: waves in 1, 2, 4}) of later potts_batch
+    calls; an empty dict returns every K to the default or the autotuner."""
+    w = (C.c_int * 5)(*[int(tiles.get(K, 0)) for K in range(2, 7)])
+    ctx.check(N.lib().sct_potts_batch_set_tiles(ctx.handle, w))
+
+
+def potts_batch_tiles(ctx: Context) -> dict:
+    """{K: waves per Xi workgroup} used by the last potts_batch call."""
+    w = (C.c_int * 5)()
+    ctx.check(N.lib().sct_potts_batch_tiles(ctx.handle, w))
+    return {K: w[K - 2] for K in range(2, 7) if w[K - 2]}
 
 
 def potts_adjoint(ctx: Context, g: DeviceField, target: DeviceField, ci, lam: float, tau1: float, tau2: float,

@@ -40,6 +40,7 @@ class SctPottsProblem(C.Structure):
 
 
 SCT_BATCH_GRAPH = 1
+SCT_BATCH_AUTOTUNE = 2
 
 
 class SctPottsOpts(C.Structure):
@@ -98,6 +99,8 @@ SIGNATURES = {
     "sct_samples_destroy": (None, [C.c_void_p]),
     "sct_gmm1d_pass": (C.c_int, [C.c_void_p, C.c_int] + [C.POINTER(C.c_double)] * 7),
     "sct_gmm1d_assign": (C.c_int, [C.c_void_p, C.c_int] + [C.POINTER(C.c_double)] * 3 + [C.POINTER(C.c_int32)]),
+    "sct_potts_batch_tiles": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "sct_potts_batch_set_tiles": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "sct_potts_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
                                   C.POINTER(C.c_void_p), C.POINTER(C.c_float)]),
     "sct_potts_solver_gap": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

@@ -26,6 +26,7 @@ namespace {
 
 struct Group {
     int K = 0;
+    int bw = kXiBlockWaves;            // Xi workgroup tile: waves per workgroup
     std::vector<int> probs;            // problem indices
     std::vector<uint64_t> r_off;       // result offset of each solve in R
     uint64_t wspan = 0, n_chunks = 0, r_total = 0, u_total = 0;
@@ -148,7 +149,13 @@ int group_phase(sct_ctx *ctx, const sct_potts_problem *pr, Group &G, int phase, 
         potts_init_kernel<K, St><<<wave_grid(L.count()), 256, 0, st>>>(L, a, prm);
         potts_xi_zero_kernel<K, St><<<wave_grid(L.count()), 256, 0, st>>>(L, a);
     } else if (phase == 2) {
-        potts_xi_kernel<K, St><<<xi_grid<St>(L.count()), kXiThreads, 0, st>>>(L, a, tau2, dykstra_limits());
+        const DykstraLimits lim = dykstra_limits();
+        if (G.bw == 1)
+            potts_xi_kernel<K, St, BatchLines, 1><<<xi_grid<St, 1>(L.count()), 64, 0, st>>>(L, a, tau2, lim);
+        else if (G.bw == 2)
+            potts_xi_kernel<K, St, BatchLines, 2><<<xi_grid<St, 2>(L.count()), 128, 0, st>>>(L, a, tau2, lim);
+        else
+            potts_xi_kernel<K, St, BatchLines, 4><<<xi_grid<St, 4>(L.count()), 256, 0, st>>>(L, a, tau2, lim);
         potts_v_kernel<K, St, false><<<wave_grid(L.count()), 256, 0, st>>>(L, a, prm, tau1, nullptr);
     } else {
         potts_labels_kernel<K, St><<<wave_grid(L.count()), 256, 0, st>>>(L, a, prm, G.R, 0u, 0xffffffffu);
@@ -176,6 +183,62 @@ int run_phase(sct_ctx *ctx, const sct_potts_problem *pr, Group &G, int storage, 
     case 5: return group_phase_k<5>(ctx, pr, G, storage, phase, t1, t2);
     default: return group_phase_k<6>(ctx, pr, G, storage, phase, t1, t2);
     }
+}
+
+// The autotuner: for each group, times kTuneIters iterations with each Xi
+// workgroup tile (BW = 4, 2, 1 waves: 1024, 512, 256 voxels and work lists
+// of 512, 256, 128 entries), keeps the fastest, and starts the group again
+// from its initial state, so the result is the untuned one.  Choices are
+// cached in the context per (K, storage, size class of the group).
+constexpr int kTuneIters = 3;
+constexpr int kTiles[3] = {4, 2, 1};
+
+uint64_t tile_key(const Group &G, int storage)
+{
+    uint64_t cls = 0;
+    for (uint64_t c = G.n_chunks; c > 1; c >>= 1) ++cls;   // log2 of the chunk count
+    return (uint64_t(G.K) << 16) | (uint64_t(storage) << 8) | cls;
+}
+
+int autotune(sct_ctx *ctx, const sct_potts_problem *pr, std::vector<Group> &groups, int storage, float t1, float t2)
+{
+    hipStream_t st = ctx->stream;
+    hipEvent_t ev[4] = {};
+    int rc = SCT_OK;
+    for (auto &e : ev)
+        if (!rc && hipEventCreate(&e) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventCreate failed");
+    for (Group &G : groups) {
+        if (rc || !G.n_chunks || ctx->forced_tiles[G.K - 2]) continue;
+        const uint64_t key = tile_key(G, storage);
+        auto hit = ctx->tiles.find(key);
+        if (hit != ctx->tiles.end()) {
+            G.bw = hit->second;
+            continue;
+        }
+        float best = 0;
+        int best_bw = kXiBlockWaves;
+        for (int i = 0; i < 3 && !rc; ++i) {
+            G.bw = kTiles[i];
+            // one untimed iteration, then kTuneIters timed ones
+            rc = run_phase(ctx, pr, G, storage, 2, t1, t2);
+            if (!rc && hipEventRecord(ev[0], st) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventRecord");
+            for (int it = 0; it < kTuneIters && !rc; ++it) rc = run_phase(ctx, pr, G, storage, 2, t1, t2);
+            if (!rc && hipEventRecord(ev[1], st) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventRecord");
+            float ms = 0;
+            if (!rc && (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess))
+                rc = sct_fail(ctx, SCT_ERR_HIP, "autotune timing failed");
+            if (!rc && (i == 0 || ms < best)) {
+                best = ms;
+                best_bw = kTiles[i];
+            }
+        }
+        G.bw = best_bw;
+        if (!rc) ctx->tiles[key] = best_bw;
+        if (!rc) rc = run_phase(ctx, pr, G, storage, 1, t1, t2);   // back to the initial state
+    }
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
 }
 
 // Runs the iterations of all groups, directly or as one captured graph, and
@@ -226,6 +289,23 @@ int run_iterations(sct_ctx *ctx, const sct_potts_problem *pr, std::vector<Group>
 
 }  // namespace
 
+extern "C" int sct_potts_batch_set_tiles(sct_ctx *ctx, const int *waves)
+{
+    if (!ctx || !waves) return SCT_ERR_INVALID;
+    for (int k = 0; k < 5; ++k)
+        if (waves[k] != 0 && waves[k] != 1 && waves[k] != 2 && waves[k] != 4)
+            return sct_fail(ctx, SCT_ERR_INVALID, "tiles are 1, 2 or 4 waves (0: not set)");
+    for (int k = 0; k < 5; ++k) ctx->forced_tiles[k] = waves[k];
+    return SCT_OK;
+}
+
+extern "C" int sct_potts_batch_tiles(sct_ctx *ctx, int *waves)
+{
+    if (!ctx || !waves) return SCT_ERR_INVALID;
+    for (int k = 0; k < 5; ++k) waves[k] = ctx->last_tiles[k];
+    return SCT_OK;
+}
+
 extern "C" int sct_potts_batch(sct_ctx *ctx, int n, const sct_potts_problem *problems, float tau1, float tau2,
                                int iters, int storage, int flags, sct_field **out, float *elapsed_ms)
 {
@@ -252,6 +332,11 @@ extern "C" int sct_potts_batch(sct_ctx *ctx, int n, const sct_potts_problem *pro
         if (!rc) rc = run_phase(ctx, problems, G, storage, 0, tau1, tau2);
     for (Group &G : groups)
         if (!rc) rc = run_phase(ctx, problems, G, storage, 1, tau1, tau2);
+    for (Group &G : groups)
+        if (ctx->forced_tiles[G.K - 2]) G.bw = ctx->forced_tiles[G.K - 2];
+    if (!rc && (flags & SCT_BATCH_AUTOTUNE) && iters > 0) rc = autotune(ctx, problems, groups, storage, tau1, tau2);
+    for (int K = 2; K <= 6; ++K) ctx->last_tiles[K - 2] = 0;
+    for (const Group &G : groups) ctx->last_tiles[G.K - 2] = G.n_chunks ? G.bw : 0;
     if (!rc) rc = run_iterations(ctx, problems, groups, storage, iters, tau1, tau2, (flags & SCT_BATCH_GRAPH) != 0,
                                  elapsed_ms);
     for (Group &G : groups)

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -65,6 +66,9 @@ struct sct_ctx {
     hipMemPool_t pool = nullptr;
     std::string err;
     uint32_t cu_count = 0;
+    std::map<uint64_t, int> tiles;   // batch autotuner: chosen Xi workgroup tiles
+    int last_tiles[5] = {};          // tiles of the last sct_potts_batch call, K = 2..6
+    int forced_tiles[5] = {};        // tiles set by sct_potts_batch_set_tiles (0: not set)
 };
 
 struct sct_field {

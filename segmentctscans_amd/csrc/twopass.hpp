@@ -135,20 +135,10 @@ struct TwoPassArgs {
     const float *u;
 };
 
-#ifndef SCT_XI_VOX
-#define SCT_XI_VOX 4
-#endif
-#ifndef SCT_XI_WAVES
-#define SCT_XI_WAVES 4
-#endif
-#ifndef SCT_XI_BLOCK_WAVES
-#define SCT_XI_BLOCK_WAVES 4
-#endif
-constexpr int kXiVoxelsPerLane = SCT_XI_VOX;  // per lane and workgroup: kXiVoxelsPerLane / VPL chunks per wave
-constexpr int kXiBlockWaves = SCT_XI_BLOCK_WAVES;
+constexpr int kXiVoxelsPerLane = 4;  // per lane and workgroup: kXiVoxelsPerLane / VPL chunks per wave
+constexpr int kXiBlockWaves = 4;     // default workgroup of the Xi kernel (its "tile": 64 BW lanes x 4 voxels)
 constexpr int kXiThreads = kXiBlockWaves * 64;
-constexpr int kXiItems = 128 * kXiBlockWaves;        // work-list capacity per workgroup
-constexpr int kXiWorkWaves = SCT_XI_WAVES;      // waves that drain the work list
+constexpr int kXiWorkWaves = 4;      // at most this many waves drain the work list
 
 // Workgroups of the one-chunk-per-wave kernels, in XCD runs of 64 / VPL
 // (about 11 lines of 1426 voxels).
@@ -205,12 +195,15 @@ __global__ __launch_bounds__(256) void potts_xi_zero_kernel(Lines lines, TwoPass
 //   B  the work list is drained one sweep per loop trip, each lane taking a new
 //      item as soon as its item converges, the result going back to LDS;
 //   C  every lane stores the Xi+ of its voxels, VPL at a time.
-template <int K, typename St, typename Lines>
+// BW: waves per workgroup (launch 64 BW threads); the workgroup covers 256 BW
+// voxels and its work list holds 128 BW of them.
+template <int K, typename St, typename Lines, int BW = kXiBlockWaves>
 __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<K, St> a, float tau2,
                                                        DykstraLimits lim)
 {
     constexpr int KM = K - 1, NP = K * (K - 1) / 2;
     constexpr int VPL = vpl_of<St>(), CPW = kXiVoxelsPerLane / VPL;
+    constexpr int kXiItems = 128 * BW;     // work-list capacity per workgroup
     using P = Packed<St, VPL>;
     __shared__ float qb[kXiItems][K][3];   // inputs, then results, of the queued voxels
     __shared__ int nq, next;
@@ -221,7 +214,7 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<
     }
     __syncthreads();
     const uint64_t total = lines.count();
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, 64 / kXiBlockWaves);
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, 64 / BW);
     float res[CPW][VPL][KM][3];
     int pos[CPW][VPL];
     uint32_t wofs[CPW];
@@ -473,11 +466,11 @@ __global__ __launch_bounds__(256) void potts_labels_kernel(Lines lines, TwoPassA
 }
 
 // Workgroups (256 threads) of the Xi kernel and of the one-chunk-per-wave kernels.
-template <typename St>
+template <typename St, int BW = kXiBlockWaves>
 inline uint32_t xi_grid(uint64_t chunks)
 {
     constexpr uint64_t cpw = kXiVoxelsPerLane / vpl_of<St>();
-    return uint32_t(((chunks + cpw - 1) / cpw + kXiBlockWaves - 1) / kXiBlockWaves);
+    return uint32_t(((chunks + cpw - 1) / cpw + BW - 1) / BW);
 }
 inline uint32_t wave_grid(uint64_t chunks) { return uint32_t((chunks + 3) / 4); }
 

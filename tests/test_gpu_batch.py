@@ -61,3 +61,42 @@ def test_batch_rejects_bad_problems(ctx, volumes):
         S.potts_batch(ctx, [(g, [0.5], 30.0)], TAU, TAU, 5)
     outs, _ = S.potts_batch(ctx, [], TAU, TAU, 5)
     assert outs == []
+
+
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_every_xi_tile_equals_single_solves(ctx, volumes, waves):
+    probs = _problems(volumes)
+    S.potts_batch_set_tiles(ctx, {K: waves for K in range(2, 7)})
+    try:
+        outs, _ = S.potts_batch(ctx, probs, TAU, TAU, 20, graph=True)
+        assert set(S.potts_batch_tiles(ctx).values()) == {waves}
+    finally:
+        S.potts_batch_set_tiles(ctx, {})
+    for (g, c, lam), o in zip(probs, outs):
+        want = S.potts_min_partition(ctx, g, c, lam, TAU, TAU, 20).to_host()
+        m = g.mapping.valid_mask()
+        assert np.array_equal(o.to_host()[m], want[m]), (waves, g.mapping.n_images, c.size, lam)
+        o.close()
+
+
+def test_autotuned_batch_equals_untuned(ctx, volumes):
+    """The autotuner's trial iterations leave no trace: the groups restart
+    from their initial state, so the labels are the untuned ones; the choice
+    is cached per context."""
+    probs = _problems(volumes)
+    a, _ = S.potts_batch(ctx, probs, TAU, TAU, 15, graph=True, autotune=True)
+    tiles = S.potts_batch_tiles(ctx)
+    assert tiles and set(tiles.values()) <= {1, 2, 4}
+    assert set(tiles) == {c.size for (_, c, _) in probs}
+    b, _ = S.potts_batch(ctx, probs, TAU, TAU, 15, graph=True)
+    for x, y, (g, _, _) in zip(a, b, probs):
+        m = g.mapping.valid_mask()
+        assert np.array_equal(x.to_host()[m], y.to_host()[m])
+        x.close()
+        y.close()
+    c2, _ = S.potts_batch(ctx, probs, TAU, TAU, 15, graph=True, autotune=True)
+    assert S.potts_batch_tiles(ctx) == tiles
+    for x in c2:
+        x.close()
+    with pytest.raises(S.SctError):
+        S.potts_batch_set_tiles(ctx, {3: 3})
