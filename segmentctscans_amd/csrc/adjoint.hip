@@ -60,6 +60,7 @@ TwoPassArgs<K, StorageF32> state_args(float *st, const float *u, uint64_t wspan)
     }
     for (int j = 0; j < 3 * (K - 1); ++j) a.xi[j] = st + uint64_t(2 * (K - 1) + j) * wspan;
     a.u = u;
+    a.in_place();
     return a;
 }
 
@@ -579,6 +580,24 @@ struct Tape : TapeBase {
         potts_xi_kernel<K, StorageF32><<<xgrid, kXiThreads, 0, st>>>(Ly.fwd, a, tau2, lim);
         potts_v_kernel<K, StorageF32, false><<<fgrid, 256, 0, st>>>(Ly.fwd, a, prm, tau1, nullptr);
     }
+    // One iteration from state src into state dst (src unchanged): the Xi
+    // step writes dst's Xi, the V step reads it with src's V.
+    void step_into(const float *src, float *dst)
+    {
+        hipStream_t st = ctx->stream;
+        const auto in = state_args<K>(const_cast<float *>(src), u(), S);
+        const auto out = state_args<K>(dst, u(), S);
+        auto ax = in;
+        for (int j = 0; j < 3 * (K - 1); ++j) ax.xi_out[j] = out.xi[j];
+        auto av = in;
+        for (int j = 0; j < 3 * (K - 1); ++j) av.xi[j] = out.xi[j];
+        for (int i = 0; i < K - 1; ++i) {
+            av.V_out[i] = out.V[i];
+            av.vbar_out[i] = out.vbar[i];
+        }
+        potts_xi_kernel<K, StorageF32><<<xgrid, kXiThreads, 0, st>>>(Ly.fwd, ax, tau2, lim);
+        potts_v_kernel<K, StorageF32, false><<<fgrid, 256, 0, st>>>(Ly.fwd, av, prm, tau1, nullptr);
+    }
 
     int allocate()
     {
@@ -701,13 +720,16 @@ struct Tape : TapeBase {
         chk(hipEventRecord(ev[0], st), "hipEventRecord");
         for (int k = nck - 1; k >= 0 && !rc; --k) {
             const int t0 = k * ck, t1 = std::min(iters, t0 + ck);
-            copy(B.seg[0], B.ck[size_t(k)]);
+            // states t0 .. t1 of the segment: the checkpoint, then each
+            // recomputed from the previous one
+            std::vector<const float *> sg(size_t(t1 - t0) + 1);
+            sg[0] = B.ck[size_t(k)];
             for (int j = 0; j < t1 - t0; ++j) {
-                copy(B.seg[size_t(j) + 1], B.seg[size_t(j)]);
-                step(B.seg[size_t(j) + 1]);
+                step_into(sg[size_t(j)], B.seg[size_t(j) + 1]);
+                sg[size_t(j) + 1] = B.seg[size_t(j) + 1];
             }
             for (int t = t1 - 1; t >= t0; --t) {
-                const float *s0 = B.seg[size_t(t - t0)], *s1 = B.seg[size_t(t - t0) + 1];
+                const float *s0 = sg[size_t(t - t0)], *s1 = sg[size_t(t - t0) + 1];
                 adj_v_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, s1, u(), prm.p, tau1, B.GP, B.gs, B.GQ,
                                                                 B.gu, B.acc, B.count);
                 adj_xi_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, tau1, tau2, lim, B.GQ, B.gs, B.GP, Q);

@@ -132,6 +132,7 @@ TwoPassArgs<K, St> group_args(const Group &G)
         a.V[i] = G.V + i * G.wspan;
     }
     a.u = G.U;
+    a.in_place();
     return a;
 }
 

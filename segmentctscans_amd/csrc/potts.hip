@@ -70,6 +70,7 @@ TwoPassArgs<K, St> twopass_args(const sct_potts_solver *s, int b)
         a.V[i] = s->V + i * s->wspan;
     }
     a.u = (const float *)s->g->data;
+    a.in_place();
     return a;
 }
 

@@ -126,13 +126,28 @@ struct Packed {
 };
 
 // Per-component base pointers; every component is addressed as
-// base[comp] + 32-bit element offset.
+// base[comp] + 32-bit element offset.  The Xi step reads xi and vbar and
+// writes xi_out; the V step reads xi, V and u and writes V_out and vbar_out.
+// The solvers iterate in place (outputs = inputs, in_place()); the adjoint's
+// recompute steps from one stored state into the next.
 template <int K, typename St>
 struct TwoPassArgs {
     typename St::T *xi[3 * (K - 1)];
     typename St::T *vbar[K - 1];
     float *V[K - 1];
     const float *u;
+    typename St::T *xi_out[3 * (K - 1)];
+    typename St::T *vbar_out[K - 1];
+    float *V_out[K - 1];
+
+    void in_place()
+    {
+        for (int j = 0; j < 3 * (K - 1); ++j) xi_out[j] = xi[j];
+        for (int i = 0; i < K - 1; ++i) {
+            vbar_out[i] = vbar[i];
+            V_out[i] = V[i];
+        }
+    }
 };
 
 constexpr int kXiVoxelsPerLane = 4;  // per lane and workgroup: kXiVoxelsPerLane / VPL chunks per wave
@@ -345,7 +360,7 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<
                 float o[VPL];
 #pragma unroll
                 for (int j = 0; j < VPL; ++j) o[j] = pos[k][j] >= 0 ? qb[pos[k][j]][i][ax] : res[k][j][i][ax];
-                P::st(a.xi[3 * i + ax] + wofs[k], o);
+                P::st(a.xi_out[3 * i + ax] + wofs[k], o);
             }
     }
 }
@@ -423,8 +438,8 @@ __global__ __launch_bounds__(256) void potts_v_kernel(Lines lines, TwoPassArgs<K
         if (nv) {
 #pragma unroll
             for (int i = 0; i < KM; ++i) {
-                PF::st_nt(a.V[i] + w0, v[i]);
-                P::st_nt(a.vbar[i] + w0, vb[i]);
+                PF::st_nt(a.V_out[i] + w0, v[i]);
+                P::st_nt(a.vbar_out[i] + w0, vb[i]);
             }
         }
     }
