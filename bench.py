@@ -190,6 +190,53 @@ def tv_workload(S, iters: int, cpu_iters: int) -> dict:
     return out
 
 
+def long_run(S, ctx, g, c, lam, tau, iters: int, gap_every: int) -> dict:
+    """The full solve of the 1426 x 1024^2 workload: iters iterations with
+    the FP64 duality gap every gap_every iterations (potts_min_partition with
+    a gap trace), timed with HIP events; the gap evaluations are inside."""
+    from segmentctscans_amd.hiptime import EventTimeline
+    tl = EventTimeline(ctx.stream, 2)
+    ctx.synchronize()
+    tl.mark()
+    res, gaps = S.potts_min_partition(ctx, g, c, lam, tau, tau, iters, gap_every=gap_every)
+    tl.mark()
+    ms = float(tl.durations()[0])
+    tl.close()
+    res.close()
+    return {"iterations": iters, "gap_every": gap_every, "ms_total": ms, "ms_per_iteration": ms / iters,
+            "relative_gap_trace": [float(x) for x in gaps]}
+
+
+def crop_agreement(S, ph, c, lam, tau, iters: int, size: int) -> dict:
+    """Label agreement and ||V - V_fp64||_inf of FP32 / BF16 / FP16 against
+    the oracle's FP64 solve on a size^3 box cut from the centre of the
+    workload, solved on its own (box domain, Neumann edges)."""
+    import oracle as O
+    from segmentctscans_amd.phantom import crop_box
+    m = ph.field.mapping
+    box = crop_box(ph, (m.n_images - size) // 2, (m.height - size) // 2, (m.width - size) // 2, size)
+    lay = O.Layout.from_mapping(box.mapping)
+    t = time.perf_counter()
+    r64 = O.potts(lay, box.data, c, lam, tau, tau, iters, precision="f64", want_state=True,
+                  nthreads=min(16, os.cpu_count() or 1))
+    log(f"FP64 oracle on the {size}^3 box: {time.perf_counter() - t:.1f} s")
+    valid = box.mapping.valid_mask()
+    ctx = S.make_context(0)
+    out = {"box": size, "iterations": iters}
+    for st in ("fp32", "bf16", "fp16"):
+        gb = S.upload(ctx, box)
+        sol = S.PottsSolver(ctx, gb, c, lam, tau, tau, storage=st)
+        sol.iterate(iters)
+        V, _ = sol.state()
+        res = sol.labels().to_host()
+        sol.close()
+        gb.close()
+        out[st] = {"label_agreement": float((res[valid] == r64["res"][valid]).mean()),
+                   "v_inf": float(np.abs(V[:, valid] - r64["V"][:, valid]).max())}
+        log(f"{size}^3 box, {st}: agreement {out[st]['label_agreement']:.6f}, ||V - V64||inf {out[st]['v_inf']:.3e}")
+    return out
+
+
 def copy_calibration(mib: int) -> int:
     """A known-size copy: FETCH_SIZE and WRITE_SIZE of its kernel should both read mib MiB."""
     import ctypes as C
@@ -296,6 +343,12 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing run")
     ap.add_argument("--bf16-steps", type=int, default=30)
     ap.add_argument("--no-tv", action="store_true", help="skip the Chambolle TV + histogram workload")
+    ap.add_argument("--no-long", action="store_true", help="skip the 1000-iteration solve with the FP64 gap")
+    ap.add_argument("--long-iters", type=int, default=1000)
+    ap.add_argument("--gap-every", type=int, default=50)
+    ap.add_argument("--agreement", action="store_true",
+                    help="also solve a 256^3 central box for --long-iters iterations in FP32 / BF16 / FP16 "
+                         "and compare with the oracle's FP64 solve (minutes of host time)")
     ap.add_argument("--tv-iters", type=int, default=200)
     ap.add_argument("--tv-cpu-iters", type=int, default=10)
     ap.add_argument("--tv-only", action="store_true",
@@ -425,7 +478,14 @@ def main() -> int:
         result["bf16_storage"] = {"ms_per_step": ms_bf16, "speedup_vs_fp32": ms_step / ms_bf16,
                                   "steps": args.bf16_steps}
         log(f"BF16 storage: {ms_bf16:.3f} ms/step, speedup {ms_step / ms_bf16:.3f}")
+    if not args.no_long:
+        result["long_run"] = long_run(S, ctx, g, c, lam, tau, args.long_iters, args.gap_every)
+        lr = result["long_run"]
+        log(f"{lr['iterations']} iterations with the FP64 gap every {lr['gap_every']}: {lr['ms_total'] / 1e3:.2f} s, "
+            f"relative gap {lr['relative_gap_trace'][0]:.3e} -> {lr['relative_gap_trace'][-1]:.3e}")
     g.close()
+    if args.agreement:
+        result["agreement"] = crop_agreement(S, ph, c, lam, tau, args.long_iters, 256)
     if not args.no_adjoint:
         result["adjoint"] = adjoint_ratio(S)
         log(f"adjoint: forward {result['adjoint']['forward_ms']:.2f} ms, reverse "
