@@ -50,7 +50,7 @@ __device__ __forceinline__ float tv_div(const TvGeom &g, const LineNbr &nb, uint
 // inside the line's storage.  The waves walk the lines in band order, every
 // XCD one contiguous eighth of it, so the neighbour lines come from the L2
 // that fetched them; the new xi is stored non-temporally (it is next read a
-// whole sweep later).
+// whole sweep later).  The kernel below walks runs of rows (tv_step_kernel).
 struct TvStep {
     const OrderedLine *ol;                     // the lines in band order
     uint32_t lines, len, chunks, stride, xg;   // chunks per line; xg: workgroups per XCD run
@@ -107,114 +107,140 @@ __device__ __forceinline__ float tv_div6(bool hpn, float x0m, bool hnn, float x0
     return d;
 }
 
+// Each wave walks a run of consecutive rows of one column (up to kMarchRows
+// lines of the band order) for one chunk of n.  Going from a line to the next
+// row reuses what the previous step loaded: the +row line's xi and g become
+// the line's own, its divergence d_r becomes d_o, and the +col line's xi_row
+// becomes the -row term of the next +col divergence, so a step loads only the
+// +row and +col lines (9 rows of lanes; 16 without the reuse).
 template <int VPL, bool TRACE>
-__global__ __launch_bounds__(256) void tv_step_kernel(TvStep a, float *dq)
+__global__ __launch_bounds__(256) void tv_step_kernel(TvStep a, const uint2 *__restrict__ runs, uint32_t n_runs,
+                                                      float *dq)
 {
     const int lane = threadIdx.x % kWave;
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x, a.xg);
     const uint64_t gw = (uint64_t(b) * blockDim.x + threadIdx.x) / kWave;
     float dd = 0.0f;
-    if (gw < uint64_t(a.lines) * a.chunks) {   // uniform over the wave
-        const OrderedLine L = a.ol[uint32_t(gw / a.chunks)];
+    if (gw < uint64_t(n_runs) * a.chunks) {   // uniform over the wave
+        const uint2 run = runs[uint32_t(gw / a.chunks)];
         const uint32_t n0 = uint32_t(gw % a.chunks) * (kWave * VPL) + lane * VPL;
         const uint32_t len = a.len;
         const int nv = n0 < len ? int(min(uint32_t(VPL), len - n0)) : 0;
-        const bool r_nrow = L.flags & kNrowHasNrow, r_ncol = L.flags & kNrowHasNcol;   // the +row line's
-        const bool c_nrow = L.flags & kNcolHasNrow, c_ncol = L.flags & kNcolHasNcol;   // the +col line's
         const auto at = [&](int32_t l) { return uint64_t(l) * a.stride; };
-        const uint64_t lo = at(L.line);
+        OrderedLine L = a.ol[run.x];
+        uint64_t lo = at(L.line);
 
-        // xi and g at the voxel (o), its -row / -col lines, its +row line (r)
-        // with that line's +row / -col / +col terms, its +col line (c) with
-        // that line's -row / +row / +col terms; absent terms stay 0.
-        float x0o[VPL], x1o[VPL], x2o[VPL], go[VPL], x1pr[VPL], x2pc[VPL];
-        float x0r[VPL], x1r[VPL], x2r[VPL], gr[VPL], x2rpc[VPL];
-        float x0c[VPL], x1c[VPL], x2c[VPL], gc[VPL], x1cpr[VPL];
+        // carried from line to line: xi and g of the line, xi_row of the
+        // -row line of the +col line, div xi of the line
+        float x0o[VPL], x1o[VPL], x2o[VPL], go[VPL], x1cpr[VPL], d_o[VPL];
+        {
+            float x1pr[VPL], x2pc[VPL];
 #pragma unroll
-        for (int j = 0; j < VPL; ++j)
-            x0o[j] = x1o[j] = x2o[j] = go[j] = x1pr[j] = x2pc[j] = x0r[j] = x1r[j] = x2r[j] = gr[j] = x2rpc[j] =
-                x0c[j] = x1c[j] = x2c[j] = gc[j] = x1cpr[j] = 0.0f;
-        if (nv) {
-            tv_ld(a.x0 + lo + n0, x0o);
-            tv_ld(a.x1 + lo + n0, x1o);
-            tv_ld(a.x2 + lo + n0, x2o);
-            tv_ld(a.g + lo + n0, go);
-            if (L.prow >= 0) tv_ld(a.x1 + at(L.prow) + n0, x1pr);
-            if (L.pcol >= 0) tv_ld(a.x2 + at(L.pcol) + n0, x2pc);
-            if (L.nrow >= 0) {
-                const uint64_t lr = at(L.nrow);
-                tv_ld(a.x0 + lr + n0, x0r);
-                tv_ld(a.g + lr + n0, gr);
-                if (r_nrow) tv_ld(a.x1 + lr + n0, x1r);
-                if (r_ncol) tv_ld(a.x2 + lr + n0, x2r);
-                if (L.nrow_pcol >= 0) tv_ld(a.x2 + at(L.nrow_pcol) + n0, x2rpc);
-            }
-            if (L.ncol >= 0) {
-                const uint64_t lc = at(L.ncol);
-                tv_ld(a.x0 + lc + n0, x0c);
-                tv_ld(a.g + lc + n0, gc);
-                if (c_nrow) tv_ld(a.x1 + lc + n0, x1c);
-                if (c_ncol) tv_ld(a.x2 + lc + n0, x2c);
+            for (int j = 0; j < VPL; ++j) x0o[j] = x1o[j] = x2o[j] = go[j] = x1cpr[j] = x1pr[j] = x2pc[j] = 0.0f;
+            if (nv) {
+                tv_ld(a.x0 + lo + n0, x0o);
+                tv_ld(a.x1 + lo + n0, x1o);
+                tv_ld(a.x2 + lo + n0, x2o);
+                tv_ld(a.g + lo + n0, go);
+                if (L.prow >= 0) tv_ld(a.x1 + at(L.prow) + n0, x1pr);
+                if (L.pcol >= 0) tv_ld(a.x2 + at(L.pcol) + n0, x2pc);
                 if (L.ncol_prow >= 0) tv_ld(a.x1 + at(L.ncol_prow) + n0, x1cpr);
             }
-        }
-        // xi_n at n - 1 on the three lines: the lane's previous element, the
-        // previous lane's last, or (first lane) the element before the chunk
-        float x0om = __shfl_up(x0o[VPL - 1], 1, kWave);
-        float x0rm = __shfl_up(x0r[VPL - 1], 1, kWave);
-        float x0cm = __shfl_up(x0c[VPL - 1], 1, kWave);
-        if (lane == 0 && nv && n0 > 0) {
-            x0om = a.x0[lo + n0 - 1];
-            if (L.nrow >= 0) x0rm = a.x0[at(L.nrow) + n0 - 1];
-            if (L.ncol >= 0) x0cm = a.x0[at(L.ncol) + n0 - 1];
-        }
-        float d_o[VPL], d_r[VPL], d_c[VPL];
+            float x0om = __shfl_up(x0o[VPL - 1], 1, kWave);
+            if (lane == 0 && nv && n0 > 0) x0om = a.x0[lo + n0 - 1];
 #pragma unroll
-        for (int j = 0; j < VPL; ++j) {
-            const uint32_t n = n0 + j;
-            const bool hpn = n > 0, hnn = n + 1 < len;
-            const float pm = j ? x0o[j - 1] : x0om, rm = j ? x0r[j - 1] : x0rm, cm = j ? x0c[j - 1] : x0cm;
-            d_o[j] = tv_div6(hpn, pm, hnn, x0o[j], L.prow >= 0, x1pr[j], L.nrow >= 0, x1o[j], L.pcol >= 0, x2pc[j],
-                             L.ncol >= 0, x2o[j]);
-            // the -row line of the +row line is this line; the -col line of the +col line is this line
-            d_r[j] = tv_div6(hpn, rm, hnn, x0r[j], true, x1o[j], r_nrow, x1r[j], L.nrow_pcol >= 0, x2rpc[j], r_ncol,
-                             x2r[j]);
-            d_c[j] = tv_div6(hpn, cm, hnn, x0c[j], L.ncol_prow >= 0, x1cpr[j], c_nrow, x1c[j], true, x2o[j], c_ncol,
-                             x2c[j]);
-        }
-        // div xi and g at n + VPL (the element after the lane's last)
-        float d_next = __shfl_down(d_o[0], 1, kWave);
-        float g_next = __shfl_down(go[0], 1, kWave);
-        if (lane == kWave - 1 && nv == VPL && n0 + VPL < len) {
-            const uint64_t m = lo + n0 + VPL;
-            const float x1p = L.prow >= 0 ? a.x1[at(L.prow) + n0 + VPL] : 0.0f;
-            const float x2p = L.pcol >= 0 ? a.x2[at(L.pcol) + n0 + VPL] : 0.0f;
-            d_next = tv_div6(true, x0o[VPL - 1], n0 + VPL + 1 < len, a.x0[m], L.prow >= 0, x1p, L.nrow >= 0, a.x1[m],
-                             L.pcol >= 0, x2p, L.ncol >= 0, a.x2[m]);
-            g_next = a.g[m];
-        }
-        float y0[VPL], y1[VPL], y2[VPL];
-#pragma unroll
-        for (int j = 0; j < VPL; ++j) {
-            const uint32_t n = n0 + j;
-            const float dn = j + 1 < VPL ? d_o[j + 1] : d_next, gn = j + 1 < VPL ? go[j + 1] : g_next;
-            float g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
-            if (n + 1 < len) g1 = (dn - d_o[j]) - (gn - go[j]) / a.lambda;
-            if (L.nrow >= 0) g2 = (d_r[j] - d_o[j]) - (gr[j] - go[j]) / a.lambda;
-            if (L.ncol >= 0) g3 = (d_c[j] - d_o[j]) - (gc[j] - go[j]) / a.lambda;
-            const float den = __fmaf_rn(a.tau, sqrtf(__fmaf_rn(g3, g3, __fmaf_rn(g2, g2, g1 * g1))), 1.0f);
-            y0[j] = __fmaf_rn(a.tau, g1, x0o[j]) / den;
-            y1[j] = __fmaf_rn(a.tau, g2, x1o[j]) / den;
-            y2[j] = __fmaf_rn(a.tau, g3, x2o[j]) / den;
-            if (TRACE && j < nv) {
-                const float d1 = x0o[j] - y0[j], d2 = x1o[j] - y1[j], d3 = x2o[j] - y2[j];
-                dd = fmaxf(dd, __fmaf_rn(d3, d3, __fmaf_rn(d2, d2, d1 * d1)));
+            for (int j = 0; j < VPL; ++j) {
+                const uint32_t n = n0 + j;
+                d_o[j] = tv_div6(n > 0, j ? x0o[j - 1] : x0om, n + 1 < len, x0o[j], L.prow >= 0, x1pr[j], L.nrow >= 0,
+                                 x1o[j], L.pcol >= 0, x2pc[j], L.ncol >= 0, x2o[j]);
             }
         }
-        if (nv) {
-            tv_st(a.y0 + lo + n0, y0, nv);
-            tv_st(a.y1 + lo + n0, y1, nv);
-            tv_st(a.y2 + lo + n0, y2, nv);
+        for (uint32_t k = 0; k < run.y; ++k) {
+            const bool r_nrow = L.flags & kNrowHasNrow, r_ncol = L.flags & kNrowHasNcol;
+            const bool c_nrow = L.flags & kNcolHasNrow, c_ncol = L.flags & kNcolHasNcol;
+            float x0r[VPL], x1r[VPL], x2r[VPL], gr[VPL], x2rpc[VPL], x0c[VPL], x1c[VPL], x2c[VPL], gc[VPL];
+#pragma unroll
+            for (int j = 0; j < VPL; ++j)
+                x0r[j] = x1r[j] = x2r[j] = gr[j] = x2rpc[j] = x0c[j] = x1c[j] = x2c[j] = gc[j] = 0.0f;
+            if (nv) {
+                if (L.nrow >= 0) {
+                    const uint64_t lr = at(L.nrow);
+                    tv_ld(a.x0 + lr + n0, x0r);
+                    tv_ld(a.g + lr + n0, gr);
+                    if (r_nrow) tv_ld(a.x1 + lr + n0, x1r);
+                    if (r_ncol) tv_ld(a.x2 + lr + n0, x2r);
+                    if (L.nrow_pcol >= 0) tv_ld(a.x2 + at(L.nrow_pcol) + n0, x2rpc);
+                }
+                if (L.ncol >= 0) {
+                    const uint64_t lc = at(L.ncol);
+                    tv_ld(a.x0 + lc + n0, x0c);
+                    tv_ld(a.g + lc + n0, gc);
+                    if (c_nrow) tv_ld(a.x1 + lc + n0, x1c);
+                    if (c_ncol) tv_ld(a.x2 + lc + n0, x2c);
+                }
+            }
+            float x0rm = __shfl_up(x0r[VPL - 1], 1, kWave);
+            float x0cm = __shfl_up(x0c[VPL - 1], 1, kWave);
+            if (lane == 0 && nv && n0 > 0) {
+                if (L.nrow >= 0) x0rm = a.x0[at(L.nrow) + n0 - 1];
+                if (L.ncol >= 0) x0cm = a.x0[at(L.ncol) + n0 - 1];
+            }
+            float d_r[VPL], d_c[VPL];
+#pragma unroll
+            for (int j = 0; j < VPL; ++j) {
+                const uint32_t n = n0 + j;
+                const bool hpn = n > 0, hnn = n + 1 < len;
+                d_r[j] = tv_div6(hpn, j ? x0r[j - 1] : x0rm, hnn, x0r[j], true, x1o[j], r_nrow, x1r[j],
+                                 L.nrow_pcol >= 0, x2rpc[j], r_ncol, x2r[j]);
+                d_c[j] = tv_div6(hpn, j ? x0c[j - 1] : x0cm, hnn, x0c[j], L.ncol_prow >= 0, x1cpr[j], c_nrow, x1c[j],
+                                 true, x2o[j], c_ncol, x2c[j]);
+            }
+            float d_next = __shfl_down(d_o[0], 1, kWave);
+            float g_next = __shfl_down(go[0], 1, kWave);
+            if (lane == kWave - 1 && nv == VPL && n0 + VPL < len) {
+                const uint64_t m = lo + n0 + VPL;
+                const float x1p = L.prow >= 0 ? a.x1[at(L.prow) + n0 + VPL] : 0.0f;
+                const float x2p = L.pcol >= 0 ? a.x2[at(L.pcol) + n0 + VPL] : 0.0f;
+                d_next = tv_div6(true, x0o[VPL - 1], n0 + VPL + 1 < len, a.x0[m], L.prow >= 0, x1p, L.nrow >= 0,
+                                 a.x1[m], L.pcol >= 0, x2p, L.ncol >= 0, a.x2[m]);
+                g_next = a.g[m];
+            }
+            float y0[VPL], y1[VPL], y2[VPL];
+#pragma unroll
+            for (int j = 0; j < VPL; ++j) {
+                const uint32_t n = n0 + j;
+                const float dn = j + 1 < VPL ? d_o[j + 1] : d_next, gn = j + 1 < VPL ? go[j + 1] : g_next;
+                float g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
+                if (n + 1 < len) g1 = (dn - d_o[j]) - (gn - go[j]) / a.lambda;
+                if (L.nrow >= 0) g2 = (d_r[j] - d_o[j]) - (gr[j] - go[j]) / a.lambda;
+                if (L.ncol >= 0) g3 = (d_c[j] - d_o[j]) - (gc[j] - go[j]) / a.lambda;
+                const float den = __fmaf_rn(a.tau, sqrtf(__fmaf_rn(g3, g3, __fmaf_rn(g2, g2, g1 * g1))), 1.0f);
+                y0[j] = __fmaf_rn(a.tau, g1, x0o[j]) / den;
+                y1[j] = __fmaf_rn(a.tau, g2, x1o[j]) / den;
+                y2[j] = __fmaf_rn(a.tau, g3, x2o[j]) / den;
+                if (TRACE && j < nv) {
+                    const float e1 = x0o[j] - y0[j], e2 = x1o[j] - y1[j], e3 = x2o[j] - y2[j];
+                    dd = fmaxf(dd, __fmaf_rn(e3, e3, __fmaf_rn(e2, e2, e1 * e1)));
+                }
+            }
+            if (nv) {
+                tv_st(a.y0 + lo + n0, y0, nv);
+                tv_st(a.y1 + lo + n0, y1, nv);
+                tv_st(a.y2 + lo + n0, y2, nv);
+            }
+            if (k + 1 < run.y) {   // the next line of the run is this line's +row line
+                L = a.ol[run.x + k + 1];
+                lo = at(L.line);
+#pragma unroll
+                for (int j = 0; j < VPL; ++j) {
+                    x0o[j] = x0r[j];
+                    x1o[j] = x1r[j];
+                    x2o[j] = x2r[j];
+                    go[j] = gr[j];
+                    x1cpr[j] = x1c[j];
+                    d_o[j] = d_r[j];
+                }
+            }
         }
     }
     if constexpr (TRACE) {
@@ -243,19 +269,16 @@ __global__ __launch_bounds__(256) void tv_update_kernel(TvGeom g, float *__restr
 
 }  // namespace
 
-// Workgroups per XCD run: 0 gives every XCD one contiguous eighth of the band
-// order, so that the neighbour lines a workgroup reads were fetched into, or
-// will be read from, the same L2.
 template <int VPL>
-static void launch_tv_step(TvStep a, float *dq, hipStream_t st)
+static void launch_tv_step(TvStep a, const Geometry &geo, float *dq, hipStream_t st)
 {
-    const uint64_t waves = uint64_t(a.lines) * a.chunks;
+    const uint64_t waves = uint64_t(geo.n_runs) * a.chunks;
     const uint32_t grid = uint32_t((waves + 3) / 4);
     a.xg = std::max(1u, grid / kXcdCount);   // one contiguous eighth of the band order per XCD
     if (dq)
-        tv_step_kernel<VPL, true><<<grid, 256, 0, st>>>(a, dq);
+        tv_step_kernel<VPL, true><<<grid, 256, 0, st>>>(a, geo.d_runs, geo.n_runs, dq);
     else
-        tv_step_kernel<VPL, false><<<grid, 256, 0, st>>>(a, nullptr);
+        tv_step_kernel<VPL, false><<<grid, 256, 0, st>>>(a, geo.d_runs, geo.n_runs, nullptr);
 }
 
 // chambolle(), chambolle.cpp:225-276.  xi starts at 0 in both buffers; the
@@ -313,11 +336,11 @@ extern "C" int sct_chambolle(sct_ctx *ctx, sct_field *gf, float lambda, float ta
         s.y2 = y + 2 * g.span;
         float *dq = d_dq ? d_dq + it : nullptr;
         if (vpl == 4)
-            launch_tv_step<4>(s, dq, ctx->stream);
+            launch_tv_step<4>(s, geo, dq, ctx->stream);
         else if (vpl == 2)
-            launch_tv_step<2>(s, dq, ctx->stream);
+            launch_tv_step<2>(s, geo, dq, ctx->stream);
         else
-            launch_tv_step<1>(s, dq, ctx->stream);
+            launch_tv_step<1>(s, geo, dq, ctx->stream);
         e = hipGetLastError();
     }
     if (e == hipSuccess) {

@@ -173,7 +173,17 @@ int sct_geometry_build(sct_ctx *ctx, const sct_layout *l, Geometry &g)
             o.flags |= (c.nrow >= 0 ? kNcolHasNrow : 0u) | (c.ncol >= 0 ? kNcolHasNcol : 0u);
         }
     }
+    std::vector<uint2> runs;
+    for (uint32_t k = 0; k < g.lines; ++k) {
+        const uint32_t L = order[k];
+        if (k && L == order[k - 1] + 1 && g.line_col[L] == g.line_col[L - 1] && runs.back().y < kMarchRows)
+            ++runs.back().y;
+        else
+            runs.push_back(make_uint2(k, 1u));
+    }
+    g.n_runs = uint32_t(runs.size());
     int rc;
+    if ((rc = sct_alloc(ctx, (void **)&g.d_runs, sizeof(uint2) * std::max<size_t>(1, runs.size())))) return rc;
     if ((rc = sct_alloc(ctx, (void **)&g.d_nbr, sizeof(LineNbr) * std::max<size_t>(1, g.lines)))) return rc;
     if ((rc = sct_alloc(ctx, (void **)&g.d_oline, sizeof(OrderedLine) * std::max<size_t>(1, g.lines)))) return rc;
     if ((rc = sct_alloc(ctx, (void **)&g.d_start_row, sizeof(uint32_t) * g.width))) return rc;
@@ -181,6 +191,8 @@ int sct_geometry_build(sct_ctx *ctx, const sct_layout *l, Geometry &g)
     if (g.lines) {
         SCT_HIP(ctx, hipMemcpyAsync(g.d_nbr, g.h_nbr.data(), sizeof(LineNbr) * g.lines, hipMemcpyHostToDevice, ctx->stream));
         SCT_HIP(ctx, hipMemcpyAsync(g.d_oline, oline.data(), sizeof(OrderedLine) * g.lines, hipMemcpyHostToDevice,
+                                    ctx->stream));
+        SCT_HIP(ctx, hipMemcpyAsync(g.d_runs, runs.data(), sizeof(uint2) * runs.size(), hipMemcpyHostToDevice,
                                     ctx->stream));
     }
     SCT_HIP(ctx, hipMemcpyAsync(g.d_start_row, g.start_row.data(), sizeof(uint32_t) * g.width, hipMemcpyHostToDevice, ctx->stream));
@@ -193,10 +205,12 @@ void sct_geometry_release(sct_ctx *ctx, Geometry &g)
 {
     sct_free(ctx, g.d_nbr);
     sct_free(ctx, g.d_oline);
+    sct_free(ctx, g.d_runs);
     sct_free(ctx, g.d_start_row);
     sct_free(ctx, g.d_base);
     g.d_nbr = nullptr;
     g.d_oline = nullptr;
+    g.d_runs = nullptr;
     g.d_start_row = g.d_base = nullptr;
 }
 

@@ -33,6 +33,7 @@ struct LineNbr {
 // that fetched them.  Only the lines at band edges have a row neighbour in
 // another band.
 constexpr uint32_t kBandRows = 32;
+constexpr uint32_t kMarchRows = 8;   // lines a marching wave walks (Chambolle)
 
 // One line of the band order with everything a stencil reaching one line in
 // each direction, and the diagonal lines a divergence at the +row / +col line
@@ -54,6 +55,10 @@ struct Geometry {
     std::vector<uint32_t> line_col, line_row;
     LineNbr *d_nbr = nullptr;                // device [lines]
     OrderedLine *d_oline = nullptr;          // device [lines], in band order
+    // Runs of up to kMarchRows consecutive rows of one column in the band
+    // order: (first position in the band order, number of lines).
+    uint32_t n_runs = 0;
+    uint2 *d_runs = nullptr;                 // device [n_runs]
     uint32_t *d_start_row = nullptr, *d_base = nullptr;   // device mapping tables
 
     uint32_t row_end(uint32_t col) const { return start_row[col] + (base[col + 1] - base[col]) / nstride; }

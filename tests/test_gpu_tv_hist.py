@@ -40,10 +40,12 @@ def test_chambolle_bitwise_vs_fp32_mirror(ctx, ph24, lam):
     assert np.array_equal(_bits(dq), _bits(dq32))
 
 
-@pytest.mark.parametrize("n_images,nstride", [(200, None), (129, None), (37, 37), (130, 131), (256, None)])
+@pytest.mark.parametrize("n_images,nstride", [(200, None), (129, None), (37, 37), (130, 131), (256, None),
+                                              (33, 34)])
 def test_chambolle_chunk_edges_bitwise(ctx, n_images, nstride):
     """Lines longer than one chunk, partial chunks, odd lengths and odd
-    strides (one voxel per lane): bitwise vs the FP32 mirror, dq trace too."""
+    strides (one voxel per lane; nstride 34: two), bitwise vs the FP32 mirror,
+    dq trace too."""
     mp = S.Mapping.cylinder(n_images, 21, 19, 9.3, 8.7, 8.9, nstride)
     rng = np.random.default_rng(n_images)
     data = np.zeros(mp.span, np.float32)
