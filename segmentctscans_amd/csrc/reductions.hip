@@ -116,16 +116,16 @@ __device__ __forceinline__ int32_t hist_bin(float x, float rs, float re, uint32_
     return max(0, min(int32_t(nbins) - 1, b));
 }
 
-// calc_hist_kernel (calc_hist.cpp:23-80).  MODE 0: u32 counters in LDS;
-// MODE 1: two u16 counters per LDS word, a block never counts more than
-// 65535 voxels so a half-word cannot carry into its neighbour; MODE 2:
-// device atomics only (bin counts too large for LDS).
+// calc_hist_kernel (calc_hist.cpp:23-80).  MODE 0: u32 counters in LDS
+// (up to kSmallBins bins); MODE 2: device atomics only (bin counts too large
+// for LDS).  Larger histograms up to kPartBins * kMaxParts bins use
+// hist_part_kernel below.
 template <int MODE, bool DITHER>
 __global__ __launch_bounds__(256) void hist_kernel(VoxGeom g, const float *__restrict__ u, HistArgs a,
                                                    uint32_t *__restrict__ counts)
 {
     extern __shared__ uint32_t sh[];
-    const uint32_t nwords = MODE == 0 ? a.nbins : (MODE == 1 ? (a.nbins + 1) / 2 : 0);
+    const uint32_t nwords = MODE == 0 ? a.nbins : 0;
     if constexpr (MODE != 2) {
         for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) sh[i] = 0;
         __syncthreads();
@@ -147,7 +147,6 @@ __global__ __launch_bounds__(256) void hist_kernel(VoxGeom g, const float *__res
         if (a.rs - half <= x && x + half <= a.re) {
             const int32_t b = hist_bin(x, a.rs, a.re, a.nbins);
             if constexpr (MODE == 0) atomicAdd(&sh[b], 1u);
-            else if constexpr (MODE == 1) atomicAdd(&sh[b >> 1], 1u << (16 * (b & 1)));
             else atomicAdd(&counts[b], 1u);
         }
     }
@@ -156,14 +155,76 @@ __global__ __launch_bounds__(256) void hist_kernel(VoxGeom g, const float *__res
         for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) {
             const uint32_t c = sh[i];
             if (c == 0) continue;
-            if constexpr (MODE == 0) {
-                atomicAdd(&counts[i], c);
-            } else {
-                if (c & 0xffffu) atomicAdd(&counts[2 * i], c & 0xffffu);
-                if (c >> 16) atomicAdd(&counts[2 * i + 1], c >> 16);
+            atomicAdd(&counts[i], c);
+        }
+    }
+}
+
+// Histograms of more than kSmallBins bins: the bins are cut into partitions of
+// kPartBins u32 LDS counters (128 KB) and the lines into slices; workgroup
+// (slice, partition) counts its slice's voxels that fall in its partition and
+// stores the counts as the slice's row of `partial` (no atomics on device
+// memory); hist_sum_kernel adds the rows.  Each voxel is read once per
+// partition.  1024-thread workgroups: one per CU (the LDS), 16 waves.
+constexpr uint32_t kSmallBins = 8192;
+constexpr uint32_t kPartBins = 32768;
+constexpr uint32_t kMaxParts = 8;
+constexpr int kPartThreads = 1024;
+
+template <bool DITHER>
+__global__ __launch_bounds__(kPartThreads) void hist_part_kernel(VoxGeom g, const float *__restrict__ u, HistArgs a,
+                                                                 uint32_t parts, uint32_t *__restrict__ partial)
+{
+    extern __shared__ uint32_t sh[];
+    const uint32_t part = blockIdx.x % parts, slice = blockIdx.x / parts;
+    const uint32_t b0 = part * kPartBins, nb = min(kPartBins, a.nbins - b0);
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) sh[i] = 0;
+    __syncthreads();
+    const uint32_t l0 = slice * a.lines_per_block;
+    const uint32_t l1 = min(g.lines, l0 + a.lines_per_block);
+    const uint32_t lane = threadIdx.x % kWave, nw = blockDim.x / kWave;
+    const uint32_t pairs = l0 < l1 ? (l1 - l0) * g.chunks : 0;
+    const float half = a.delta / 2;
+    constexpr int U = 4;   // independent loads in flight per lane
+    for (uint32_t q0 = threadIdx.x / kWave; q0 < pairs; q0 += nw * U) {
+        float x[U];
+        uint32_t line[U], n[U];
+        bool ok[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const uint32_t q = q0 + uint32_t(k) * nw;
+            line[k] = l0 + q / g.chunks;
+            n[k] = (q % g.chunks) * kWave + lane;
+            ok[k] = q < pairs && n[k] < g.len;
+            x[k] = ok[k] ? u[uint64_t(line[k]) * g.stride + n[k]] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            if (!ok[k]) continue;
+            float v = x[k];
+            if constexpr (DITHER) {
+                const uint32_t r = philox_first(line[k], n[k], a.k0, a.k1);
+                v = v + (-half + a.delta * (float(r >> 8) * (1.0f / 16777216.0f)));
+            }
+            if (a.rs - half <= v && v + half <= a.re) {
+                const uint32_t b = uint32_t(hist_bin(v, a.rs, a.re, a.nbins)) - b0;
+                if (b < nb) atomicAdd(&sh[b], 1u);
             }
         }
     }
+    __syncthreads();
+    uint32_t *row = partial + uint64_t(slice) * a.nbins + b0;
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) row[i] = sh[i];
+}
+
+__global__ __launch_bounds__(256) void hist_sum_kernel(const uint32_t *__restrict__ partial, uint32_t slices,
+                                                       uint32_t nbins, uint32_t *__restrict__ counts)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbins) return;
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < slices; ++k) s += partial[uint64_t(k) * nbins + b];
+    counts[b] = s;
 }
 
 // grad_kernel of grad_magn.cpp:17-58.
@@ -277,45 +338,51 @@ extern "C" int sct_calc_hist(sct_ctx *ctx, const sct_field *f, float rs, float r
     a.k0 = uint32_t(seed);
     a.k1 = uint32_t(seed >> 32);
     const bool dither = delta != 0.0f;
-    int mode = nbins <= 32768 ? 0 : ((nbins <= 65536 && g.len <= 65535) ? 1 : 2);
-    size_t lds = mode == 0 ? sizeof(uint32_t) * nbins : (mode == 1 ? sizeof(uint32_t) * ((nbins + 1) / 2) : 0);
-    if (mode == 1) {
-        a.lines_per_block = std::max<uint32_t>(1, 65535u / std::max<uint32_t>(1, g.len));
-    } else {
-        const uint32_t blocks = std::max<uint32_t>(1, ctx->cu_count * 4);
-        a.lines_per_block = std::max<uint32_t>(1, (g.lines + blocks - 1) / blocks);
-    }
-    const uint32_t grid = std::max<uint32_t>(1, (g.lines + a.lines_per_block - 1) / a.lines_per_block);
-    auto launch = [&](auto kernel) -> hipError_t {
-        if (lds > 65536) {
+    const uint32_t parts = (nbins + kPartBins - 1) / kPartBins;
+    int mode = nbins <= kSmallBins ? 0 : (parts <= kMaxParts ? 3 : 2);
+    uint32_t *partial = nullptr;
+    if (mode == 3 && g.lines && g.len) {
+        // slices: about two workgroups per CU over all partitions
+        const uint32_t slices = std::max<uint32_t>(1, std::min<uint32_t>(g.lines, 2 * ctx->cu_count / parts));
+        a.lines_per_block = (g.lines + slices - 1) / slices;
+        const uint32_t used = (g.lines + a.lines_per_block - 1) / a.lines_per_block;
+        rc = sct_alloc(ctx, (void **)&partial, sizeof(uint32_t) * uint64_t(used) * nbins);
+        if (rc) {
+            sct_free(ctx, d);
+            return rc;
+        }
+        const size_t lds = sizeof(uint32_t) * std::min(nbins, kPartBins);
+        auto launch = [&](auto kernel) -> hipError_t {
             hipError_t ea = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
             if (ea != hipSuccess) return ea;
+            kernel<<<used * parts, kPartThreads, lds, ctx->stream>>>(g, (const float *)f->data, a, parts, partial);
+            return hipGetLastError();
+        };
+        if (e == hipSuccess) e = dither ? launch(hist_part_kernel<true>) : launch(hist_part_kernel<false>);
+        if (e == hipSuccess) {
+            hist_sum_kernel<<<(nbins + 255) / 256, 256, 0, ctx->stream>>>(partial, used, nbins, d);
+            e = hipGetLastError();
         }
-        kernel<<<grid, 256, lds, ctx->stream>>>(g, (const float *)f->data, a, d);
-        return hipGetLastError();
-    };
-    if (e == hipSuccess && g.lines && g.len) {
-        if (mode == 0) e = dither ? launch(hist_kernel<0, true>) : launch(hist_kernel<0, false>);
-        else if (mode == 1) e = dither ? launch(hist_kernel<1, true>) : launch(hist_kernel<1, false>);
-        else e = dither ? launch(hist_kernel<2, true>) : launch(hist_kernel<2, false>);
-        if (e != hipSuccess && mode != 2) {
-            // LDS request refused: count with device atomics instead.
-            (void)hipGetLastError();
-            mode = 2;
-            lds = 0;
-            const uint32_t blocks = std::max<uint32_t>(1, ctx->cu_count * 4);
-            a.lines_per_block = std::max<uint32_t>(1, (g.lines + blocks - 1) / blocks);
-            e = hipMemsetAsync(d, 0, sizeof(uint32_t) * nbins, ctx->stream);
-            if (e == hipSuccess) {
-                const uint32_t grid2 = std::max<uint32_t>(1, (g.lines + a.lines_per_block - 1) / a.lines_per_block);
-                if (dither) hist_kernel<2, true><<<grid2, 256, 0, ctx->stream>>>(g, (const float *)f->data, a, d);
-                else hist_kernel<2, false><<<grid2, 256, 0, ctx->stream>>>(g, (const float *)f->data, a, d);
-                e = hipGetLastError();
+    } else if (g.lines && g.len) {
+        const size_t lds = mode == 0 ? sizeof(uint32_t) * nbins : 0;
+        const uint32_t blocks = std::max<uint32_t>(1, ctx->cu_count * 4);
+        a.lines_per_block = std::max<uint32_t>(1, (g.lines + blocks - 1) / blocks);
+        const uint32_t grid = std::max<uint32_t>(1, (g.lines + a.lines_per_block - 1) / a.lines_per_block);
+        if (e == hipSuccess) {
+            if (mode == 0) {
+                if (dither) hist_kernel<0, true><<<grid, 256, lds, ctx->stream>>>(g, (const float *)f->data, a, d);
+                else hist_kernel<0, false><<<grid, 256, lds, ctx->stream>>>(g, (const float *)f->data, a, d);
+            } else {
+                if (dither) hist_kernel<2, true><<<grid, 256, 0, ctx->stream>>>(g, (const float *)f->data, a, d);
+                else hist_kernel<2, false><<<grid, 256, 0, ctx->stream>>>(g, (const float *)f->data, a, d);
             }
+            e = hipGetLastError();
         }
     }
     if (e == hipSuccess) e = hipMemcpyAsync(counts, d, sizeof(uint32_t) * nbins, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) (void)hipStreamSynchronize(ctx->stream);
+    sct_free(ctx, partial);
     sct_free(ctx, d);
     if (e != hipSuccess) return sct_hip_fail(ctx, e, "calc_hist");
     if (density) {

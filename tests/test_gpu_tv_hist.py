@@ -81,7 +81,7 @@ def test_tv_then_65536_bin_histogram_bit_exact(ctx, ph128):
         assert cnt.sum() == np.count_nonzero((got[m] >= mn) & (got[m] <= mx))
 
 
-@pytest.mark.parametrize("nbins", [1, 256, 1000, 1024, 32768, 65536, 100000])
+@pytest.mark.parametrize("nbins", [1, 256, 1000, 1024, 8192, 8193, 32768, 65536, 100000, 300000])
 def test_calc_hist_modes_bit_exact(ctx, ph128, nbins):
     lay = O.Layout.from_mapping(ph128.field.mapping)
     g = S.upload(ctx, ph128.field)
@@ -101,6 +101,12 @@ def test_calc_hist_dither_deterministic(ctx, ph128):
     plain = S.calc_hist(ctx, g, 0.0, 1.0, 1024, 0.0, counts=True)
     assert np.array_equal(a, b) and not np.array_equal(a, c)
     assert abs(int(a.sum()) - int(plain.sum())) < 0.01 * plain.sum()
+    # the dither of a voxel does not depend on the bins: every layout of the
+    # counters (LDS, partitioned LDS, device atomics) accepts the same voxels
+    for nb in (8193, 65536, 300000):
+        big = S.calc_hist(ctx, g, 0.0, 1.0, nb, 1e-3, seed=5, counts=True)
+        assert int(big.sum()) == int(a.sum())
+        assert np.array_equal(big, S.calc_hist(ctx, g, 0.0, 1.0, nb, 1e-3, seed=5, counts=True))
 
 
 def test_histogram_known_answers_on_device(ctx):
