@@ -154,6 +154,11 @@ constexpr int kXiVoxelsPerLane = 4;  // per lane and workgroup: kXiVoxelsPerLane
 constexpr int kXiBlockWaves = 4;     // default workgroup of the Xi kernel (its "tile": 64 BW lanes x 4 voxels)
 constexpr int kXiThreads = kXiBlockWaves * 64;
 constexpr int kXiWorkWaves = 4;      // at most this many waves drain the work list
+// Sweeps an item of the work list gets in the first stage; the few that need
+// more (about 16% of the queued items, up to 20 sweeps) are parked and finish
+// in a second stage on the fewest waves, so the long tail of sweep counts no
+// longer sets the time of every draining wave.
+constexpr int kXiFirstSweeps = 2;
 
 // Workgroups of the one-chunk-per-wave kernels, in XCD runs of 64 / VPL
 // (about 11 lines of 1426 voxels).
@@ -220,12 +225,19 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<
     constexpr int VPL = vpl_of<St>(), CPW = kXiVoxelsPerLane / VPL;
     constexpr int kXiItems = 128 * BW;     // work-list capacity per workgroup
     using P = Packed<St, VPL>;
+    // Parking pays for K <= 3; for K >= 4 the corrections of the parked
+    // items (NP = 6..15 pairs) cost LDS, hence workgroups per CU.
+    constexpr int kFirst = K <= 3 ? kXiFirstSweeps : 20;
+    constexpr int kXiLong = K <= 3 ? 32 * BW : 1;   // items parked after kFirst sweeps
     __shared__ float qb[kXiItems][K][3];   // inputs, then results, of the queued voxels
-    __shared__ int nq, next;
+    __shared__ float rb[kXiLong][NP][3];   // Dykstra corrections of the parked items
+    __shared__ int lt[kXiLong];            // parked item: index | sweeps << 16
+    __shared__ int nq, next, nl;
     const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
     if (threadIdx.x == 0) {
         nq = 0;
         next = 0;
+        nl = 0;
     }
     __syncthreads();
     const uint64_t total = lines.count();
@@ -345,9 +357,51 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<
 #pragma unroll
                         for (int ax = 0; ax < 3; ++ax) qb[t][i][ax] = q[i][ax];
                     have = false;
+                } else if (kFirst < 20 && sw == kFirst) {
+                    // park the item (q, r, sweep count) for the second stage
+                    // if there is room; otherwise keep sweeping it here
+                    const int slot = atomicAdd(&nl, 1);
+                    if (slot < kXiLong) {
+#pragma unroll
+                        for (int i = 0; i < K; ++i)
+#pragma unroll
+                            for (int ax = 0; ax < 3; ++ax) qb[t][i][ax] = q[i][ax];
+#pragma unroll
+                        for (int pp = 0; pp < NP; ++pp)
+#pragma unroll
+                            for (int ax = 0; ax < 3; ++ax) rb[slot][pp][ax] = r[pp][ax];
+                        lt[slot] = t | (sw << 16);
+                        have = false;
+                    }
                 }
             }
         }
+    }
+    if constexpr (kFirst < 20) {
+    __syncthreads();   // first stage done: the parked items continue on the fewest waves
+    const int nlong = min(nl, kXiLong);
+    for (int i = w * kWave + lane; w * kWave < nlong && i < nlong; i += int(blockDim.x)) {
+        const int t = lt[i] & 0xffff;
+        int sw = lt[i] >> 16;
+        float q[K][3], r[NP][3];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) q[k][ax] = qb[t][k][ax];
+#pragma unroll
+        for (int pp = 0; pp < NP; ++pp)
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) r[pp][ax] = rb[i][pp][ax];
+        while (true) {
+            const float m2 = dykstra_sweep<K>(q, r, lim);
+            ++sw;
+            if (m2 < lim.stol || sw == 20) break;
+        }
+#pragma unroll
+        for (int k = 0; k < KM; ++k)
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) qb[t][k][ax] = q[k][ax];
+    }
     }
     __syncthreads();   // results of the work list in LDS
 #pragma unroll
