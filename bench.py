@@ -190,7 +190,7 @@ def tv_workload(S, iters: int, cpu_iters: int) -> dict:
     return out
 
 
-def long_run(S, ctx, g, c, lam, tau, iters: int, gap_every: int) -> dict:
+def long_run(S, ctx, g, c, lam, tau, iters: int, gap_every: int, storage: str = "fp32") -> dict:
     """The full solve of the 1426 x 1024^2 workload: iters iterations with
     the FP64 duality gap every gap_every iterations (potts_min_partition with
     a gap trace), timed with HIP events; the gap evaluations are inside."""
@@ -198,13 +198,13 @@ def long_run(S, ctx, g, c, lam, tau, iters: int, gap_every: int) -> dict:
     tl = EventTimeline(ctx.stream, 2)
     ctx.synchronize()
     tl.mark()
-    res, gaps = S.potts_min_partition(ctx, g, c, lam, tau, tau, iters, gap_every=gap_every)
+    res, gaps = S.potts_min_partition(ctx, g, c, lam, tau, tau, iters, gap_every=gap_every, storage=storage)
     tl.mark()
     ms = float(tl.durations()[0])
     tl.close()
     res.close()
-    return {"iterations": iters, "gap_every": gap_every, "ms_total": ms, "ms_per_iteration": ms / iters,
-            "relative_gap_trace": [float(x) for x in gaps]}
+    return {"storage": storage, "iterations": iters, "gap_every": gap_every, "ms_total": ms,
+            "ms_per_iteration": ms / iters, "relative_gap_trace": [float(x) for x in gaps]}
 
 
 def crop_agreement(S, ph, c, lam, tau, iters: int, size: int) -> dict:
@@ -479,10 +479,12 @@ def main() -> int:
                                   "steps": args.bf16_steps}
         log(f"BF16 storage: {ms_bf16:.3f} ms/step, speedup {ms_step / ms_bf16:.3f}")
     if not args.no_long:
-        result["long_run"] = long_run(S, ctx, g, c, lam, tau, args.long_iters, args.gap_every)
-        lr = result["long_run"]
-        log(f"{lr['iterations']} iterations with the FP64 gap every {lr['gap_every']}: {lr['ms_total'] / 1e3:.2f} s, "
-            f"relative gap {lr['relative_gap_trace'][0]:.3e} -> {lr['relative_gap_trace'][-1]:.3e}")
+        for st in ("fp32", "bf16", "fp16"):
+            lr = long_run(S, ctx, g, c, lam, tau, args.long_iters, args.gap_every, st)
+            result["long_run" if st == "fp32" else f"long_run_{st}"] = lr
+            log(f"{st}: {lr['iterations']} iterations with the FP64 gap every {lr['gap_every']}: "
+                f"{lr['ms_total'] / 1e3:.2f} s, relative gap {lr['relative_gap_trace'][0]:.3e} -> "
+                f"{lr['relative_gap_trace'][-1]:.3e}")
     g.close()
     if args.agreement:
         result["agreement"] = crop_agreement(S, ph, c, lam, tau, args.long_iters, 256)
