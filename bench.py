@@ -237,6 +237,51 @@ def crop_agreement(S, ph, c, lam, tau, iters: int, size: int) -> dict:
     return out
 
 
+def reductions_workload(S, reps: int = 5) -> dict:
+    """The reductions on the 512^3 phantom (85,370,880 voxels): min_max,
+    calc_hist with 1024 and 65536 bins over [min, max], grad_magn,
+    grad_min_max and the 512 x 512 uv_hist.  Each call is timed with HIP
+    events on the library's stream (median of reps); achieved bandwidth is
+    the algorithmic bytes per voxel (4 for one read, 8 for two arrays) over
+    that time.  Calls include their set-up and, where the result goes to the
+    host, the copy back."""
+    from segmentctscans_amd.hiptime import EventTimeline
+    from segmentctscans_amd.phantom import make_phantom
+    ph = make_phantom(512, 512, 4.0, 20.0, seed=7)
+    vox = ph.field.mapping.data_size
+    ctx = S.make_context(0)
+    u = S.upload(ctx, ph.field)
+    v = S.grad_magn(ctx, u)
+    mn, mx = S.min_max(ctx, u)
+    v0, v1 = S.grad_min_max(ctx, v)
+    ops = {
+        "min_max": (4, lambda: S.min_max(ctx, u)),
+        "calc_hist_1024": (4, lambda: S.calc_hist(ctx, u, mn, mx, 1024, 0.0, counts=True)),
+        "calc_hist_65536": (4, lambda: S.calc_hist(ctx, u, mn, mx, 65536, 0.0, counts=True)),
+        "grad_magn": (8, lambda: S.grad_magn(ctx, u).close()),
+        "grad_min_max": (4, lambda: S.grad_min_max(ctx, v)),
+        "uv_hist_512": (8, lambda: S.uv_hist(ctx, u, v, mn, mx, v0, v1, 512)),
+    }
+    out = {"workload": f"512^3 cylinder r=230.4 ({vox} voxels)"}
+    for name, (bpv, fn) in ops.items():
+        fn()
+        ms = []
+        for _ in range(reps):
+            ctx.synchronize()
+            tl = EventTimeline(ctx.stream, 2)
+            tl.mark()
+            fn()
+            tl.mark()
+            ms.append(float(tl.durations()[0]))
+            tl.close()
+        t = float(np.median(ms))
+        out[name] = {"ms": t, "bytes_per_voxel": bpv, "achieved_GBs": bpv * vox / (t / 1e3) / 1e9,
+                     "frac": bpv * vox / (t / 1e3) / 1e9 / HBM_PEAK_GBS}
+    v.close()
+    u.close()
+    return out
+
+
 def copy_calibration(mib: int) -> int:
     """A known-size copy: FETCH_SIZE and WRITE_SIZE of its kernel should both read mib MiB."""
     import ctypes as C
@@ -353,6 +398,9 @@ def main() -> int:
     ap.add_argument("--tv-cpu-iters", type=int, default=10)
     ap.add_argument("--tv-only", action="store_true",
                     help="run only the Chambolle TV + histogram workload (512^3) and print its JSON")
+    ap.add_argument("--no-reductions", action="store_true", help="skip the reductions workload")
+    ap.add_argument("--reductions-only", action="store_true",
+                    help="run only the reductions on the 512^3 phantom and print their JSON")
     ap.add_argument("--batch", action="store_true",
                     help="run the batched workload (576 solves) instead of the single-volume one")
     ap.add_argument("--batch-volumes", type=int, default=64)
@@ -366,6 +414,12 @@ def main() -> int:
         return batch_workload(args)
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    if args.reductions_only:
+        import segmentctscans_amd as S
+        r = reductions_workload(S)
+        log(", ".join(f"{k} {v['ms']:.3f} ms ({v['frac']:.2f})" for k, v in r.items() if isinstance(v, dict)))
+        print(json.dumps(r), flush=True)
+        return 0
     if args.tv_only:
         import segmentctscans_amd as S
         r = tv_workload(S, args.tv_iters, 0 if args.no_cpu_baseline else args.tv_cpu_iters)
@@ -497,6 +551,10 @@ def main() -> int:
         log(f"TV: {result['tv']['ms_per_iteration']:.3f} ms/iteration, "
             f"{result['tv']['voxel_iters_per_s']:.3e} voxel-iters/s, {result['tv']['frac']:.3f} of HBM peak; "
             f"65536-bin histogram {np.median(result['tv']['hist_ms']):.3f} ms")
+    if not args.no_reductions:
+        result["reductions"] = reductions_workload(S)
+        log("reductions: " + ", ".join(f"{k} {v['ms']:.3f} ms ({v['frac']:.2f} of peak)"
+                                       for k, v in result["reductions"].items() if isinstance(v, dict)))
     if not args.no_cpu_baseline:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(ph, c, tau, args.cpu_iters, args.cpu_size)

@@ -669,11 +669,9 @@ struct Tape : TapeBase {
     int export_v(sct_field **V)
     {
         const Geometry &geo = g->geo;
-        sct_layout l{geo.n_images, geo.height, geo.width, geo.nstride, geo.col_begin, geo.col_end,
-                     geo.start_row.data(), geo.base.data()};
         VFields<K> out{};
         for (int i = 0; i < K && !rc; ++i) {
-            rc = sct_field_create(ctx, &l, SCT_F32, &V[i]);
+            rc = sct_field_like(ctx, g, SCT_F32, &V[i]);
             if (!rc) {
                 out.p[i] = (float *)V[i]->data;
                 chk(hipMemsetAsync(out.p[i], 0, sizeof(float) * geo.span(), ctx->stream), "memset");
@@ -747,9 +745,7 @@ struct Tape : TapeBase {
             if (e != hipSuccess) rc = sct_hip_fail(ctx, e, "adjoint");
         }
         if (!rc && grad_u) {
-            sct_layout l{geo.n_images, geo.height, geo.width, geo.nstride, geo.col_begin, geo.col_end,
-                         geo.start_row.data(), geo.base.data()};
-            rc = sct_field_create(ctx, &l, SCT_F32, grad_u);
+            rc = sct_field_like(ctx, g, SCT_F32, grad_u);
             if (!rc) {
                 hipError_t e = hipMemsetAsync((*grad_u)->data, 0, sizeof(float) * geo.span(), st);
                 if (e == hipSuccess && geo.lines)

@@ -346,9 +346,7 @@ extern "C" int sct_potts_batch(sct_ctx *ctx, int n, const sct_potts_problem *pro
         for (size_t s = 0; s < G.probs.size() && !rc; ++s) {
             const int p = G.probs[s];
             const Geometry &geo = problems[p].g->geo;
-            sct_layout l{geo.n_images, geo.height, geo.width, geo.nstride, geo.col_begin, geo.col_end,
-                         geo.start_row.data(), geo.base.data()};
-            if ((rc = sct_field_create(ctx, &l, SCT_F32, &out[p]))) break;
+            if ((rc = sct_field_like(ctx, problems[p].g, SCT_F32, &out[p]))) break;
             const hipError_t e = hipMemcpyAsync(out[p]->data, G.R + G.r_off[s], sizeof(float) * geo.span(),
                                                 hipMemcpyDeviceToDevice, ctx->stream);
             if (e != hipSuccess) rc = sct_hip_fail(ctx, e, "batch result copy");

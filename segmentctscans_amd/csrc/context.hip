@@ -214,11 +214,41 @@ void sct_geometry_release(sct_ctx *ctx, Geometry &g)
     g.d_start_row = g.d_base = nullptr;
 }
 
+// A copy of a built geometry: the host tables by value, the device tables by
+// stream-ordered device-to-device copies (no host rebuild).
 int sct_geometry_copy(sct_ctx *ctx, const Geometry &src, Geometry &dst)
 {
-    sct_layout l{src.n_images, src.height, src.width, src.nstride, src.col_begin, src.col_end,
-                 src.start_row.data(), src.base.data()};
-    return sct_geometry_build(ctx, &l, dst);
+    dst = Geometry{};
+    dst.n_images = src.n_images;
+    dst.height = src.height;
+    dst.width = src.width;
+    dst.nstride = src.nstride;
+    dst.col_begin = src.col_begin;
+    dst.col_end = src.col_end;
+    dst.start_row = src.start_row;
+    dst.base = src.base;
+    dst.lines = src.lines;
+    dst.data_size = src.data_size;
+    dst.h_nbr = src.h_nbr;
+    dst.line_col = src.line_col;
+    dst.line_row = src.line_row;
+    dst.n_runs = src.n_runs;
+    int rc;
+    auto dup = [&](auto *&d, auto *s, size_t n) -> int {
+        const size_t bytes = sizeof(*s) * std::max<size_t>(1, n);
+        int r = sct_alloc(ctx, (void **)&d, bytes);
+        if (!r && n) {
+            const hipError_t e = hipMemcpyAsync(d, s, sizeof(*s) * n, hipMemcpyDeviceToDevice, ctx->stream);
+            if (e != hipSuccess) r = sct_hip_fail(ctx, e, "geometry copy");
+        }
+        return r;
+    };
+    if ((rc = dup(dst.d_nbr, src.d_nbr, src.lines))) return rc;
+    if ((rc = dup(dst.d_oline, src.d_oline, src.lines))) return rc;
+    if ((rc = dup(dst.d_runs, src.d_runs, src.n_runs))) return rc;
+    if ((rc = dup(dst.d_start_row, src.d_start_row, src.width))) return rc;
+    if ((rc = dup(dst.d_base, src.d_base, size_t(src.width) + 1))) return rc;
+    return SCT_OK;
 }
 
 // ---- context ---------------------------------------------------------------------
@@ -291,6 +321,26 @@ extern "C" int sct_field_create(sct_ctx *ctx, const sct_layout *layout, int dtyp
     f->dtype = dtype;
     int rc = sct_geometry_build(ctx, layout, f->geo);
     if (!rc) rc = sct_alloc(ctx, &f->data, f->geo.span() * elem_size(dtype));
+    if (rc) {
+        sct_geometry_release(ctx, f->geo);
+        delete f;
+        return rc;
+    }
+    *out = f;
+    return SCT_OK;
+}
+
+// A new field with the layout of src (its geometry copied on the device).
+int sct_field_like(sct_ctx *ctx, const sct_field *src, int dtype, sct_field **out)
+{
+    if (!ctx || !src || !out) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (dtype != SCT_F32 && dtype != SCT_U16) return sct_fail(ctx, SCT_ERR_INVALID, "bad dtype");
+    auto *f = new sct_field;
+    f->ctx = ctx;
+    f->dtype = dtype;
+    int rc = sct_geometry_copy(ctx, src->geo, f->geo);
+    if (!rc) rc = sct_alloc(ctx, &f->data, std::max<size_t>(1, f->geo.span()) * elem_size(dtype));
     if (rc) {
         sct_geometry_release(ctx, f->geo);
         delete f;

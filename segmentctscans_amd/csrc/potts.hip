@@ -500,9 +500,7 @@ extern "C" int sct_potts_min_partition(sct_ctx *ctx, const sct_field *g, int K, 
     if ((rc = potts_plan(ctx, g, K, pick_kernel(o), iters, &o, &plan))) return rc;
 
     sct_field *res = nullptr;
-    sct_layout l{g->geo.n_images, g->geo.height, g->geo.width, g->geo.nstride, g->geo.col_begin, g->geo.col_end,
-                 g->geo.start_row.data(), g->geo.base.data()};
-    if ((rc = sct_field_create(ctx, &l, SCT_F32, &res))) return rc;
+    if ((rc = sct_field_like(ctx, g, SCT_F32, &res))) return rc;
     SCT_HIP(ctx, hipMemsetAsync(res->data, 0, sizeof(float) * res->geo.span(), ctx->stream));
 
     sct_potts_solver s;

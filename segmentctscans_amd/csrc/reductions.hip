@@ -251,6 +251,31 @@ struct UvArgs {
     uint32_t nbins, hstride;
 };
 
+// counts[bin] += 1 for every lane of the wave with bin >= 0.  The lanes that
+// share the leader's bin are added with one device atomic, for up to
+// kAggRounds distinct bins; the rest add one each.  Neighbouring voxels
+// mostly fall in the same few bins (intensity phases, small gradients), so
+// this removes most of the contention on the hot counters.  Called by all
+// lanes of the wave.
+constexpr int kAggRounds = 8;
+
+__device__ __forceinline__ void wave_count(uint32_t *counts, long long bin)
+{
+    const uint32_t lane = threadIdx.x % kWave;
+    bool active = bin >= 0;
+    for (int r = 0; r < kAggRounds; ++r) {
+        const uint64_t act = __ballot(active);
+        if (!act) return;
+        const int leader = __ffsll((unsigned long long)act) - 1;
+        const long long key = __shfl(bin, leader, kWave);
+        const bool mine = active && bin == key;
+        const uint64_t same = __ballot(mine);
+        if (int(lane) == leader) atomicAdd(&counts[key], uint32_t(__popcll(same)));
+        if (mine) active = false;
+    }
+    if (active) atomicAdd(&counts[bin], 1u);
+}
+
 // The bin-id functor and histogram of uv-hist-kernel.cpp:121-185.
 __global__ __launch_bounds__(256) void uv_hist_kernel(VoxGeom g, const float *__restrict__ u,
                                                       const float *__restrict__ v, UvArgs a, uint32_t *counts)
@@ -261,14 +286,66 @@ __global__ __launch_bounds__(256) void uv_hist_kernel(VoxGeom g, const float *__
     for (uint64_t w = uint64_t(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; w < total; w += nwaves) {
         const uint32_t line = uint32_t(w / g.chunks);
         const uint32_t n = uint32_t(w % g.chunks) * kWave + lane;
-        if (n >= g.len) continue;
-        const uint64_t o = uint64_t(line) * g.stride + n;
-        const float x = u[o], y = v[o];
-        if (!(a.u0 <= x && x <= a.u1 && a.v0 <= y && y <= a.v1)) continue;
-        const int32_t xb = hist_bin(x, a.u0, a.u1, a.nbins);
-        const int32_t yb = hist_bin(y, a.v0, a.v1, a.nbins);
-        atomicAdd(&counts[uint64_t(yb) * a.hstride + uint64_t(xb)], 1u);
+        long long bin = -1;
+        if (n < g.len) {
+            const uint64_t o = uint64_t(line) * g.stride + n;
+            const float x = u[o], y = v[o];
+            if (a.u0 <= x && x <= a.u1 && a.v0 <= y && y <= a.v1) {
+                const int32_t xb = hist_bin(x, a.u0, a.u1, a.nbins);
+                const int32_t yb = hist_bin(y, a.v0, a.v1, a.nbins);
+                bin = (long long)yb * a.hstride + xb;
+            }
+        }
+        wave_count(counts, bin);
     }
+}
+
+// uv_hist with up to kPartBins * kMaxParts counters: workgroup (slice,
+// partition) counts the slice's voxels whose (v, u) bin falls in its 32768
+// counters of LDS and stores them as its slice's row; hist_sum_kernel adds
+// the rows.  Hot bins (the phases' intensities at small gradients) then cost
+// LDS atomics within a workgroup instead of device atomics from every CU on
+// one address.
+__global__ __launch_bounds__(kPartThreads) void uv_part_kernel(VoxGeom g, const float *__restrict__ u,
+                                                               const float *__restrict__ v, UvArgs a, uint32_t parts,
+                                                               uint32_t lines_per_slice, uint32_t total,
+                                                               uint32_t *__restrict__ partial)
+{
+    extern __shared__ uint32_t sh[];
+    const uint32_t part = blockIdx.x % parts, slice = blockIdx.x / parts;
+    const uint32_t b0 = part * kPartBins, nb = min(kPartBins, total - b0);
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) sh[i] = 0;
+    __syncthreads();
+    const uint32_t l0 = slice * lines_per_slice;
+    const uint32_t l1 = min(g.lines, l0 + lines_per_slice);
+    const uint32_t lane = threadIdx.x % kWave, nw = blockDim.x / kWave;
+    const uint32_t pairs = l0 < l1 ? (l1 - l0) * g.chunks : 0;
+    constexpr int U = 4;
+    for (uint32_t q0 = threadIdx.x / kWave; q0 < pairs; q0 += nw * U) {
+        float x[U], y[U];
+        bool ok[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const uint32_t q = q0 + uint32_t(k) * nw;
+            const uint32_t line = l0 + q / g.chunks;
+            const uint32_t n = (q % g.chunks) * kWave + lane;
+            ok[k] = q < pairs && n < g.len;
+            const uint64_t o = uint64_t(line) * g.stride + n;
+            x[k] = ok[k] ? u[o] : 0.0f;
+            y[k] = ok[k] ? v[o] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            if (!ok[k] || !(a.u0 <= x[k] && x[k] <= a.u1 && a.v0 <= y[k] && y[k] <= a.v1)) continue;
+            const uint32_t xb = uint32_t(hist_bin(x[k], a.u0, a.u1, a.nbins));
+            const uint32_t yb = uint32_t(hist_bin(y[k], a.v0, a.v1, a.nbins));
+            const uint32_t b = yb * a.hstride + xb - b0;
+            if (b < nb) atomicAdd(&sh[b], 1u);
+        }
+    }
+    __syncthreads();
+    uint32_t *row = partial + uint64_t(slice) * total + b0;
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) row[i] = sh[i];
 }
 
 int run_minmax(sct_ctx *ctx, const sct_field *f, bool positive, float *mn, float *mx)
@@ -412,10 +489,8 @@ extern "C" int sct_grad_magn(sct_ctx *ctx, const sct_field *u, sct_field **v)
     if (!ctx || !u || !v) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
     *v = nullptr;
     if (u->dtype != SCT_F32) return sct_fail(ctx, SCT_ERR_INVALID, "float field required");
-    sct_layout l{u->geo.n_images, u->geo.height, u->geo.width, u->geo.nstride, u->geo.col_begin, u->geo.col_end,
-                 u->geo.start_row.data(), u->geo.base.data()};
     sct_field *out = nullptr;
-    int rc = sct_field_create(ctx, &l, SCT_F32, &out);
+    int rc = sct_field_like(ctx, u, SCT_F32, &out);
     if (rc) return rc;
     const VoxGeom g = vox_geom(u);
     hipError_t e = hipMemsetAsync(out->data, 0, sizeof(float) * out->geo.span(), ctx->stream);
@@ -455,12 +530,37 @@ extern "C" int sct_uv_hist(sct_ctx *ctx, const sct_field *u, const sct_field *v,
     if (rc) return rc;
     const VoxGeom g = vox_geom(u);
     hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t) * total, ctx->stream);
-    if (e == hipSuccess && g.lines && g.len) {
+    const uint32_t parts = uint32_t((total + kPartBins - 1) / kPartBins);
+    uint32_t *partial = nullptr;
+    if (e == hipSuccess && g.lines && g.len && parts <= kMaxParts) {
+        // counters in LDS, by partitions of the bins (see uv_part_kernel)
+        const uint32_t slices = std::max<uint32_t>(1, std::min<uint32_t>(g.lines, 2 * ctx->cu_count / parts));
+        const uint32_t lps = (g.lines + slices - 1) / slices;
+        const uint32_t used = (g.lines + lps - 1) / lps;
+        rc = sct_alloc(ctx, (void **)&partial, sizeof(uint32_t) * uint64_t(used) * total);
+        if (rc) {
+            sct_free(ctx, d);
+            return rc;
+        }
+        const size_t lds = sizeof(uint32_t) * std::min<size_t>(total, kPartBins);
+        e = hipFuncSetAttribute((const void *)uv_part_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e == hipSuccess) {
+            uv_part_kernel<<<used * parts, kPartThreads, lds, ctx->stream>>>(
+                g, (const float *)u->data, (const float *)v->data, a, parts, lps, uint32_t(total), partial);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) {
+            hist_sum_kernel<<<uint32_t((total + 255) / 256), 256, 0, ctx->stream>>>(partial, used, uint32_t(total), d);
+            e = hipGetLastError();
+        }
+    } else if (e == hipSuccess && g.lines && g.len) {
         uv_hist_kernel<<<stream_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const float *)u->data, (const float *)v->data, a, d);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(counts, d, sizeof(uint32_t) * total, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) (void)hipStreamSynchronize(ctx->stream);
+    sct_free(ctx, partial);
     sct_free(ctx, d);
     if (e != hipSuccess) return sct_hip_fail(ctx, e, "uv_hist");
     return SCT_OK;
@@ -549,10 +649,8 @@ extern "C" int sct_convert_to_fp(sct_ctx *ctx, const sct_field *in, float lo, fl
     if (!ctx || !in || !out) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
     *out = nullptr;
     const Geometry &geo = in->geo;
-    sct_layout l{geo.n_images, geo.height, geo.width, geo.nstride, geo.col_begin, geo.col_end,
-                 geo.start_row.data(), geo.base.data()};
     sct_field *f = nullptr;
-    int rc = sct_field_create(ctx, &l, SCT_F32, &f);
+    int rc = sct_field_like(ctx, in, SCT_F32, &f);
     if (rc) return rc;
     const VoxGeom g = vox_geom(in);
     hipError_t e = hipMemsetAsync(f->data, 0, sizeof(float) * geo.span(), ctx->stream);

@@ -101,6 +101,7 @@ void sct_free(sct_ctx *ctx, void *p);
 int sct_geometry_build(sct_ctx *ctx, const sct_layout *l, Geometry &g);
 void sct_geometry_release(sct_ctx *ctx, Geometry &g);
 int sct_geometry_copy(sct_ctx *ctx, const Geometry &src, Geometry &dst);
+int sct_field_like(sct_ctx *ctx, const sct_field *src, int dtype, sct_field **out);
 
 // ---- device helpers --------------------------------------------------------
 constexpr int kWave = 64;

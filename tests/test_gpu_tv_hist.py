@@ -144,6 +144,10 @@ def test_grad_magn_uv_hist_bit_exact(ctx, ph128):
     h2, s2 = S.uv_hist(ctx, u, v, 0.2, 0.6, 0.0, 0.5, 100)
     w2, _ = O.uv_hist(lay, ph128.field.data, vh, 0.2, 0.6, 0.0, 0.5, 100)
     assert np.array_equal(h2.reshape(-1)[: w2.size], w2)
+    # more bins than the LDS partitions hold: device atomics, wave-aggregated
+    h3, _ = S.uv_hist(ctx, u, v, u0, u1, v0, v1, 600)
+    w3, _ = O.uv_hist(lay, ph128.field.data, vh, u0, u1, v0, v1, 600)
+    assert np.array_equal(h3.reshape(-1)[: w3.size], w3)
 
 
 def test_download_size_mismatch_raises(ctx, ph24):
