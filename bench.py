@@ -86,7 +86,9 @@ def cpu_baseline(ph, c, tau, iters: int, size: int) -> dict:
 
 def adjoint_ratio(S) -> dict:
     """Reverse pass / forward pass of the adjoint on the 128^3 workload (K = 3,
-    200 iterations, checkpoints every 16; median of 3 runs)."""
+    200 iterations; median of 3 runs), with checkpoints every 16 (the states
+    inside each segment recomputed) and with every state stored (no
+    recompute)."""
     from segmentctscans_amd import Field
     from segmentctscans_amd.phantom import make_phantom
     tau = float(np.float32(1.0 / np.sqrt(12.0)))
@@ -95,14 +97,19 @@ def adjoint_ratio(S) -> dict:
     g = S.upload(ctx, ph.field)
     t = S.upload(ctx, Field(ph.field.mapping, ph.labels.astype(np.float32)))
     c = ph.normalized_labels()
-    runs = [S.potts_adjoint(ctx, g, t, c, 30.0, tau, tau, 200, checkpoint_every=16, want_grad_u=False)
-            for _ in range(4)][1:]
-    fw = float(np.median([r["forward_ms"] for r in runs]))
-    rv = float(np.median([r["backward_ms"] for r in runs]))
+    def timed(ck):
+        runs = [S.potts_adjoint(ctx, g, t, c, 30.0, tau, tau, 200, checkpoint_every=ck, want_grad_u=False)
+                for _ in range(4)][1:]
+        return (float(np.median([r["forward_ms"] for r in runs])),
+                float(np.median([r["backward_ms"] for r in runs])))
+    fw, rv = timed(16)
+    fa, ra = timed(1)
     g.close()
     t.close()
     return {"workload": "128^3 cylinder (1,334,784 voxels), K=3, 200 iterations, checkpoints every 16",
-            "forward_ms": fw, "reverse_ms": rv, "ratio": rv / fw, "ratio_total": (fw + rv) / fw}
+            "forward_ms": fw, "reverse_ms": rv, "ratio": rv / fw, "ratio_total": (fw + rv) / fw,
+            "every_state_stored": {"forward_ms": fa, "reverse_ms": ra, "ratio": ra / fa,
+                                   "states": 201}}
 
 
 TV_BYTES = 28        # R xi 3, R g, W xi 3 (FP32) per voxel and iteration
@@ -545,7 +552,8 @@ def main() -> int:
     if not args.no_adjoint:
         result["adjoint"] = adjoint_ratio(S)
         log(f"adjoint: forward {result['adjoint']['forward_ms']:.2f} ms, reverse "
-            f"{result['adjoint']['reverse_ms']:.2f} ms, ratio {result['adjoint']['ratio']:.2f}")
+            f"{result['adjoint']['reverse_ms']:.2f} ms, ratio {result['adjoint']['ratio']:.2f}; every state "
+            f"stored: ratio {result['adjoint']['every_state_stored']['ratio']:.2f}")
     if not args.no_tv:
         result["tv"] = tv_workload(S, args.tv_iters, 0 if args.no_cpu_baseline else args.tv_cpu_iters)
         log(f"TV: {result['tv']['ms_per_iteration']:.3f} ms/iteration, "

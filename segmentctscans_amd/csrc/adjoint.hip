@@ -24,9 +24,16 @@
 // on the stored components.
 //
 // States are checkpointed every `ck` iterations during the forward pass; the
-// reverse pass recomputes each segment of ck states from its checkpoint
-// (memory ~ (T/ck + ck) states, ck = sqrt(T) by default) with the forward
-// kernels of the solver, so the replayed values are bitwise the forward's.
+// reverse pass recomputes the ck - 1 states inside each segment from its
+// checkpoint (memory ~ (T/ck + ck) states, ck = sqrt(T) by default; ck = 1
+// keeps every state and recomputes none) with the forward kernels of the
+// solver, so the replayed values are bitwise the forward's.
+//
+// Every forward step also stores its SweepRecord (per voxel: 0 when the
+// Dykstra input was feasible, else the sweeps it took) after the state.  B
+// passes the feasible voxels straight through and replays the others in
+// buckets of equal sweep count, so that a replay wave does not run as long as
+// its slowest of 64 unrelated voxels (about 3x the mean sweeps in queue order).
 #include "potts.hpp"
 #include "twopass.hpp"
 
@@ -40,7 +47,17 @@ using namespace sct::tp;
 namespace {
 
 constexpr int kAdjThreads = 256;
-constexpr int kQueues = 64;   // sub-queues of the Dykstra replay work list
+constexpr int kBuckets = 21;  // replay items by sweep count 1..20 (0: no replay)
+// Bucket counters are kept per stripe of consecutive workgroups, so that no
+// counter takes the device atomics of more than 1/kStripes of a grid.
+constexpr int kStripes = 64;
+// [stripe][bucket], one 128-byte line per stripe: device atomics on one line
+// are serialised, so a line takes only its own stripe's workgroups.
+constexpr int kStripeWords = 32;
+static_assert(kBuckets <= kStripeWords, "a stripe's counters fit one line");
+constexpr int kCounters = kStripes * kStripeWords;
+constexpr int kBucketWords = 2 * kCounters;
+static_assert(kStripes == kWave, "one stripe per lane in bucket_starts");
 
 // Work layout of the adjoint: state = V (K-1) | V-bar (K-1) | Xi 3(K-1)
 // components of wspan floats each.
@@ -151,19 +168,42 @@ __device__ __forceinline__ void vbar_adjoint(const Voxel &vx, uint64_t wspan, co
     for (int i = 0; i < K - 1; ++i) gvb[i] = full[i] - full[K - 1];
 }
 
+// Replay items of one reverse step, ordered by sweep count: hist[z][s] counts
+// the voxels of stripe z whose Dykstra took s sweeps (from the forward's
+// SweepRecord); offset[b][s] is where workgroup b's items of bucket s begin
+// within its stripe's (the value of hist[z][s] before b added its count).  Two
+// histograms alternate between reverse steps so that A can clear the next one.
+struct ReplayBuckets {
+    uint32_t *hist_cur, *hist_next, *offset;
+};
+
+__device__ __forceinline__ uint32_t stripe_of(uint32_t block, uint32_t blocks)
+{
+    return uint32_t(uint64_t(block) * kStripes / blocks);
+}
+
 // A: through V-bar+ = 2 V+ - V and V+ = Pi_S(q).  gs = gV | gXi.
-// Also empties the replay queue of the following B1 launch.
+// Also counts the replay items of the following B1 launch by sweep count
+// (rec1: the SweepRecord of the step s -> s1) and places each workgroup's
+// items within its stripe.
 template <int K>
 __global__ __launch_bounds__(kAdjThreads) void adj_v_kernel(VolumeLines L, uint64_t wspan, const float *st,
-                                                            const float *st1, const float *u, SolveParams<K> sp,
-                                                            float tau1, const float *GP, float *gs, float *GQ,
-                                                            float *gu, double *acc, uint32_t *queue_count)
+                                                            const float *st1, const uint8_t *rec1, const float *u,
+                                                            SolveParams<K> sp, float tau1, const float *GP,
+                                                            float *gs, float *GQ, float *gu, double *acc,
+                                                            ReplayBuckets rb)
 {
     constexpr int KM = K - 1;
-    if (blockIdx.x == 0 && threadIdx.x < kQueues) queue_count[threadIdx.x] = 0;
+    __shared__ uint32_t hb[kBuckets];
+    if (threadIdx.x < kBuckets) hb[threadIdx.x] = 0;
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < kCounters; i += blockDim.x) rb.hist_next[i] = 0;
+    __syncthreads();
     const Voxel vx = voxel_of(L);
     double part[K + 1] = {};
     if (vx.ok) {
+        const uint32_t sw = rec1[vx.w];
+        if (sw) atomicAdd(&hb[min(sw, uint32_t(kBuckets - 1))], 1u);
         const Chunk &ch = vx.ch;
         const uint32_t lane = threadIdx.x % kWave;
         float gvb[KM];
@@ -208,7 +248,13 @@ __global__ __launch_bounds__(kAdjThreads) void adj_v_kernel(VolumeLines L, uint6
         for (int i = 0; i < KM; ++i) gs[uint64_t(i) * wspan + vx.w] = gq[i] - gq[KM] - gvb[i];
         gu[vx.w] += gul;
     }
-    block_add<K + 1>(part, acc);
+    block_add<K + 1>(part, acc);   // its barrier also completes hb
+    if (threadIdx.x < kBuckets) {
+        const uint32_t h = hb[threadIdx.x];
+        const uint32_t o =
+            h ? atomicAdd(&rb.hist_cur[stripe_of(blockIdx.x, gridDim.x) * kStripeWords + threadIdx.x], h) : 0u;
+        rb.offset[uint64_t(blockIdx.x) * kStripeWords + threadIdx.x] = o;
+    }
 }
 
 // Dykstra's sweeps replayed from x with the solver's arithmetic, recording h.
@@ -344,32 +390,95 @@ __device__ __forceinline__ void xi_outputs(uint64_t wspan, uint32_t w, float tau
         for (int a = 0; a < 3; ++a) GP[uint64_t(3 * i + a) * wspan + w] = tau2 * g[i][a];
 }
 
-// Voxels whose Dykstra replay is needed, appended by workgroup b to sub-queue
-// b % kQueues (one atomic per workgroup); an item is x (K x 3), the output
-// adjoint g (K x 3) and the voxel's work offset.
+// Voxels whose Dykstra replay is needed, in buckets of equal sweep count,
+// longest first (bucket s starts after the items of buckets s+1..20), each
+// bucket stripe by stripe, so that the lanes of a replay wave run the same
+// number of sweeps on voxels of nearby workgroups.  An item is x (K x 3), the
+// output adjoint g (K x 3) and the voxel's work offset, item after item (a
+// workgroup's items of one bucket are contiguous).
 struct XiQueue {
     float *items;
-    uint32_t *count;   // [kQueues]
-    uint32_t cap;      // items per sub-queue
+    uint32_t cap;      // items
 };
+
+// Where stripe z's items of each bucket start in the item list (start[s]),
+// and the total (start[kBuckets]), into LDS, in two parts so that the loads
+// can be issued early: bucket_loads reads, per lane = stripe, the counts of
+// the buckets its wave scans (wave wv: buckets 1 + wv, 1 + wv + nw, ...);
+// bucket_starts scans them.  Called by every thread of a kAdjThreads
+// workgroup; bucket_starts ends with a barrier.
+constexpr uint32_t kScanWaves = kAdjThreads / kWave;
+constexpr int kPerWave = int((kBuckets - 1 + kScanWaves - 1) / kScanWaves);
+
+__device__ __forceinline__ void bucket_loads(const uint32_t *hist, uint32_t (&hv)[kPerWave])
+{
+    const uint32_t lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < kPerWave; ++k) {
+        const uint32_t s = 1 + wv + k * kScanWaves;
+        hv[k] = s < kBuckets ? hist[lane * kStripeWords + s] : 0u;
+    }
+}
+
+__device__ void bucket_starts(const uint32_t (&hv)[kPerWave], uint32_t z, uint32_t (&start)[kBuckets + 1])
+{
+    __shared__ uint32_t tot[kBuckets], before[kBuckets];
+    const uint32_t lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < kPerWave; ++k) {
+        const uint32_t s = 1 + wv + k * kScanWaves;
+        if (s >= kBuckets) break;
+        const uint32_t h = hv[k];
+        uint32_t incl = h;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o, kWave);
+            if (lane >= uint32_t(o)) incl += v;
+        }
+        const uint32_t t = __shfl(incl, kWave - 1, kWave);
+        const uint32_t b = __shfl(incl - h, z, kWave);
+        if (lane == 0) {
+            tot[s] = t;
+            before[s] = b;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int s = kBuckets - 1; s >= 1; --s) {
+            start[s] = acc + before[s];
+            acc += tot[s];
+        }
+        start[0] = acc;
+        start[kBuckets] = acc;
+    }
+    __syncthreads();
+}
+
 template <int K>
 constexpr int item_floats() { return 6 * K + 1; }
 
-// B1: div's adjoint (-grad of tau1 GQ); voxels whose Dykstra input is feasible
-// (one sweep, no change) pass the adjoint through Pi_K, the others are queued.
+// B1: div's adjoint (-grad of tau1 GQ).  Voxels whose Dykstra input was
+// feasible (SweepRecord 0: the projection was the identity) pass the adjoint
+// through Pi_K; the others recompute x = Xi + tau2 grad V-bar as the solver
+// does and go to the bucket of their sweep count, at the place kernel A
+// reserved for the workgroup (same grid).
 template <int K>
 __global__ __launch_bounds__(kAdjThreads) void adj_xi_kernel(VolumeLines L, uint64_t wspan, const float *st,
-                                                             float tau1, float tau2, DykstraLimits lim,
-                                                             const float *GQ, float *gs, float *GP, XiQueue Q)
+                                                             const uint8_t *rec1, float tau1, float tau2,
+                                                             const float *GQ, float *gs, float *GP, XiQueue Q,
+                                                             const uint32_t *hist, const uint32_t *offset)
 {
-    constexpr int KM = K - 1, IF = item_floats<K>();
-    __shared__ int nq;
-    __shared__ uint32_t base;
-    if (threadIdx.x == 0) nq = 0;
+    constexpr int KM = K - 1;
+    __shared__ uint32_t start[kBuckets + 1], cnt[kBuckets], base[kBuckets];
+    const uint32_t z = stripe_of(blockIdx.x, gridDim.x);
+    uint32_t hv[kPerWave];
+    bucket_loads(hist, hv);   // used after the voxel's work
+    if (threadIdx.x < kBuckets) cnt[threadIdx.x] = 0;
     __syncthreads();
     const Voxel vx = voxel_of(L);
     float *gXi = gs + uint64_t(KM) * wspan;
-    bool queued = false;
+    uint32_t sw = 0, rank = 0;
     float x[K][3], g[K][3];
     if (vx.ok) {
         const Chunk &ch = vx.ch;
@@ -385,65 +494,56 @@ __global__ __launch_bounds__(kAdjThreads) void adj_xi_kernel(VolumeLines L, uint
             for (int i = 0; i < KM; ++i) g[i][a] = gXi[uint64_t(3 * i + a) * wspan + vx.w] + full[i][a] - full[KM][a];
             g[KM][a] = 0.0f;
         }
-        // recompute x = Xi + tau2 grad V-bar as the solver does
-        const float *Vb = st + uint64_t(KM) * wspan, *Xi = st + uint64_t(2 * KM) * wspan;
-        float vb[KM], vbn[3][KM], xo[KM][3];
-        for (int i = 0; i < KM; ++i) {
-            vb[i] = Vb[uint64_t(i) * wspan + vx.w];
-            for (int a = 0; a < 3; ++a) {
-                vbn[a][i] = nx[a] ? Vb[uint64_t(i) * wspan + wn[a]] : 0.0f;
-                xo[i][a] = Xi[uint64_t(3 * i + a) * wspan + vx.w];
+        sw = min(uint32_t(rec1[vx.w]), uint32_t(kBuckets - 1));
+        if (sw) {
+            // recompute x = Xi + tau2 grad V-bar as the solver does
+            const float *Vb = st + uint64_t(KM) * wspan, *Xi = st + uint64_t(2 * KM) * wspan;
+            float vb[KM], vbn[3][KM], xo[KM][3];
+            for (int i = 0; i < KM; ++i) {
+                vb[i] = Vb[uint64_t(i) * wspan + vx.w];
+                for (int a = 0; a < 3; ++a) {
+                    vbn[a][i] = nx[a] ? Vb[uint64_t(i) * wspan + wn[a]] : 0.0f;
+                    xo[i][a] = Xi[uint64_t(3 * i + a) * wspan + vx.w];
+                }
             }
+            xi_input<K>(vb, vbn, nx, xo, tau2, x);
+            rank = atomicAdd(&cnt[sw], 1u);
+        } else {
+            xi_outputs<K>(wspan, vx.w, tau2, g, gXi, GP);
         }
-        xi_input<K>(vb, vbn, nx, xo, tau2, x);
-        float x1[K][3];
-        for (int i = 0; i < K; ++i)
-            for (int a = 0; a < 3; ++a) x1[i][a] = x[i][a];
-        queued = !dykstra_first_sweep_done<K>(x1, lim);
-        if (!queued) xi_outputs<K>(wspan, vx.w, tau2, g, gXi, GP);
     }
-    const int lane = threadIdx.x % kWave;
-    const uint64_t mask = __ballot(queued);
-    int p = 0;
-    if (mask) {
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        if (lane == leader) p = atomicAdd(&nq, __popcll(mask));
-        p = __shfl(p, leader, kWave) + __popcll(mask & ((1ull << lane) - 1ull));
-    }
-    __syncthreads();   // block count known
-    const uint32_t qi = blockIdx.x % kQueues;
-    if (threadIdx.x == 0) base = nq ? atomicAdd(&Q.count[qi], uint32_t(nq)) : 0u;
-    __syncthreads();
-    if (queued) {
-        float *it = Q.items + (uint64_t(qi) * Q.cap + base + uint32_t(p)) * IF;
-        for (int i = 0; i < K; ++i)
-            for (int a = 0; a < 3; ++a) {
-                it[3 * i + a] = x[i][a];
-                it[3 * K + 3 * i + a] = g[i][a];
-            }
-        it[6 * K] = __uint_as_float(vx.w);
+    if (threadIdx.x < kBuckets) base[threadIdx.x] = offset[uint64_t(blockIdx.x) * kStripeWords + threadIdx.x];
+    bucket_starts(hv, z, start);   // its barriers also publish base
+    if (sw) {
+        const uint32_t at = start[sw] + base[sw] + rank;
+        if (at < Q.cap) {
+            float *it = Q.items + uint64_t(at) * item_floats<K>();
+            for (int i = 0; i < K; ++i)
+                for (int a = 0; a < 3; ++a) {
+                    it[3 * i + a] = x[i][a];
+                    it[3 * K + 3 * i + a] = g[i][a];
+                }
+            it[6 * K] = __uint_as_float(vx.w);
+        }
     }
 }
 
-// B2: the queued voxels, one per lane: Dykstra replayed from x recording each
-// pair's h, then back-propagated.
+// B2: the bucketed voxels, one per lane (longest first): Dykstra replayed from
+// x recording each pair's h, then back-propagated.
 template <int K>
 __global__ __launch_bounds__(kAdjThreads) void adj_xi_replay_kernel(uint64_t wspan, float tau2, DykstraLimits lim,
-                                                                    float *gs, float *GP, XiQueue Q)
+                                                                    float *gs, float *GP, XiQueue Q,
+                                                                    const uint32_t *hist)
 {
-    constexpr int KM = K - 1, NP = K * (K - 1) / 2, IF = item_floats<K>();
-    __shared__ uint32_t start[kQueues + 1];
-    if (threadIdx.x == 0) {
-        start[0] = 0;
-        for (int q = 0; q < kQueues; ++q) start[q + 1] = start[q] + min(Q.count[q], Q.cap);
-    }
-    __syncthreads();
-    const uint32_t total = start[kQueues];
+    constexpr int KM = K - 1, NP = K * (K - 1) / 2;
+    __shared__ uint32_t start[kBuckets + 1];
+    uint32_t hv[kPerWave];
+    bucket_loads(hist, hv);
+    bucket_starts(hv, 0, start);
+    const uint32_t total = min(start[kBuckets], Q.cap);
     float *gXi = gs + uint64_t(KM) * wspan;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-        int q = 0;
-        while (start[q + 1] <= t) ++q;
-        const float *it = Q.items + (uint64_t(q) * Q.cap + (t - start[q])) * IF;
+        const float *it = Q.items + uint64_t(t) * item_floats<K>();
         float x[K][3], g[K][3];
         for (int i = 0; i < K; ++i)
             for (int a = 0; a < 3; ++a) {
@@ -478,7 +578,8 @@ __global__ __launch_bounds__(kAdjThreads) void adj_xi_replay_kernel(uint64_t wsp
 struct Buffers {
     std::vector<float *> ck, seg;
     float *gs = nullptr, *GQ = nullptr, *GP = nullptr, *gu = nullptr, *items = nullptr;
-    uint32_t *count = nullptr;
+    uint32_t *buckets = nullptr;   // ReplayBuckets: the two histograms
+    uint32_t *offset = nullptr;    // ReplayBuckets: kStripeWords per workgroup of A and B1
     double *acc = nullptr;   // dc[K], dlambda, loss
 };
 
@@ -486,7 +587,7 @@ void free_buffers(sct_ctx *ctx, Buffers &b)
 {
     for (float *p : b.ck) sct_free(ctx, p);
     for (float *p : b.seg) sct_free(ctx, p);
-    for (void *p : {(void *)b.gs, (void *)b.GQ, (void *)b.GP, (void *)b.gu, (void *)b.items, (void *)b.count,
+    for (void *p : {(void *)b.gs, (void *)b.GQ, (void *)b.GP, (void *)b.gu, (void *)b.items, (void *)b.buckets, (void *)b.offset,
                     (void *)b.acc})
         sct_free(ctx, p);
     b = Buffers{};
@@ -552,7 +653,7 @@ struct Tape : TapeBase {
     DykstraLimits lim;
     float tau1, tau2;
     uint32_t fgrid = 0, xgrid = 0, agrid = 0, rgrid = 0;
-    float *final = nullptr;   // the state after the last iteration (a recompute buffer)
+    float *final = nullptr;   // the state after the last iteration (seg[0], never recomputed into)
     int rc = SCT_OK;
 
     Tape(sct_ctx *c, const sct_field *gf, const float *cv, float lambda, float t1, float t2, int it, int ckp)
@@ -568,16 +669,16 @@ struct Tape : TapeBase {
     {
         if (e != hipSuccess && !rc) rc = sct_hip_fail(ctx, e, what);
     }
-    void copy(float *dst, const float *src)
-    {
-        chk(hipMemcpyAsync(dst, src, sizeof(float) * state, hipMemcpyDeviceToDevice, ctx->stream), "adjoint state copy");
-    }
     const float *u() const { return (const float *)g->data; }
+    // The SweepRecord of the step that produced a state, after its 5 (K - 1)
+    // components.
+    uint8_t *rec(const float *s) const { return (uint8_t *)(s + 5 * uint64_t(K - 1) * S); }
     void step(float *s)
     {
         hipStream_t st = ctx->stream;
         const auto a = state_args<K>(s, u(), S);
-        potts_xi_kernel<K, StorageF32><<<xgrid, kXiThreads, 0, st>>>(Ly.fwd, a, tau2, lim);
+        potts_xi_kernel<K, StorageF32, VolumeLines, kXiBlockWaves, true>
+            <<<xgrid, kXiThreads, 0, st>>>(Ly.fwd, a, tau2, lim, SweepRecord{rec(s)});
         potts_v_kernel<K, StorageF32, false><<<fgrid, 256, 0, st>>>(Ly.fwd, a, prm, tau1, nullptr);
     }
     // One iteration from state src into state dst (src unchanged): the Xi
@@ -595,7 +696,8 @@ struct Tape : TapeBase {
             av.V_out[i] = out.V[i];
             av.vbar_out[i] = out.vbar[i];
         }
-        potts_xi_kernel<K, StorageF32><<<xgrid, kXiThreads, 0, st>>>(Ly.fwd, ax, tau2, lim);
+        potts_xi_kernel<K, StorageF32, VolumeLines, kXiBlockWaves, true>
+            <<<xgrid, kXiThreads, 0, st>>>(Ly.fwd, ax, tau2, lim, SweepRecord{rec(dst)});
         potts_v_kernel<K, StorageF32, false><<<fgrid, 256, 0, st>>>(Ly.fwd, av, prm, tau1, nullptr);
     }
 
@@ -607,14 +709,17 @@ struct Tape : TapeBase {
         Ly.one = VolumeLines{geo.d_nbr, geo.lines, n, (n + 63) / 64, ws, geo.nstride, 0};
         Ly.wspan = uint64_t(geo.lines) * ws;
         S = std::max<uint64_t>(1, Ly.wspan);
-        state = 5 * uint64_t(K - 1) * S;
+        state = 5 * uint64_t(K - 1) * S + (S + 3) / 4;   // components, then the SweepRecord
         nck = (iters + ck - 1) / ck;
         seg_len = std::min(ck, iters);
         auto alloc = [&](float *&p, uint64_t elems) {
             if (!rc) rc = sct_alloc(ctx, (void **)&p, sizeof(float) * elems);
         };
         B.ck.assign(size_t(nck), nullptr);
-        B.seg.assign(size_t(seg_len) + 1, nullptr);
+        // seg[0]: the forward's working state, which ends as the final state;
+        // seg[1 .. seg_len - 1]: the states a segment recomputes between its
+        // two checkpoints
+        B.seg.assign(size_t(seg_len), nullptr);
         for (auto &p : B.ck) alloc(p, state);
         for (auto &p : B.seg) alloc(p, state);
         alloc(B.gs, 4 * uint64_t(K - 1) * S);   // gV | gXi
@@ -625,34 +730,42 @@ struct Tape : TapeBase {
         // per-workgroup sums: dc[K], dlambda of every A launch, then the loss
         nacc = uint64_t(agrid) * (K + 2);
         if (!rc) rc = sct_alloc(ctx, (void **)&B.acc, sizeof(double) * std::max<uint64_t>(1, nacc));
-        // replay queue: a sub-queue serves every kQueues-th workgroup of <= 256 voxels
-        Q = XiQueue{nullptr, nullptr, ((agrid + kQueues - 1) / kQueues) * uint32_t(kAdjThreads)};
-        alloc(B.items, uint64_t(kQueues) * Q.cap * item_floats<K>());
-        if (!rc) rc = sct_alloc(ctx, (void **)&B.count, sizeof(uint32_t) * kQueues);
+        // replay items: at most one per voxel slot of the one-voxel-per-lane grid
+        Q = XiQueue{nullptr, agrid * uint32_t(kAdjThreads)};
+        alloc(B.items, uint64_t(Q.cap) * item_floats<K>());
+        if (!rc) rc = sct_alloc(ctx, (void **)&B.buckets, sizeof(uint32_t) * kBucketWords);
+        if (!rc) rc = sct_alloc(ctx, (void **)&B.offset, sizeof(uint32_t) * kStripeWords * uint64_t(agrid));
         Q.items = B.items;
-        Q.count = B.count;
         rgrid = std::max<uint32_t>(1, ctx->cu_count * 4);
         fgrid = wave_grid(Ly.fwd.count());
         xgrid = xi_grid<StorageF32>(Ly.fwd.count());
         return rc;
     }
 
-    // iters iterations from the nearest-label start, checkpointing every ck.
+    // iters iterations from the nearest-label start.  State t = j ck is
+    // stepped straight into checkpoint j (no copies); the others are stepped
+    // in seg[0], which ends holding the final state.
     int forward(float *ms)
     {
         hipStream_t st = ctx->stream;
         hipEvent_t ev[2] = {};
         for (auto &e : ev)
             if (!rc && hipEventCreate(&e) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventCreate failed");
-        float *cur = B.seg.empty() ? nullptr : B.seg[0];
+        float *work = B.seg.empty() ? nullptr : B.seg[0];
+        float *cur = B.ck.empty() ? nullptr : B.ck[0];
         if (!rc) {
             chk(hipMemsetAsync(cur, 0, sizeof(float) * state, st), "memset");
+            chk(hipMemsetAsync(work, 0, sizeof(float) * state, st), "memset");
             chk(hipEventRecord(ev[0], st), "hipEventRecord");
             potts_init_kernel<K, StorageF32><<<fgrid, 256, 0, st>>>(Ly.fwd, state_args<K>(cur, u(), S), prm);
-            copy(B.ck[0], cur);
             for (int it = 0; it < iters && !rc; ++it) {
-                step(cur);
-                if ((it + 1) % ck == 0 && (it + 1) / ck < nck) copy(B.ck[(it + 1) / ck], cur);
+                const int t = it + 1;
+                float *dst = (t % ck == 0 && t / ck < nck) ? B.ck[size_t(t / ck)] : work;
+                if (dst == cur)
+                    step(cur);
+                else
+                    step_into(cur, dst);
+                cur = dst;
             }
             chk(hipEventRecord(ev[1], st), "hipEventRecord");
             chk(hipGetLastError(), "adjoint forward launch");
@@ -691,7 +804,6 @@ struct Tape : TapeBase {
         const Geometry &geo = g->geo;
         hipStream_t st = ctx->stream;
         const uint32_t n = geo.n_images, ws = sct_get_stride(n);
-        if (target && !final) return sct_fail(ctx, SCT_ERR_INVALID, "the final state was overwritten");
         hipEvent_t ev[2] = {};
         for (auto &e : ev)
             if (!rc && hipEventCreate(&e) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventCreate failed");
@@ -704,7 +816,7 @@ struct Tape : TapeBase {
         chk(hipMemsetAsync(B.gu, 0, sizeof(float) * S, st), "memset");
         chk(hipMemsetAsync(B.GP, 0, sizeof(float) * 3 * K * S, st), "memset");
         chk(hipMemsetAsync(B.acc, 0, sizeof(double) * std::max<uint64_t>(1, nacc), st), "memset");
-        chk(hipMemsetAsync(B.count, 0, sizeof(uint32_t) * kQueues, st), "memset");
+        chk(hipMemsetAsync(B.buckets, 0, sizeof(uint32_t) * kBucketWords, st), "memset");
         if (target) {
             const float wgt = float(1.0 / (double(std::max<uint64_t>(1, geo.data_size)) * K));
             adj_loss_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, final, (const float *)target->data, wgt,
@@ -716,27 +828,33 @@ struct Tape : TapeBase {
         }
         // reverse pass, one checkpoint segment at a time
         chk(hipEventRecord(ev[0], st), "hipEventRecord");
+        int par = 0;   // which bucket histogram the current reverse step uses
         for (int k = nck - 1; k >= 0 && !rc; --k) {
             const int t0 = k * ck, t1 = std::min(iters, t0 + ck);
-            // states t0 .. t1 of the segment: the checkpoint, then each
-            // recomputed from the previous one
-            std::vector<const float *> sg(size_t(t1 - t0) + 1);
+            // states t0 .. t1 of the segment: checkpoint k, the states between
+            // recomputed from it, and t1 = checkpoint k + 1 or the final state
+            const int m = t1 - t0;
+            std::vector<const float *> sg(size_t(m) + 1);
             sg[0] = B.ck[size_t(k)];
-            for (int j = 0; j < t1 - t0; ++j) {
-                step_into(sg[size_t(j)], B.seg[size_t(j) + 1]);
-                sg[size_t(j) + 1] = B.seg[size_t(j) + 1];
+            for (int j = 1; j < m; ++j) {
+                step_into(sg[size_t(j) - 1], B.seg[size_t(j)]);
+                sg[size_t(j)] = B.seg[size_t(j)];
             }
+            sg[size_t(m)] = k + 1 < nck ? B.ck[size_t(k) + 1] : final;
             for (int t = t1 - 1; t >= t0; --t) {
                 const float *s0 = sg[size_t(t - t0)], *s1 = sg[size_t(t - t0) + 1];
-                adj_v_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, s1, u(), prm.p, tau1, B.GP, B.gs, B.GQ,
-                                                                B.gu, B.acc, B.count);
-                adj_xi_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, tau1, tau2, lim, B.GQ, B.gs, B.GP, Q);
-                adj_xi_replay_kernel<K><<<rgrid, kAdjThreads, 0, st>>>(S, tau2, lim, B.gs, B.GP, Q);
+                uint32_t *const bw = B.buckets;
+                const ReplayBuckets rb{bw + par * kCounters, bw + (1 - par) * kCounters, B.offset};
+                adj_v_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, s1, rec(s1), u(), prm.p, tau1, B.GP,
+                                                                B.gs, B.GQ, B.gu, B.acc, rb);
+                adj_xi_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, rec(s1), tau1, tau2, B.GQ, B.gs, B.GP,
+                                                                 Q, rb.hist_cur, rb.offset);
+                adj_xi_replay_kernel<K><<<rgrid, kAdjThreads, 0, st>>>(S, tau2, lim, B.gs, B.GP, Q, rb.hist_cur);
+                par = 1 - par;
             }
         }
         chk(hipEventRecord(ev[1], st), "hipEventRecord");
         chk(hipGetLastError(), "adjoint launch");
-        final = nullptr;   // the recompute buffers no longer hold the final state
         std::vector<double> acc(nacc);
         (void)hipStreamSynchronize(st);
         if (!rc && nacc) {

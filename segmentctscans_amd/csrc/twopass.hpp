@@ -160,6 +160,13 @@ constexpr int kXiWorkWaves = 4;      // at most this many waves drain the work l
 // longer sets the time of every draining wave.
 constexpr int kXiFirstSweeps = 2;
 
+// Per-voxel record of the Xi step's Dykstra, written by the REC variant of
+// potts_xi_kernel for the adjoint: sweeps[w] = 0 when the input was feasible
+// (the first sweep changes nothing), else the sweeps performed (1..20).
+struct SweepRecord {
+    uint8_t *sweeps;
+};
+
 // Workgroups of the one-chunk-per-wave kernels, in XCD runs of 64 / VPL
 // (about 11 lines of 1426 voxels).
 template <int VPL>
@@ -216,10 +223,11 @@ __global__ __launch_bounds__(256) void potts_xi_zero_kernel(Lines lines, TwoPass
 //      item as soon as its item converges, the result going back to LDS;
 //   C  every lane stores the Xi+ of its voxels, VPL at a time.
 // BW: waves per workgroup (launch 64 BW threads); the workgroup covers 256 BW
-// voxels and its work list holds 128 BW of them.
-template <int K, typename St, typename Lines, int BW = kXiBlockWaves>
+// voxels and its work list holds 128 BW of them.  REC: also store the
+// SweepRecord of every voxel (the solver runs without).
+template <int K, typename St, typename Lines, int BW = kXiBlockWaves, bool REC = false>
 __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<K, St> a, float tau2,
-                                                       DykstraLimits lim)
+                                                       DykstraLimits lim, SweepRecord rec = {})
 {
     constexpr int KM = K - 1, NP = K * (K - 1) / 2;
     constexpr int VPL = vpl_of<St>(), CPW = kXiVoxelsPerLane / VPL;
@@ -232,6 +240,7 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<
     __shared__ float qb[kXiItems][K][3];   // inputs, then results, of the queued voxels
     __shared__ float rb[kXiLong][NP][3];   // Dykstra corrections of the parked items
     __shared__ int lt[kXiLong];            // parked item: index | sweeps << 16
+    __shared__ uint8_t sq[REC ? kXiItems : 1];   // sweeps of the work-list items
     __shared__ int nq, next, nl;
     const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
     if (threadIdx.x == 0) {
@@ -244,6 +253,7 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, 64 / BW);
     float res[CPW][VPL][KM][3];
     int pos[CPW][VPL];
+    int sfull[CPW][VPL];   // REC: sweeps of voxels not on the work list
     uint32_t wofs[CPW];
     int nv[CPW];   // valid voxels of this lane in chunk k
 #pragma unroll
@@ -252,7 +262,10 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<
         nv[k] = 0;
         wofs[k] = 0;
 #pragma unroll
-        for (int j = 0; j < VPL; ++j) pos[k][j] = -1;
+        for (int j = 0; j < VPL; ++j) {
+            pos[k][j] = -1;
+            sfull[k][j] = 0;
+        }
         if (gw >= total) continue;   // uniform over the wave
         const Chunk ch = lines.template chunk<VPL>(gw);
         const uint32_t n0 = ch.n0 + lane * VPL, l0 = lane * VPL;
@@ -309,7 +322,7 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<
                     for (int ax = 0; ax < 3; ++ax) qb[p][i][ax] = q[i][ax];
                 pos[k][j] = p;
             } else if (queued) {
-                dykstra<K>(q, lim);   // work list full: solve in place
+                sfull[k][j] = dykstra<K>(q, lim);   // work list full: solve in place
             }
 #pragma unroll
             for (int i = 0; i < KM; ++i)
@@ -356,6 +369,7 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<
                     for (int i = 0; i < KM; ++i)
 #pragma unroll
                         for (int ax = 0; ax < 3; ++ax) qb[t][i][ax] = q[i][ax];
+                    if constexpr (REC) sq[t] = uint8_t(sw);
                     have = false;
                 } else if (kFirst < 20 && sw == kFirst) {
                     // park the item (q, r, sweep count) for the second stage
@@ -401,6 +415,7 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<
         for (int k = 0; k < KM; ++k)
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) qb[t][k][ax] = q[k][ax];
+        if constexpr (REC) sq[t] = uint8_t(sw);
     }
     }
     __syncthreads();   // results of the work list in LDS
@@ -416,6 +431,11 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<
                 for (int j = 0; j < VPL; ++j) o[j] = pos[k][j] >= 0 ? qb[pos[k][j]][i][ax] : res[k][j][i][ax];
                 P::st(a.xi_out[3 * i + ax] + wofs[k], o);
             }
+        if constexpr (REC) {
+#pragma unroll
+            for (int j = 0; j < VPL; ++j)
+                if (j < nv[k]) rec.sweeps[wofs[k] + j] = uint8_t(pos[k][j] >= 0 ? sq[pos[k][j]] : sfull[k][j]);
+        }
     }
 }
 

@@ -47,12 +47,17 @@ def test_adjoint_matches_fp64_host(ctx, K):
     assert np.all(r["grad_u"][~m] == 0)
 
 
-def test_checkpoint_interval_does_not_change_result(ctx):
+@pytest.mark.parametrize("iters,cks", [(13, (0, 3, 13, 20)), (12, (2, 4, 6, 12))])
+def test_checkpoint_interval_does_not_change_result(ctx, iters, cks):
+    """Every state is stored with checkpoint_every=1 (no recompute); other
+    intervals recompute the states inside each segment and take its last state
+    from the next checkpoint or the final state, including intervals that
+    divide iters and intervals longer than the run."""
     ph = make_phantom(18, 18, 2.0, 4.0, seed=99)
     c, tgt, g, t, _ = _setup(ctx, ph, 3)
-    ref = S.potts_adjoint(ctx, g, t, c, 30.0, TAU, TAU, 13, checkpoint_every=1)
-    for ck in (0, 3, 13, 20):
-        r = S.potts_adjoint(ctx, g, t, c, 30.0, TAU, TAU, 13, checkpoint_every=ck)
+    ref = S.potts_adjoint(ctx, g, t, c, 30.0, TAU, TAU, iters, checkpoint_every=1)
+    for ck in cks:
+        r = S.potts_adjoint(ctx, g, t, c, 30.0, TAU, TAU, iters, checkpoint_every=ck)
         assert r["loss"] == pytest.approx(ref["loss"], rel=1e-12)
         assert np.allclose(r["grad_c"], ref["grad_c"], rtol=1e-9)
         assert r["grad_lambda"] == pytest.approx(ref["grad_lambda"], rel=1e-9)
