@@ -394,7 +394,8 @@ __device__ __forceinline__ void xi_outputs(uint64_t wspan, uint32_t w, float tau
 // longest first (bucket s starts after the items of buckets s+1..20), each
 // bucket stripe by stripe, so that the lanes of a replay wave run the same
 // number of sweeps on voxels of nearby workgroups.  An item is x (K x 3), the
-// output adjoint g (K x 3) and the voxel's work offset, item after item (a
+// output adjoint g (K - 1 x 3; g_{K-1} is 0) and the voxel's work offset:
+// 64 bytes for K = 3.  Items are stored one after the other (a
 // workgroup's items of one bucket are contiguous).
 struct XiQueue {
     float *items;
@@ -456,7 +457,7 @@ __device__ void bucket_starts(const uint32_t (&hv)[kPerWave], uint32_t z, uint32
 }
 
 template <int K>
-constexpr int item_floats() { return 6 * K + 1; }
+constexpr int item_floats() { return 6 * K - 2; }   // x: 3K, g: 3(K - 1) (g_{K-1} = 0), offset
 
 // B1: div's adjoint (-grad of tau1 GQ).  Voxels whose Dykstra input was
 // feasible (SweepRecord 0: the projection was the identity) pass the adjoint
@@ -521,9 +522,9 @@ __global__ __launch_bounds__(kAdjThreads) void adj_xi_kernel(VolumeLines L, uint
             for (int i = 0; i < K; ++i)
                 for (int a = 0; a < 3; ++a) {
                     it[3 * i + a] = x[i][a];
-                    it[3 * K + 3 * i + a] = g[i][a];
+                    if (i < KM) it[3 * K + 3 * i + a] = g[i][a];
                 }
-            it[6 * K] = __uint_as_float(vx.w);
+            it[6 * K - 3] = __uint_as_float(vx.w);
         }
     }
 }
@@ -548,7 +549,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_xi_replay_kernel(uint64_t wsp
         for (int i = 0; i < K; ++i)
             for (int a = 0; a < 3; ++a) {
                 x[i][a] = it[3 * i + a];
-                g[i][a] = it[3 * K + 3 * i + a];
+                g[i][a] = i < KM ? it[3 * K + 3 * i + a] : 0.0f;
             }
         if constexpr (NP <= 3) {
             // whole history in registers: both sweep loops fully unrolled
@@ -571,7 +572,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_xi_replay_kernel(uint64_t wsp
             const int sw = dykstra_replay<K>(x, hist, lim);
             dykstra_back<K>(sw, hist, g, lim);
         }
-        xi_outputs<K>(wspan, __float_as_uint(it[6 * K]), tau2, g, gXi, GP);
+        xi_outputs<K>(wspan, __float_as_uint(it[6 * K - 3]), tau2, g, gXi, GP);
     }
 }
 
