@@ -40,6 +40,44 @@ void sct_free(sct_ctx *, void *p)
     if (p) (void)hipFree(p);
 }
 
+// Sizes round up to 1 MiB (device) and 64 KiB (host), so that calls of
+// slightly different sizes do not reallocate.
+int sct_ctx_scratch(sct_ctx *ctx, size_t bytes, void **p)
+{
+    *p = nullptr;
+    if (ctx->scratch_bytes < bytes) {
+        sct_free(ctx, ctx->scratch);   // hipFree waits for the work that uses it
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+        const size_t want = (bytes + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);
+        const int rc = sct_alloc(ctx, &ctx->scratch, want);
+        if (rc) return rc;
+        ctx->scratch_bytes = want;
+    }
+    *p = ctx->scratch;
+    return SCT_OK;
+}
+
+int sct_ctx_host_scratch(sct_ctx *ctx, size_t bytes, void **p)
+{
+    *p = nullptr;
+    if (ctx->host_scratch_bytes < bytes) {
+        if (ctx->host_scratch) (void)hipHostFree(ctx->host_scratch);
+        ctx->host_scratch = nullptr;
+        ctx->host_scratch_bytes = 0;
+        const size_t want = (bytes + (size_t(1) << 16) - 1) & ~((size_t(1) << 16) - 1);
+        const hipError_t e = hipHostMalloc(&ctx->host_scratch, want, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            ctx->host_scratch = nullptr;
+            (void)hipGetLastError();
+            return sct_hip_fail(ctx, e, "hipHostMalloc");
+        }
+        ctx->host_scratch_bytes = want;
+    }
+    *p = ctx->host_scratch;
+    return SCT_OK;
+}
+
 // ---- layout helpers ---------------------------------------------------------------
 // layout_cylinder::mapping_storage(extents, xc, yc, r, nstride), layout_cylinder.h:152-195:
 // a column is live when r^2 - (col - xc)^2 > 0; its rows are the integers strictly
@@ -276,6 +314,8 @@ extern "C" void sct_ctx_destroy(sct_ctx *ctx)
 {
     if (!ctx) return;
     (void)hipStreamSynchronize(ctx->stream);
+    sct_free(ctx, ctx->scratch);
+    if (ctx->host_scratch) (void)hipHostFree(ctx->host_scratch);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstring>
 
 namespace {
 
@@ -29,12 +30,100 @@ VoxGeom vox_geom(const sct_field *f)
     return g;
 }
 
+// True when the voxels of the lines cover the whole span (no padding after
+// n_images, no gaps between columns), so that a kernel writing every voxel
+// of every line leaves nothing to clear.
+bool lines_cover_span(const Geometry &geo)
+{
+    return geo.nstride == geo.n_images && geo.span() == uint64_t(geo.lines) * geo.nstride;
+}
+
+// Grid of a wave_lines kernel whose workgroups end in device atomics on a
+// few words: four workgroups per CU (a wave per line, two 16-byte loads in
+// flight per lane keep enough bytes moving), lines permitting.
+uint32_t line_grid(const sct_ctx *ctx, const VoxGeom &g)
+{
+    const uint32_t cap = std::max<uint32_t>(ctx->cu_count, 1) * 4;
+    return std::max<uint32_t>(1, std::min<uint32_t>((g.lines + 3) / 4, cap));
+}
+
 uint32_t stream_grid(const sct_ctx *ctx, const VoxGeom &g)
 {
     const uint64_t waves = uint64_t(g.lines) * g.chunks;
     const uint64_t cap = uint64_t(std::max<uint32_t>(ctx->cu_count, 1)) * 16;
     return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, cap)));
 }
+
+// Calls f(x, line, n) for every voxel of the lines first, first + step, ...
+// below end, a line per wave at a time, x[k] the voxel's value in array k.
+// Each lane takes 4 consecutive voxels per trip, as one 16-byte load per array
+// when the line stride is a multiple of 4 (field data is 256-byte aligned),
+// with two trips' loads in flight; other strides load one voxel per lane.
+// Called by all lanes of the wave.
+template <int NA, typename F>
+__device__ __forceinline__ void wave_lines_n(const VoxGeom &g, const float *const (&arr)[NA], uint32_t first,
+                                             uint32_t end, uint32_t step, F &&f)
+{
+    const uint32_t lane = threadIdx.x % kWave;
+    if (g.stride % 4 == 0) {
+        for (uint32_t line = first; line < end; line += step) {
+            const uint64_t lo = uint64_t(line) * g.stride;
+            for (uint32_t n0 = lane * 4; n0 < g.len; n0 += 2 * kWave * 4) {
+                float v[2][NA][4];
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const uint32_t n = n0 + uint32_t(k) * kWave * 4;
+#pragma unroll
+                    for (int c = 0; c < NA; ++c) {
+                        const float *row = arr[c] + lo;
+                        if (n + 4 <= g.len) {
+                            const float4 q = *reinterpret_cast<const float4 *>(row + n);
+                            v[k][c][0] = q.x;
+                            v[k][c][1] = q.y;
+                            v[k][c][2] = q.z;
+                            v[k][c][3] = q.w;
+                        } else {
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) v[k][c][j] = n + j < g.len ? row[n + j] : 0.0f;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const uint32_t n = n0 + uint32_t(k) * kWave * 4 + uint32_t(j);
+                        float x[NA];
+#pragma unroll
+                        for (int c = 0; c < NA; ++c) x[c] = v[k][c][j];
+                        if (n < g.len) f(x, line, n);
+                    }
+            }
+        }
+    } else {
+        for (uint32_t line = first; line < end; line += step) {
+            const uint64_t lo = uint64_t(line) * g.stride;
+            for (uint32_t n = lane; n < g.len; n += kWave) {
+                float x[NA];
+#pragma unroll
+                for (int c = 0; c < NA; ++c) x[c] = arr[c][lo + n];
+                f(x, line, n);
+            }
+        }
+    }
+}
+
+template <typename F>
+__device__ __forceinline__ void wave_lines(const VoxGeom &g, const float *u, uint32_t first, uint32_t end,
+                                           uint32_t step, F &&f)
+{
+    const float *const arr[1] = {u};
+    wave_lines_n<1>(g, arr, first, end, step, [&](const float (&x)[1], uint32_t line, uint32_t n) { f(x[0], line, n); });
+}
+
+// The min and max keys sit on separate 128-byte lines: device atomics on one
+// line are serialised.
+constexpr uint32_t kKeyMax = 32;
 
 // Block-wide reduction of (mn, mx) followed by one atomic per block on
 // order-preserving keys (min_max.cpp:45-53 uses one device atomic per block).
@@ -55,7 +144,7 @@ __device__ void block_minmax_atomic(float mn, float mx, uint32_t *keys)
             mx = fmaxf(mx, smx[i]);
         }
         atomicMin(&keys[0], float_key(mn));
-        atomicMax(&keys[1], float_key(mx));
+        atomicMax(&keys[kKeyMax], float_key(mx));
     }
 }
 
@@ -65,20 +154,14 @@ template <bool POSITIVE>
 __global__ __launch_bounds__(256) void minmax_kernel(VoxGeom g, const float *__restrict__ u, uint32_t *keys)
 {
     float mn = FLT_MAX, mx = -FLT_MAX;
-    const uint64_t total = uint64_t(g.lines) * g.chunks;
-    const uint64_t nwaves = uint64_t(gridDim.x) * (blockDim.x / kWave);
-    const uint32_t lane = threadIdx.x % kWave;
-    for (uint64_t w = uint64_t(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; w < total; w += nwaves) {
-        const uint32_t line = uint32_t(w / g.chunks);
-        const uint32_t n = uint32_t(w % g.chunks) * kWave + lane;
-        if (n < g.len) {
-            const float x = u[uint64_t(line) * g.stride + n];
-            if (!POSITIVE || 0.0f < x) {
-                mn = (x < mn) ? x : mn;
-                mx = (mx < x) ? x : mx;
-            }
-        }
-    }
+    const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
+    wave_lines(g, u, blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave, g.lines, nwaves,
+               [&](float x, uint32_t, uint32_t) {
+                   if (!POSITIVE || 0.0f < x) {
+                       mn = (x < mn) ? x : mn;
+                       mx = (mx < x) ? x : mx;
+                   }
+               });
     block_minmax_atomic(mn, mx, keys);
 }
 
@@ -132,14 +215,9 @@ __global__ __launch_bounds__(256) void hist_kernel(VoxGeom g, const float *__res
     }
     const uint32_t l0 = blockIdx.x * a.lines_per_block;
     const uint32_t l1 = min(g.lines, l0 + a.lines_per_block);
-    const uint32_t lane = threadIdx.x % kWave, nw = blockDim.x / kWave;
-    const uint32_t pairs = l0 < l1 ? (l1 - l0) * g.chunks : 0;
+    const uint32_t nw = blockDim.x / kWave;
     const float half = a.delta / 2;
-    for (uint32_t p = threadIdx.x / kWave; p < pairs; p += nw) {
-        const uint32_t line = l0 + p / g.chunks;
-        const uint32_t n = (p % g.chunks) * kWave + lane;
-        if (n >= g.len) continue;
-        float x = u[uint64_t(line) * g.stride + n];
+    wave_lines(g, u, l0 + threadIdx.x / kWave, l1, nw, [&](float x, uint32_t line, uint32_t n) {
         if constexpr (DITHER) {
             const uint32_t r = philox_first(line, n, a.k0, a.k1);
             x = x + (-half + a.delta * (float(r >> 8) * (1.0f / 16777216.0f)));
@@ -149,7 +227,7 @@ __global__ __launch_bounds__(256) void hist_kernel(VoxGeom g, const float *__res
             if constexpr (MODE == 0) atomicAdd(&sh[b], 1u);
             else atomicAdd(&counts[b], 1u);
         }
-    }
+    });
     if constexpr (MODE != 2) {
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) {
@@ -182,36 +260,17 @@ __global__ __launch_bounds__(kPartThreads) void hist_part_kernel(VoxGeom g, cons
     __syncthreads();
     const uint32_t l0 = slice * a.lines_per_block;
     const uint32_t l1 = min(g.lines, l0 + a.lines_per_block);
-    const uint32_t lane = threadIdx.x % kWave, nw = blockDim.x / kWave;
-    const uint32_t pairs = l0 < l1 ? (l1 - l0) * g.chunks : 0;
     const float half = a.delta / 2;
-    constexpr int U = 4;   // independent loads in flight per lane
-    for (uint32_t q0 = threadIdx.x / kWave; q0 < pairs; q0 += nw * U) {
-        float x[U];
-        uint32_t line[U], n[U];
-        bool ok[U];
-#pragma unroll
-        for (int k = 0; k < U; ++k) {
-            const uint32_t q = q0 + uint32_t(k) * nw;
-            line[k] = l0 + q / g.chunks;
-            n[k] = (q % g.chunks) * kWave + lane;
-            ok[k] = q < pairs && n[k] < g.len;
-            x[k] = ok[k] ? u[uint64_t(line[k]) * g.stride + n[k]] : 0.0f;
+    wave_lines(g, u, l0 + threadIdx.x / kWave, l1, blockDim.x / kWave, [&](float v, uint32_t line, uint32_t n) {
+        if constexpr (DITHER) {
+            const uint32_t r = philox_first(line, n, a.k0, a.k1);
+            v = v + (-half + a.delta * (float(r >> 8) * (1.0f / 16777216.0f)));
         }
-#pragma unroll
-        for (int k = 0; k < U; ++k) {
-            if (!ok[k]) continue;
-            float v = x[k];
-            if constexpr (DITHER) {
-                const uint32_t r = philox_first(line[k], n[k], a.k0, a.k1);
-                v = v + (-half + a.delta * (float(r >> 8) * (1.0f / 16777216.0f)));
-            }
-            if (a.rs - half <= v && v + half <= a.re) {
-                const uint32_t b = uint32_t(hist_bin(v, a.rs, a.re, a.nbins)) - b0;
-                if (b < nb) atomicAdd(&sh[b], 1u);
-            }
+        if (a.rs - half <= v && v + half <= a.re) {
+            const uint32_t b = uint32_t(hist_bin(v, a.rs, a.re, a.nbins)) - b0;
+            if (b < nb) atomicAdd(&sh[b], 1u);
         }
-    }
+    });
     __syncthreads();
     uint32_t *row = partial + uint64_t(slice) * a.nbins + b0;
     for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) row[i] = sh[i];
@@ -318,31 +377,16 @@ __global__ __launch_bounds__(kPartThreads) void uv_part_kernel(VoxGeom g, const 
     __syncthreads();
     const uint32_t l0 = slice * lines_per_slice;
     const uint32_t l1 = min(g.lines, l0 + lines_per_slice);
-    const uint32_t lane = threadIdx.x % kWave, nw = blockDim.x / kWave;
-    const uint32_t pairs = l0 < l1 ? (l1 - l0) * g.chunks : 0;
-    constexpr int U = 4;
-    for (uint32_t q0 = threadIdx.x / kWave; q0 < pairs; q0 += nw * U) {
-        float x[U], y[U];
-        bool ok[U];
-#pragma unroll
-        for (int k = 0; k < U; ++k) {
-            const uint32_t q = q0 + uint32_t(k) * nw;
-            const uint32_t line = l0 + q / g.chunks;
-            const uint32_t n = (q % g.chunks) * kWave + lane;
-            ok[k] = q < pairs && n < g.len;
-            const uint64_t o = uint64_t(line) * g.stride + n;
-            x[k] = ok[k] ? u[o] : 0.0f;
-            y[k] = ok[k] ? v[o] : 0.0f;
-        }
-#pragma unroll
-        for (int k = 0; k < U; ++k) {
-            if (!ok[k] || !(a.u0 <= x[k] && x[k] <= a.u1 && a.v0 <= y[k] && y[k] <= a.v1)) continue;
-            const uint32_t xb = uint32_t(hist_bin(x[k], a.u0, a.u1, a.nbins));
-            const uint32_t yb = uint32_t(hist_bin(y[k], a.v0, a.v1, a.nbins));
-            const uint32_t b = yb * a.hstride + xb - b0;
-            if (b < nb) atomicAdd(&sh[b], 1u);
-        }
-    }
+    const float *const arr[2] = {u, v};
+    wave_lines_n<2>(g, arr, l0 + threadIdx.x / kWave, l1, blockDim.x / kWave,
+                    [&](const float (&xy)[2], uint32_t, uint32_t) {
+                        const float x = xy[0], y = xy[1];
+                        if (!(a.u0 <= x && x <= a.u1 && a.v0 <= y && y <= a.v1)) return;
+                        const uint32_t xb = uint32_t(hist_bin(x, a.u0, a.u1, a.nbins));
+                        const uint32_t yb = uint32_t(hist_bin(y, a.v0, a.v1, a.nbins));
+                        const uint32_t b = yb * a.hstride + xb - b0;
+                        if (b < nb) atomicAdd(&sh[b], 1u);
+                    });
     __syncthreads();
     uint32_t *row = partial + uint64_t(slice) * total + b0;
     for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) row[i] = sh[i];
@@ -353,33 +397,34 @@ int run_minmax(sct_ctx *ctx, const sct_field *f, bool positive, float *mn, float
     if (!ctx || !f || !mn || !mx) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
     if (f->dtype != SCT_F32) return sct_fail(ctx, SCT_ERR_INVALID, "float field required");
     const VoxGeom g = vox_geom(f);
-    uint32_t *keys = nullptr;
-    int rc = sct_alloc(ctx, (void **)&keys, 2 * sizeof(uint32_t));
+    uint32_t *keys = nullptr, *h = nullptr;
+    constexpr size_t kKeyBytes = sizeof(uint32_t) * (kKeyMax + 1);
+    int rc = sct_ctx_scratch(ctx, kKeyBytes, (void **)&keys);
+    if (!rc) rc = sct_ctx_host_scratch(ctx, kKeyBytes, (void **)&h);
     if (rc) return rc;
-    uint32_t h[2] = {0, 0};
     {
         // keys start at (FLT_MAX, lowest), the reference's initial values (min_max.cpp:61-65)
         const float init[2] = {FLT_MAX, -FLT_MAX};
         for (int i = 0; i < 2; ++i) {
             uint32_t b;
             std::memcpy(&b, &init[i], 4);
-            h[i] = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+            h[i * kKeyMax] = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
         }
     }
-    hipError_t e = hipMemcpyAsync(keys, h, sizeof(h), hipMemcpyHostToDevice, ctx->stream);
+    hipError_t e = hipMemcpyAsync(keys, h, kKeyBytes, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess && g.lines && g.len) {
         if (positive)
-            minmax_kernel<true><<<stream_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const float *)f->data, keys);
+            minmax_kernel<true><<<line_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const float *)f->data, keys);
         else
-            minmax_kernel<false><<<stream_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const float *)f->data, keys);
+            minmax_kernel<false><<<line_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const float *)f->data, keys);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(h, keys, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h, keys, kKeyBytes, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    sct_free(ctx, keys);
+    if (e != hipSuccess) (void)hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return sct_hip_fail(ctx, e, "min_max");
     *mn = key_float(h[0]);
-    *mx = key_float(h[1]);
+    *mx = key_float(h[kKeyMax]);
     return SCT_OK;
 }
 
@@ -403,10 +448,6 @@ extern "C" int sct_calc_hist(sct_ctx *ctx, const sct_field *f, float rs, float r
     if (f->dtype != SCT_F32) return sct_fail(ctx, SCT_ERR_INVALID, "float field required");
     if (nbins == 0) return sct_fail(ctx, SCT_ERR_INVALID, "nbins must be positive");
     const VoxGeom g = vox_geom(f);
-    uint32_t *d = nullptr;
-    int rc = sct_alloc(ctx, (void **)&d, sizeof(uint32_t) * nbins);
-    if (rc) return rc;
-    hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t) * nbins, ctx->stream);
     HistArgs a;
     a.rs = rs;
     a.re = re;
@@ -417,17 +458,23 @@ extern "C" int sct_calc_hist(sct_ctx *ctx, const sct_field *f, float rs, float r
     const bool dither = delta != 0.0f;
     const uint32_t parts = (nbins + kPartBins - 1) / kPartBins;
     int mode = nbins <= kSmallBins ? 0 : (parts <= kMaxParts ? 3 : 2);
-    uint32_t *partial = nullptr;
+    // counts d, then (partitioned mode) a row of partial counts per slice of
+    // lines, in the context's scratch
+    const size_t dbytes = (sizeof(uint32_t) * size_t(nbins) + 255) & ~size_t(255);
+    uint32_t slices = 0, used = 0;
     if (mode == 3 && g.lines && g.len) {
-        // slices: about two workgroups per CU over all partitions
-        const uint32_t slices = std::max<uint32_t>(1, std::min<uint32_t>(g.lines, 2 * ctx->cu_count / parts));
+        // slices: one workgroup per CU over all partitions (the LDS holds one)
+        slices = std::max<uint32_t>(1, std::min<uint32_t>(g.lines, ctx->cu_count / parts));
         a.lines_per_block = (g.lines + slices - 1) / slices;
-        const uint32_t used = (g.lines + a.lines_per_block - 1) / a.lines_per_block;
-        rc = sct_alloc(ctx, (void **)&partial, sizeof(uint32_t) * uint64_t(used) * nbins);
-        if (rc) {
-            sct_free(ctx, d);
-            return rc;
-        }
+        used = (g.lines + a.lines_per_block - 1) / a.lines_per_block;
+    }
+    void *mem = nullptr, *host = nullptr;
+    int rc = sct_ctx_scratch(ctx, dbytes + sizeof(uint32_t) * uint64_t(used) * nbins, &mem);
+    if (!rc) rc = sct_ctx_host_scratch(ctx, sizeof(uint32_t) * size_t(nbins), &host);
+    if (rc) return rc;
+    uint32_t *d = (uint32_t *)mem, *partial = used ? (uint32_t *)((char *)mem + dbytes) : nullptr;
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t) * nbins, ctx->stream);
+    if (mode == 3 && g.lines && g.len) {
         const size_t lds = sizeof(uint32_t) * std::min(nbins, kPartBins);
         auto launch = [&](auto kernel) -> hipError_t {
             hipError_t ea = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
@@ -456,12 +503,11 @@ extern "C" int sct_calc_hist(sct_ctx *ctx, const sct_field *f, float rs, float r
             e = hipGetLastError();
         }
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(counts, d, sizeof(uint32_t) * nbins, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(host, d, sizeof(uint32_t) * nbins, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) (void)hipStreamSynchronize(ctx->stream);
-    sct_free(ctx, partial);
-    sct_free(ctx, d);
     if (e != hipSuccess) return sct_hip_fail(ctx, e, "calc_hist");
+    std::memcpy(counts, host, sizeof(uint32_t) * nbins);
     if (density) {
         // calc_hist.cpp:131-137
         const double bin_width = (double(re) - double(rs)) / nbins;
@@ -493,7 +539,9 @@ extern "C" int sct_grad_magn(sct_ctx *ctx, const sct_field *u, sct_field **v)
     int rc = sct_field_like(ctx, u, SCT_F32, &out);
     if (rc) return rc;
     const VoxGeom g = vox_geom(u);
-    hipError_t e = hipMemsetAsync(out->data, 0, sizeof(float) * out->geo.span(), ctx->stream);
+    // the kernel writes every voxel of every line
+    hipError_t e = hipSuccess;
+    if (!lines_cover_span(out->geo)) e = hipMemsetAsync(out->data, 0, sizeof(float) * out->geo.span(), ctx->stream);
     if (e == hipSuccess && g.lines && g.len) {
         const uint32_t grid = uint32_t((uint64_t(g.lines) * g.chunks + 3) / 4);
         grad_magn_kernel<<<grid, 256, 0, ctx->stream>>>(g, (const float *)u->data, (float *)out->data);
@@ -525,23 +573,26 @@ extern "C" int sct_uv_hist(sct_ctx *ctx, const sct_field *u, const sct_field *v,
     a.hstride = ((nbins + 31u) / 32u) * 32u;   // 128-byte rows, uv-hist-kernel.cpp:115-121
     const size_t total = size_t(nbins - 1) * a.hstride + nbins;
     if (stride) *stride = a.hstride;
-    uint32_t *d = nullptr;
-    int rc = sct_alloc(ctx, (void **)&d, sizeof(uint32_t) * total);
-    if (rc) return rc;
     const VoxGeom g = vox_geom(u);
-    hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t) * total, ctx->stream);
     const uint32_t parts = uint32_t((total + kPartBins - 1) / kPartBins);
-    uint32_t *partial = nullptr;
-    if (e == hipSuccess && g.lines && g.len && parts <= kMaxParts) {
+    // counts d, then (partitioned mode) a row of partial counts per slice of
+    // lines, in the context's scratch
+    const size_t dbytes = (sizeof(uint32_t) * total + 255) & ~size_t(255);
+    const bool part_mode = g.lines && g.len && parts <= kMaxParts;
+    uint32_t lps = 0, used = 0;
+    if (part_mode) {
+        const uint32_t slices = std::max<uint32_t>(1, std::min<uint32_t>(g.lines, ctx->cu_count / parts));
+        lps = (g.lines + slices - 1) / slices;
+        used = (g.lines + lps - 1) / lps;
+    }
+    void *mem = nullptr, *host = nullptr;
+    int rc = sct_ctx_scratch(ctx, dbytes + sizeof(uint32_t) * uint64_t(used) * total, &mem);
+    if (!rc) rc = sct_ctx_host_scratch(ctx, sizeof(uint32_t) * total, &host);
+    if (rc) return rc;
+    uint32_t *d = (uint32_t *)mem, *partial = used ? (uint32_t *)((char *)mem + dbytes) : nullptr;
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t) * total, ctx->stream);
+    if (e == hipSuccess && part_mode) {
         // counters in LDS, by partitions of the bins (see uv_part_kernel)
-        const uint32_t slices = std::max<uint32_t>(1, std::min<uint32_t>(g.lines, 2 * ctx->cu_count / parts));
-        const uint32_t lps = (g.lines + slices - 1) / slices;
-        const uint32_t used = (g.lines + lps - 1) / lps;
-        rc = sct_alloc(ctx, (void **)&partial, sizeof(uint32_t) * uint64_t(used) * total);
-        if (rc) {
-            sct_free(ctx, d);
-            return rc;
-        }
         const size_t lds = sizeof(uint32_t) * std::min<size_t>(total, kPartBins);
         e = hipFuncSetAttribute((const void *)uv_part_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
         if (e == hipSuccess) {
@@ -557,12 +608,11 @@ extern "C" int sct_uv_hist(sct_ctx *ctx, const sct_field *u, const sct_field *v,
         uv_hist_kernel<<<stream_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const float *)u->data, (const float *)v->data, a, d);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(counts, d, sizeof(uint32_t) * total, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(host, d, sizeof(uint32_t) * total, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) (void)hipStreamSynchronize(ctx->stream);
-    sct_free(ctx, partial);
-    sct_free(ctx, d);
     if (e != hipSuccess) return sct_hip_fail(ctx, e, "uv_hist");
+    std::memcpy(counts, host, sizeof(uint32_t) * total);
     return SCT_OK;
 }
 
@@ -653,7 +703,9 @@ extern "C" int sct_convert_to_fp(sct_ctx *ctx, const sct_field *in, float lo, fl
     int rc = sct_field_like(ctx, in, SCT_F32, &f);
     if (rc) return rc;
     const VoxGeom g = vox_geom(in);
-    hipError_t e = hipMemsetAsync(f->data, 0, sizeof(float) * geo.span(), ctx->stream);
+    // the kernel writes every voxel of every line
+    hipError_t e = hipSuccess;
+    if (!lines_cover_span(geo)) e = hipMemsetAsync(f->data, 0, sizeof(float) * geo.span(), ctx->stream);
     if (e == hipSuccess && g.lines && g.len) {
         if (in->dtype == SCT_U16)
             convert_kernel<uint16_t><<<stream_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const uint16_t *)in->data,
@@ -680,8 +732,9 @@ extern "C" int sct_find_intensity_range(sct_ctx *ctx, const sct_field *f, double
     if (!ctx || !f || !lo || !hi) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
     if (f->dtype != SCT_U16) return sct_fail(ctx, SCT_ERR_INVALID, "u16 field required");
     constexpr uint32_t nbins = 65536;
-    uint32_t *d = nullptr;
-    int rc = sct_alloc(ctx, (void **)&d, sizeof(uint32_t) * nbins);
+    uint32_t *d = nullptr, *h = nullptr;
+    int rc = sct_ctx_scratch(ctx, sizeof(uint32_t) * nbins, (void **)&d);
+    if (!rc) rc = sct_ctx_host_scratch(ctx, sizeof(uint32_t) * nbins, (void **)&h);
     if (rc) return rc;
     const VoxGeom g = vox_geom(f);
     hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t) * nbins, ctx->stream);
@@ -707,13 +760,12 @@ extern "C" int sct_find_intensity_range(sct_ctx *ctx, const sct_field *f, double
             if (e == hipSuccess) e = hipGetLastError();
         }
     }
-    std::vector<uint32_t> h(nbins);
-    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, sizeof(uint32_t) * nbins, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(uint32_t) * nbins, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    sct_free(ctx, d);
+    if (e != hipSuccess) (void)hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return sct_hip_fail(ctx, e, "find_intensity_range");
     uint64_t total = 0;
-    for (uint32_t c : h) total += c;
+    for (uint32_t i = 0; i < nbins; ++i) total += h[i];
     const double pct_lo = tail_thr * double(total), pct_hi = (1.0 - tail_thr) * double(total);
     size_t l = 0, u = nbins;
     uint64_t cum = 0;

@@ -74,6 +74,10 @@ struct sct_ctx {
     std::map<uint64_t, int> tiles;   // batch autotuner: chosen Xi workgroup tiles
     int last_tiles[5] = {};          // tiles of the last sct_potts_batch call, K = 2..6
     int forced_tiles[5] = {};        // tiles set by sct_potts_batch_set_tiles (0: not set)
+    void *scratch = nullptr;         // device scratch of the reductions (sct_ctx_scratch)
+    size_t scratch_bytes = 0;
+    void *host_scratch = nullptr;    // pinned host staging of their results
+    size_t host_scratch_bytes = 0;
 };
 
 struct sct_field {
@@ -93,9 +97,15 @@ int sct_hip_fail(sct_ctx *ctx, hipError_t e, const char *what);
         if (e_ != hipSuccess) return sct_hip_fail((ctx), e_, #expr); \
     } while (0)
 
-// ---- memory (stream-ordered, from the device's default pool) --------------
+// ---- memory (hipMalloc / hipFree) -------------------------------------------
 int sct_alloc(sct_ctx *ctx, void **p, size_t bytes);
 void sct_free(sct_ctx *ctx, void *p);
+// Device / pinned host scratch of at least `bytes`, owned by the context and
+// grown on demand.  For temporaries within one call: the next call on the
+// context's stream reuses the same memory (stream order keeps the uses apart;
+// the host side must have synchronised before it reuses host scratch).
+int sct_ctx_scratch(sct_ctx *ctx, size_t bytes, void **p);
+int sct_ctx_host_scratch(sct_ctx *ctx, size_t bytes, void **p);
 
 // ---- geometry construction ------------------------------------------------
 int sct_geometry_build(sct_ctx *ctx, const sct_layout *l, Geometry &g);

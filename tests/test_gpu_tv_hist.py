@@ -173,3 +173,30 @@ def test_dykstra_cr_sqrt_and_division_match_ieee(ctx):
     bs, bd = C.c_uint64(), C.c_uint64()
     ctx.check(N.lib().sct_selftest_cr_math(ctx.handle, 1 << 26, 12345, C.byref(bs), C.byref(bd)))
     assert bs.value == 0 and bd.value == 0, (bs.value, bd.value)
+
+
+@pytest.mark.parametrize("n_images,nstride", [(40, None), (40, 45), (64, 64)])
+def test_output_fields_padding(ctx, n_images, nstride):
+    """grad_magn and convert_to_fp write every voxel of every line; where the
+    lines do not cover the span (nstride > n_images) the rest of the new field
+    is zero.  The second round runs after the first round's fields were
+    freed, so memory handed out again would show any stale values."""
+    mp = S.Mapping.cylinder(n_images, 17, 15, 7.1, 8.2, 6.9, nstride)
+    rng = np.random.default_rng(n_images + (nstride or 0))
+    data = np.zeros(mp.span, np.float32)
+    valid = mp.valid_mask()
+    data[valid] = rng.random(int(valid.sum()), dtype=np.float32)
+    lay = O.Layout.from_mapping(mp)
+    u = S.upload(ctx, S.Field(mp, data))
+    for _ in range(2):
+        fv = S.grad_magn(ctx, u)
+        v = fv.to_host()
+        fv.close()
+        assert np.array_equal(_bits(v[valid]), _bits(O.grad_magn(lay, data)[valid]))
+        assert np.all(v[~valid] == 0)
+        fc = S.convert_to_fp(ctx, u, 0.25, 0.75)
+        c = fc.to_host()
+        fc.close()
+        assert np.array_equal(_bits(c[valid]), _bits(O.convert_to_fp(lay, data, 0.25, 0.75)[valid]))
+        assert np.all(c[~valid] == 0)
+    u.close()
