@@ -286,23 +286,81 @@ __global__ __launch_bounds__(256) void hist_sum_kernel(const uint32_t *__restric
     counts[b] = s;
 }
 
-// grad_kernel of grad_magn.cpp:17-58.
+// grad_kernel of grad_magn.cpp:17-58 for one voxel: x its value's
+// neighbours along n (xm, xp), along rows (rm, rp) and columns (cm, cp).
+__device__ __forceinline__ float grad_value(bool inner, float xm, float xp, float rm, float rp, float cm, float cp)
+{
+    if (!inner) return -1.0f;
+    const float q1 = xp - xm, q2 = rp - rm, q3 = cp - cm;
+    return 0.5f * sqrtf(__fmaf_rn(q3, q3, __fmaf_rn(q2, q2, q1 * q1)));
+}
+
+// A line per wave; each lane takes 4 consecutive voxels per trip, as 16-byte
+// loads of the line and its four neighbour lines (stride a multiple of 4);
+// the values at n - 1 and n + 4 come from the neighbouring lanes.  Voxels
+// without all six neighbours get -1.
 __global__ __launch_bounds__(256) void grad_magn_kernel(VoxGeom g, const float *__restrict__ u, float *__restrict__ v)
 {
-    const uint64_t w = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
-    const uint32_t line = uint32_t(w / g.chunks);
-    const uint32_t n = uint32_t(w % g.chunks) * kWave + threadIdx.x % kWave;
-    if (line >= g.lines || n >= g.len) return;
-    const LineNbr nb = g.nbr[line];
-    const uint64_t o = uint64_t(line) * g.stride + n;
-    if (!(n + 1 < g.len) || nb.nrow < 0 || nb.ncol < 0 || !(n > 0) || nb.prow < 0 || nb.pcol < 0) {
-        v[o] = -1.0f;
-        return;
+    const uint32_t lane = threadIdx.x % kWave;
+    const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
+    for (uint32_t line = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; line < g.lines; line += nwaves) {
+        const LineNbr nb = g.nbr[line];
+        const bool inner_line = nb.nrow >= 0 && nb.ncol >= 0 && nb.prow >= 0 && nb.pcol >= 0;
+        const uint64_t o = uint64_t(line) * g.stride;
+        const float *r = u + o;
+        if (g.stride % 4 != 0) {
+            for (uint32_t n = lane; n < g.len; n += kWave) {
+                const bool inner = inner_line && n > 0 && n + 1 < g.len;
+                v[o + n] = inner ? grad_value(true, r[n - 1], r[n + 1], u[uint64_t(nb.prow) * g.stride + n],
+                                              u[uint64_t(nb.nrow) * g.stride + n], u[uint64_t(nb.pcol) * g.stride + n],
+                                              u[uint64_t(nb.ncol) * g.stride + n])
+                                 : -1.0f;
+            }
+            continue;
+        }
+        const float *nl[4] = {u + uint64_t(max(nb.prow, 0)) * g.stride, u + uint64_t(max(nb.nrow, 0)) * g.stride,
+                              u + uint64_t(max(nb.pcol, 0)) * g.stride, u + uint64_t(max(nb.ncol, 0)) * g.stride};
+        // trips are uniform over the wave (the shuffles need every lane)
+        for (uint32_t base = 0; base < g.len; base += kWave * 4) {
+            const uint32_t n0 = base + lane * 4;
+            const bool full = n0 + 4 <= g.len;
+            auto load4 = [&](const float *row, float (&x)[4]) {
+                if (full) {
+                    const float4 q = *reinterpret_cast<const float4 *>(row + n0);
+                    x[0] = q.x;
+                    x[1] = q.y;
+                    x[2] = q.z;
+                    x[3] = q.w;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) x[j] = n0 + j < g.len ? row[n0 + j] : 0.0f;
+                }
+            };
+            float c[4], nbv[4][4] = {};
+            load4(r, c);
+            if (inner_line)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) load4(nl[k], nbv[k]);
+            float left = __shfl_up(c[3], 1, kWave), right = __shfl_down(c[0], 1, kWave);
+            if (lane == 0) left = n0 > 0 && n0 - 1 < g.len ? r[n0 - 1] : 0.0f;
+            if (lane == kWave - 1) right = n0 + 4 < g.len ? r[n0 + 4] : 0.0f;
+            float out[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t n = n0 + uint32_t(j);
+                const bool inner = inner_line && n > 0 && n + 1 < g.len;
+                const float xm = j == 0 ? left : c[j - 1], xp = j == 3 ? right : c[j + 1];
+                out[j] = grad_value(inner, xm, xp, nbv[0][j], nbv[1][j], nbv[2][j], nbv[3][j]);
+            }
+            if (full) {
+                *reinterpret_cast<float4 *>(v + o + n0) = make_float4(out[0], out[1], out[2], out[3]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (n0 + j < g.len) v[o + n0 + j] = out[j];
+            }
+        }
     }
-    const float q1 = u[o + 1] - u[o - 1];
-    const float q2 = u[uint64_t(nb.nrow) * g.stride + n] - u[uint64_t(nb.prow) * g.stride + n];
-    const float q3 = u[uint64_t(nb.ncol) * g.stride + n] - u[uint64_t(nb.pcol) * g.stride + n];
-    v[o] = 0.5f * sqrtf(__fmaf_rn(q3, q3, __fmaf_rn(q2, q2, q1 * q1)));
 }
 
 struct UvArgs {
@@ -543,8 +601,8 @@ extern "C" int sct_grad_magn(sct_ctx *ctx, const sct_field *u, sct_field **v)
     hipError_t e = hipSuccess;
     if (!lines_cover_span(out->geo)) e = hipMemsetAsync(out->data, 0, sizeof(float) * out->geo.span(), ctx->stream);
     if (e == hipSuccess && g.lines && g.len) {
-        const uint32_t grid = uint32_t((uint64_t(g.lines) * g.chunks + 3) / 4);
-        grad_magn_kernel<<<grid, 256, 0, ctx->stream>>>(g, (const float *)u->data, (float *)out->data);
+        grad_magn_kernel<<<stream_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const float *)u->data,
+                                                                     (float *)out->data);
         e = hipGetLastError();
     }
     if (e != hipSuccess) {

@@ -175,11 +175,13 @@ def test_dykstra_cr_sqrt_and_division_match_ieee(ctx):
     assert bs.value == 0 and bd.value == 0, (bs.value, bd.value)
 
 
-@pytest.mark.parametrize("n_images,nstride", [(40, None), (40, 45), (64, 64)])
+@pytest.mark.parametrize("n_images,nstride", [(40, None), (40, 45), (64, 64), (42, 44), (301, 304)])
 def test_output_fields_padding(ctx, n_images, nstride):
     """grad_magn and convert_to_fp write every voxel of every line; where the
     lines do not cover the span (nstride > n_images) the rest of the new field
-    is zero.  The second round runs after the first round's fields were
+    is zero.  Strides that are a multiple of 4 take 16-byte accesses (42 in 44:
+    a partial last vector; 301 in 304: two trips per line, the second partial);
+    45 takes one voxel per lane.  The second round runs after the first round's fields were
     freed, so memory handed out again would show any stale values."""
     mp = S.Mapping.cylinder(n_images, 17, 15, 7.1, 8.2, 6.9, nstride)
     rng = np.random.default_rng(n_images + (nstride or 0))
