@@ -75,7 +75,7 @@ int build_group(sct_ctx *ctx, const sct_potts_problem *pr, Group &G)
         G.r_off.push_back(rb);
         const uint32_t line0 = uint32_t(G.h_lines.size());
         for (uint32_t l = 0; l < geo.lines; ++l) {
-            const LineNbr nb = geo.h_nbr[l];
+            const LineNbr nb = geo.tables->h_nbr[l];
             const auto at = [&](int32_t x) { return x >= 0 ? uint32_t(wb + uint64_t(x) * ws) : kNone; };
             BatchLine L{};
             L.w = uint32_t(wb + uint64_t(l) * ws);

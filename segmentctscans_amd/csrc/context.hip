@@ -159,9 +159,10 @@ int sct_geometry_build(sct_ctx *ctx, const sct_layout *l, Geometry &g)
     }
     if (g.base[g.width] % g.nstride != 0) return sct_fail(ctx, SCT_ERR_INVALID, "span not a multiple of nstride");
     g.lines = g.base[g.width] / g.nstride;
-    g.h_nbr.assign(g.lines, LineNbr{-1, -1, -1, -1});
-    g.line_col.assign(g.lines, 0);
-    g.line_row.assign(g.lines, 0);
+    auto t = std::make_shared<GeoTables>();
+    t->h_nbr.assign(g.lines, LineNbr{-1, -1, -1, -1});
+    t->line_col.assign(g.lines, 0);
+    t->line_row.assign(g.lines, 0);
     uint64_t rows = 0;
     auto line_of = [&](uint32_t row, uint32_t col) { return int32_t(g.base[col] / g.nstride + (row - g.start_row[col])); };
     auto in_col = [&](uint32_t row, uint32_t col) {
@@ -176,9 +177,9 @@ int sct_geometry_build(sct_ctx *ctx, const sct_layout *l, Geometry &g)
             if (row + 1 < re) nb.nrow = L + 1;
             if (col > g.col_begin && in_col(row, col - 1)) nb.pcol = line_of(row, col - 1);
             if (col + 1 < g.col_end && in_col(row, col + 1)) nb.ncol = line_of(row, col + 1);
-            g.h_nbr[size_t(L)] = nb;
-            g.line_col[size_t(L)] = col;
-            g.line_row[size_t(L)] = row;
+            t->h_nbr[size_t(L)] = nb;
+            t->line_col[size_t(L)] = col;
+            t->line_row[size_t(L)] = row;
             ++rows;
         }
     }
@@ -197,16 +198,16 @@ int sct_geometry_build(sct_ctx *ctx, const sct_layout *l, Geometry &g)
     std::vector<OrderedLine> oline(g.lines);
     for (uint32_t k = 0; k < g.lines; ++k) {
         const uint32_t L = order[k];
-        const LineNbr &nb = g.h_nbr[L];
+        const LineNbr &nb = t->h_nbr[L];
         OrderedLine &o = oline[k];
         o = OrderedLine{int32_t(L), nb.prow, nb.nrow, nb.pcol, nb.ncol, -1, -1, 0u};
         if (nb.nrow >= 0) {
-            const LineNbr &r = g.h_nbr[size_t(nb.nrow)];
+            const LineNbr &r = t->h_nbr[size_t(nb.nrow)];
             o.nrow_pcol = r.pcol;
             o.flags |= (r.nrow >= 0 ? kNrowHasNrow : 0u) | (r.ncol >= 0 ? kNrowHasNcol : 0u);
         }
         if (nb.ncol >= 0) {
-            const LineNbr &c = g.h_nbr[size_t(nb.ncol)];
+            const LineNbr &c = t->h_nbr[size_t(nb.ncol)];
             o.ncol_prow = c.prow;
             o.flags |= (c.nrow >= 0 ? kNcolHasNrow : 0u) | (c.ncol >= 0 ? kNcolHasNcol : 0u);
         }
@@ -214,20 +215,26 @@ int sct_geometry_build(sct_ctx *ctx, const sct_layout *l, Geometry &g)
     std::vector<uint2> runs;
     for (uint32_t k = 0; k < g.lines; ++k) {
         const uint32_t L = order[k];
-        if (k && L == order[k - 1] + 1 && g.line_col[L] == g.line_col[L - 1] && runs.back().y < kMarchRows)
+        if (k && L == order[k - 1] + 1 && t->line_col[L] == t->line_col[L - 1] && runs.back().y < kMarchRows)
             ++runs.back().y;
         else
             runs.push_back(make_uint2(k, 1u));
     }
     g.n_runs = uint32_t(runs.size());
     int rc;
-    if ((rc = sct_alloc(ctx, (void **)&g.d_runs, sizeof(uint2) * std::max<size_t>(1, runs.size())))) return rc;
-    if ((rc = sct_alloc(ctx, (void **)&g.d_nbr, sizeof(LineNbr) * std::max<size_t>(1, g.lines)))) return rc;
-    if ((rc = sct_alloc(ctx, (void **)&g.d_oline, sizeof(OrderedLine) * std::max<size_t>(1, g.lines)))) return rc;
-    if ((rc = sct_alloc(ctx, (void **)&g.d_start_row, sizeof(uint32_t) * g.width))) return rc;
-    if ((rc = sct_alloc(ctx, (void **)&g.d_base, sizeof(uint32_t) * (g.width + 1)))) return rc;
+    // t frees what it holds if anything below fails
+    if ((rc = sct_alloc(ctx, (void **)&t->d_runs, sizeof(uint2) * std::max<size_t>(1, runs.size())))) return rc;
+    if ((rc = sct_alloc(ctx, (void **)&t->d_nbr, sizeof(LineNbr) * std::max<size_t>(1, g.lines)))) return rc;
+    if ((rc = sct_alloc(ctx, (void **)&t->d_oline, sizeof(OrderedLine) * std::max<size_t>(1, g.lines)))) return rc;
+    if ((rc = sct_alloc(ctx, (void **)&t->d_start_row, sizeof(uint32_t) * g.width))) return rc;
+    if ((rc = sct_alloc(ctx, (void **)&t->d_base, sizeof(uint32_t) * (g.width + 1)))) return rc;
+    g.d_runs = t->d_runs;
+    g.d_nbr = t->d_nbr;
+    g.d_oline = t->d_oline;
+    g.d_start_row = t->d_start_row;
+    g.d_base = t->d_base;
     if (g.lines) {
-        SCT_HIP(ctx, hipMemcpyAsync(g.d_nbr, g.h_nbr.data(), sizeof(LineNbr) * g.lines, hipMemcpyHostToDevice, ctx->stream));
+        SCT_HIP(ctx, hipMemcpyAsync(g.d_nbr, t->h_nbr.data(), sizeof(LineNbr) * g.lines, hipMemcpyHostToDevice, ctx->stream));
         SCT_HIP(ctx, hipMemcpyAsync(g.d_oline, oline.data(), sizeof(OrderedLine) * g.lines, hipMemcpyHostToDevice,
                                     ctx->stream));
         SCT_HIP(ctx, hipMemcpyAsync(g.d_runs, runs.data(), sizeof(uint2) * runs.size(), hipMemcpyHostToDevice,
@@ -236,56 +243,31 @@ int sct_geometry_build(sct_ctx *ctx, const sct_layout *l, Geometry &g)
     SCT_HIP(ctx, hipMemcpyAsync(g.d_start_row, g.start_row.data(), sizeof(uint32_t) * g.width, hipMemcpyHostToDevice, ctx->stream));
     SCT_HIP(ctx, hipMemcpyAsync(g.d_base, g.base.data(), sizeof(uint32_t) * (g.width + 1), hipMemcpyHostToDevice, ctx->stream));
     SCT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    g.tables = std::move(t);
     return SCT_OK;
 }
 
-void sct_geometry_release(sct_ctx *ctx, Geometry &g)
+GeoTables::~GeoTables()
 {
-    sct_free(ctx, g.d_nbr);
-    sct_free(ctx, g.d_oline);
-    sct_free(ctx, g.d_runs);
-    sct_free(ctx, g.d_start_row);
-    sct_free(ctx, g.d_base);
+    // hipFree waits for the work that uses the tables; no context needed
+    for (void *p : {(void *)d_nbr, (void *)d_oline, (void *)d_runs, (void *)d_start_row, (void *)d_base})
+        sct_free(nullptr, p);
+}
+
+void sct_geometry_release(sct_ctx *, Geometry &g)
+{
+    g.tables.reset();
     g.d_nbr = nullptr;
     g.d_oline = nullptr;
     g.d_runs = nullptr;
     g.d_start_row = g.d_base = nullptr;
 }
 
-// A copy of a built geometry: the host tables by value, the device tables by
-// stream-ordered device-to-device copies (no host rebuild).
-int sct_geometry_copy(sct_ctx *ctx, const Geometry &src, Geometry &dst)
+// A copy of a built geometry: the host mapping tables by value, the line
+// tables shared (they never change).
+int sct_geometry_copy(sct_ctx *, const Geometry &src, Geometry &dst)
 {
-    dst = Geometry{};
-    dst.n_images = src.n_images;
-    dst.height = src.height;
-    dst.width = src.width;
-    dst.nstride = src.nstride;
-    dst.col_begin = src.col_begin;
-    dst.col_end = src.col_end;
-    dst.start_row = src.start_row;
-    dst.base = src.base;
-    dst.lines = src.lines;
-    dst.data_size = src.data_size;
-    dst.h_nbr = src.h_nbr;
-    dst.line_col = src.line_col;
-    dst.line_row = src.line_row;
-    dst.n_runs = src.n_runs;
-    int rc;
-    auto dup = [&](auto *&d, auto *s, size_t n) -> int {
-        const size_t bytes = sizeof(*s) * std::max<size_t>(1, n);
-        int r = sct_alloc(ctx, (void **)&d, bytes);
-        if (!r && n) {
-            const hipError_t e = hipMemcpyAsync(d, s, sizeof(*s) * n, hipMemcpyDeviceToDevice, ctx->stream);
-            if (e != hipSuccess) r = sct_hip_fail(ctx, e, "geometry copy");
-        }
-        return r;
-    };
-    if ((rc = dup(dst.d_nbr, src.d_nbr, src.lines))) return rc;
-    if ((rc = dup(dst.d_oline, src.d_oline, src.lines))) return rc;
-    if ((rc = dup(dst.d_runs, src.d_runs, src.n_runs))) return rc;
-    if ((rc = dup(dst.d_start_row, src.d_start_row, src.width))) return rc;
-    if ((rc = dup(dst.d_base, src.d_base, size_t(src.width) + 1))) return rc;
+    dst = src;
     return SCT_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -46,13 +47,29 @@ struct OrderedLine {
 };
 constexpr uint32_t kNrowHasNrow = 1, kNrowHasNcol = 2, kNcolHasNrow = 4, kNcolHasNcol = 8;
 
+// The line tables of a built geometry.  They never change, so every field
+// with that geometry shares them (a field made like another copies only the
+// pointer); the last owner frees the device tables.
+struct GeoTables {
+    std::vector<LineNbr> h_nbr;
+    std::vector<uint32_t> line_col, line_row;
+    LineNbr *d_nbr = nullptr;
+    OrderedLine *d_oline = nullptr;
+    uint2 *d_runs = nullptr;
+    uint32_t *d_start_row = nullptr, *d_base = nullptr;
+    GeoTables() = default;
+    GeoTables(const GeoTables &) = delete;
+    GeoTables &operator=(const GeoTables &) = delete;
+    ~GeoTables();
+};
+
 struct Geometry {
     uint32_t n_images = 0, height = 0, width = 0, nstride = 0, col_begin = 0, col_end = 0;
     std::vector<uint32_t> start_row, base;   // host copies of the mapping tables
     uint32_t lines = 0;
     uint64_t data_size = 0;
-    std::vector<LineNbr> h_nbr;
-    std::vector<uint32_t> line_col, line_row;
+    std::shared_ptr<const GeoTables> tables;
+    // the device tables of `tables`, for kernel arguments
     LineNbr *d_nbr = nullptr;                // device [lines]
     OrderedLine *d_oline = nullptr;          // device [lines], in band order
     // Runs of up to kMarchRows consecutive rows of one column in the band
