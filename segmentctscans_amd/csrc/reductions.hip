@@ -423,6 +423,27 @@ __global__ __launch_bounds__(256) void uv_hist_kernel(VoxGeom g, const float *__
 // the rows.  Hot bins (the phases' intensities at small gradients) then cost
 // LDS atomics within a workgroup instead of device atomics from every CU on
 // one address.
+// The joint bin of every voxel (~0u outside the ranges), for the partitioned
+// count when more than kBinPassParts partitions would each recompute it.
+constexpr uint32_t kBinPassParts = 2;
+
+__global__ __launch_bounds__(256) void uv_bin_kernel(VoxGeom g, const float *__restrict__ u,
+                                                     const float *__restrict__ v, UvArgs a, uint32_t *__restrict__ bins)
+{
+    const float *const arr[2] = {u, v};
+    wave_lines_n<2>(g, arr, blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave, g.lines,
+                    gridDim.x * (blockDim.x / kWave), [&](const float (&xy)[2], uint32_t line, uint32_t n) {
+                        const float x = xy[0], y = xy[1];
+                        uint32_t b = ~0u;
+                        if (a.u0 <= x && x <= a.u1 && a.v0 <= y && y <= a.v1)
+                            b = uint32_t(hist_bin(y, a.v0, a.v1, a.nbins)) * a.hstride +
+                                uint32_t(hist_bin(x, a.u0, a.u1, a.nbins));
+                        bins[uint64_t(line) * g.stride + n] = b;
+                    });
+}
+
+// BINS: u holds uv_bin_kernel's bins (read as 32-bit words), v is unused.
+template <bool BINS>
 __global__ __launch_bounds__(kPartThreads) void uv_part_kernel(VoxGeom g, const float *__restrict__ u,
                                                                const float *__restrict__ v, UvArgs a, uint32_t parts,
                                                                uint32_t lines_per_slice, uint32_t total,
@@ -435,16 +456,23 @@ __global__ __launch_bounds__(kPartThreads) void uv_part_kernel(VoxGeom g, const 
     __syncthreads();
     const uint32_t l0 = slice * lines_per_slice;
     const uint32_t l1 = min(g.lines, l0 + lines_per_slice);
-    const float *const arr[2] = {u, v};
-    wave_lines_n<2>(g, arr, l0 + threadIdx.x / kWave, l1, blockDim.x / kWave,
-                    [&](const float (&xy)[2], uint32_t, uint32_t) {
-                        const float x = xy[0], y = xy[1];
-                        if (!(a.u0 <= x && x <= a.u1 && a.v0 <= y && y <= a.v1)) return;
-                        const uint32_t xb = uint32_t(hist_bin(x, a.u0, a.u1, a.nbins));
-                        const uint32_t yb = uint32_t(hist_bin(y, a.v0, a.v1, a.nbins));
-                        const uint32_t b = yb * a.hstride + xb - b0;
-                        if (b < nb) atomicAdd(&sh[b], 1u);
-                    });
+    if constexpr (BINS) {
+        wave_lines(g, u, l0 + threadIdx.x / kWave, l1, blockDim.x / kWave, [&](float w, uint32_t, uint32_t) {
+            const uint32_t b = __float_as_uint(w) - b0;   // ~0u - b0 >= nb
+            if (b < nb) atomicAdd(&sh[b], 1u);
+        });
+    } else {
+        const float *const arr[2] = {u, v};
+        wave_lines_n<2>(g, arr, l0 + threadIdx.x / kWave, l1, blockDim.x / kWave,
+                        [&](const float (&xy)[2], uint32_t, uint32_t) {
+                            const float x = xy[0], y = xy[1];
+                            if (!(a.u0 <= x && x <= a.u1 && a.v0 <= y && y <= a.v1)) return;
+                            const uint32_t xb = uint32_t(hist_bin(x, a.u0, a.u1, a.nbins));
+                            const uint32_t yb = uint32_t(hist_bin(y, a.v0, a.v1, a.nbins));
+                            const uint32_t b = yb * a.hstride + xb - b0;
+                            if (b < nb) atomicAdd(&sh[b], 1u);
+                        });
+    }
     __syncthreads();
     uint32_t *row = partial + uint64_t(slice) * total + b0;
     for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) row[i] = sh[i];
@@ -643,20 +671,39 @@ extern "C" int sct_uv_hist(sct_ctx *ctx, const sct_field *u, const sct_field *v,
         lps = (g.lines + slices - 1) / slices;
         used = (g.lines + lps - 1) / lps;
     }
+    // with many partitions, the bins are computed once into a third region
+    const bool bin_pass = part_mode && parts > kBinPassParts;
+    const size_t pbytes = (sizeof(uint32_t) * uint64_t(used) * total + 255) & ~size_t(255);
     void *mem = nullptr, *host = nullptr;
-    int rc = sct_ctx_scratch(ctx, dbytes + sizeof(uint32_t) * uint64_t(used) * total, &mem);
+    int rc = sct_ctx_scratch(ctx, dbytes + pbytes + (bin_pass ? sizeof(uint32_t) * u->geo.span() : 0), &mem);
     if (!rc) rc = sct_ctx_host_scratch(ctx, sizeof(uint32_t) * total, &host);
     if (rc) return rc;
     uint32_t *d = (uint32_t *)mem, *partial = used ? (uint32_t *)((char *)mem + dbytes) : nullptr;
+    uint32_t *bins = bin_pass ? (uint32_t *)((char *)mem + dbytes + pbytes) : nullptr;
     hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t) * total, ctx->stream);
     if (e == hipSuccess && part_mode) {
         // counters in LDS, by partitions of the bins (see uv_part_kernel)
         const size_t lds = sizeof(uint32_t) * std::min<size_t>(total, kPartBins);
-        e = hipFuncSetAttribute((const void *)uv_part_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-        if (e == hipSuccess) {
-            uv_part_kernel<<<used * parts, kPartThreads, lds, ctx->stream>>>(
-                g, (const float *)u->data, (const float *)v->data, a, parts, lps, uint32_t(total), partial);
+        if (bin_pass) {
+            uv_bin_kernel<<<stream_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const float *)u->data,
+                                                                         (const float *)v->data, a, bins);
             e = hipGetLastError();
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute((const void *)uv_part_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        int(lds));
+            if (e == hipSuccess) {
+                uv_part_kernel<true><<<used * parts, kPartThreads, lds, ctx->stream>>>(
+                    g, (const float *)bins, nullptr, a, parts, lps, uint32_t(total), partial);
+                e = hipGetLastError();
+            }
+        } else {
+            e = hipFuncSetAttribute((const void *)uv_part_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(lds));
+            if (e == hipSuccess) {
+                uv_part_kernel<false><<<used * parts, kPartThreads, lds, ctx->stream>>>(
+                    g, (const float *)u->data, (const float *)v->data, a, parts, lps, uint32_t(total), partial);
+                e = hipGetLastError();
+            }
         }
         if (e == hipSuccess) {
             hist_sum_kernel<<<uint32_t((total + 255) / 256), 256, 0, ctx->stream>>>(partial, used, uint32_t(total), d);

@@ -118,9 +118,10 @@ int sct_hip_fail(sct_ctx *ctx, hipError_t e, const char *what);
 int sct_alloc(sct_ctx *ctx, void **p, size_t bytes);
 void sct_free(sct_ctx *ctx, void *p);
 // Device / pinned host scratch of at least `bytes`, owned by the context and
-// grown on demand.  For temporaries within one call: the next call on the
-// context's stream reuses the same memory (stream order keeps the uses apart;
-// the host side must have synchronised before it reuses host scratch).
+// grown on demand (kept until the context is destroyed).  For temporaries
+// within one call: the next call on the context's stream reuses the same
+// memory (stream order keeps the uses apart; the host side must have
+// synchronised before it reuses host scratch).
 int sct_ctx_scratch(sct_ctx *ctx, size_t bytes, void **p);
 int sct_ctx_host_scratch(sct_ctx *ctx, size_t bytes, void **p);
 
