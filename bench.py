@@ -36,6 +36,12 @@ def kernel_bytes(K: int, storage: str) -> dict:
             "potts_fused_kernel": 8 * (K - 1) * sb + 8 * (K - 1) + 4}
 
 
+def single_pass_bytes(K: int, storage: str) -> int:
+    """Compulsory bytes of one iteration done in a single pass: RW Xi and RW
+    V-bar at the storage size, RW V in FP32, R u (84 B for K = 3 in FP32)."""
+    return kernel_bytes(K, storage)["potts_fused_kernel"]
+
+
 KERNEL_NAMES = {1: ["potts_fused_kernel"], 2: ["potts_xi_kernel", "potts_v_kernel"]}
 
 
@@ -510,6 +516,10 @@ def main() -> int:
         "kernel_ms": {n: float(a) for n, a in zip(names, kavg)},
         "hbm_peak_frac": step_gbs / HBM_PEAK_GBS,
         "step_bytes_per_voxel_iter": step_bytes,
+        # the same rate priced at the single-pass compulsory bytes (RW Xi, RW V,
+        # RW V-bar, R u): what a one-pass iteration would have to move
+        "single_pass_bytes_per_voxel_iter": single_pass_bytes(K, args.storage),
+        "hbm_peak_frac_single_pass": single_pass_bytes(K, args.storage) * vox / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
         "roofline": {
             "bound": "hbm",
             "kernel": dom_name,
