@@ -48,6 +48,7 @@ namespace {
 
 constexpr int kAdjThreads = 256;
 constexpr int kBuckets = 21;  // replay items by sweep count 1..20 (0: no replay)
+constexpr int kShortSweeps = 4;   // replay items whose Dykstra history B2 keeps in registers (replay_short)
 // Bucket counters are kept per stripe of consecutive workgroups, so that no
 // counter takes the device atomics of more than 1/kStripes of a grid.
 constexpr int kStripes = 64;
@@ -238,7 +239,10 @@ __global__ __launch_bounds__(kAdjThreads) void adj_v_kernel(VolumeLines L, uint6
         float gq[K], gul = 0.0f;
         for (int i = 0; i < K; ++i) {
             gq[i] = act[i] ? ge[i] - mean : 0.0f;
-            GQ[uint64_t(i) * wspan + vx.w] = gq[i];
+            // gq is the projection of ge onto the active face's tangent space,
+            // so it sums to 0 over the labels: GQ keeps K - 1 components and
+            // B1 rebuilds the last
+            if (i < KM) GQ[uint64_t(i) * wspan + vx.w] = gq[i];
             const float d = uu - sp.c[i];
             const float gG = -tau1 * gq[i];
             gul += gG * 2.0f * sp.lambda * d;
@@ -307,11 +311,84 @@ __device__ __forceinline__ void dykstra_back(int sweeps, const float (*hist)[K *
             }
 }
 
+// One sweep of dykstra_back: the pairs in reverse order, h the sweep's
+// recorded pre-projection differences, gr the corrections' adjoints.
+template <int K>
+__device__ __forceinline__ void back_sweep(const float (&h)[K * (K - 1) / 2][3], float (&g)[K][3],
+                                           float (&gr)[K * (K - 1) / 2][3], DykstraLimits lim)
+{
+#pragma unroll
+    for (int i1 = K - 2; i1 >= 0; --i1)
+#pragma unroll
+        for (int i2 = K - 1; i2 > i1; --i2) {
+            const int p = (2 * K - 1 - i1) * i1 / 2 + i2 - i1 - 1;
+            const float h0 = h[p][0], h1 = h[p][1], h2 = h[p][2];
+            const float ss = __fmaf_rn(h2, h2, __fmaf_rn(h1, h1, h0 * h0));
+            float gd[3], ge[3], gh[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                gd[a] = g[i1][a] - g[i2][a];
+                ge[a] = 0.5f * gd[a] + gr[p][a];
+            }
+            if (ss > lim.t1) {
+                const float l = sqrtf(ss), il = 1.0f / l;
+                const float hg = h0 * ge[0] + h1 * ge[1] + h2 * ge[2];
+                const float k = hg * il * il * il, s1 = 1.0f - il;
+                gh[0] = s1 * ge[0] + k * h0;
+                gh[1] = s1 * ge[1] + k * h1;
+                gh[2] = s1 * ge[2] + k * h2;
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                g[i1][a] -= gh[a];
+                g[i2][a] += gh[a];
+                gr[p][a] = -0.5f * gd[a] + gh[a];
+            }
+        }
+}
+
+// Replay and back-propagation of an item whose Dykstra converges within
+// kShortSweeps = 4 sweeps, each sweep's history in its own small array: the
+// compiler keeps those in registers, where one 20-sweep array goes to private
+// memory.  Returns false, with g unchanged, if it has not converged by then.
+template <int K>
+__device__ __forceinline__ bool replay_short(const float (&x)[K][3], float (&g)[K][3], DykstraLimits lim)
+{
+    constexpr int NP = K * (K - 1) / 2;
+    float q[K][3], r[NP][3], h0[NP][3], h1[NP][3], h2[NP][3], h3[NP][3];
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) q[i][a] = x[i][a];
+#pragma unroll
+    for (int pp = 0; pp < NP; ++pp) r[pp][0] = r[pp][1] = r[pp][2] = 0.0f;
+    int sw = 1;
+    if (!(dykstra_sweep<K>(q, r, lim, h0) < lim.stol)) {
+        sw = 2;
+        if (!(dykstra_sweep<K>(q, r, lim, h1) < lim.stol)) {
+            sw = 3;
+            if (!(dykstra_sweep<K>(q, r, lim, h2) < lim.stol)) {
+                sw = 4;
+                if (!(dykstra_sweep<K>(q, r, lim, h3) < lim.stol)) return false;
+            }
+        }
+    }
+    static_assert(kShortSweeps == 4, "replay_short records four sweeps");
+    float gr[NP][3];
+#pragma unroll
+    for (int pp = 0; pp < NP; ++pp) gr[pp][0] = gr[pp][1] = gr[pp][2] = 0.0f;
+    if (sw >= 4) back_sweep<K>(h3, g, gr, lim);
+    if (sw >= 3) back_sweep<K>(h2, g, gr, lim);
+    if (sw >= 2) back_sweep<K>(h1, g, gr, lim);
+    back_sweep<K>(h0, g, gr, lim);
+    return true;
+}
+
 // dykstra_back with the sweep loop unrolled (constant history indices) and
 // the history of sweep s - 1 loaded while sweep s is back-propagated, so the
 // replay waits for one load latency per sweep, not one per pair step.
-template <int K>
-__device__ __forceinline__ void dykstra_back_unrolled(int sweeps, const float (&hist)[20][K * (K - 1) / 2][3],
+template <int K, int H = 20>
+__device__ __forceinline__ void dykstra_back_unrolled(int sweeps, const float (&hist)[H][K * (K - 1) / 2][3],
                                                       float (&g)[K][3], DykstraLimits lim)
 {
     constexpr int NP = K * (K - 1) / 2;
@@ -320,14 +397,14 @@ __device__ __forceinline__ void dykstra_back_unrolled(int sweeps, const float (&
     for (int p = 0; p < NP; ++p) gr[p][0] = gr[p][1] = gr[p][2] = 0.0f;
 #pragma unroll
     for (int p = 0; p < NP; ++p) hn[p][0] = hn[p][1] = hn[p][2] = 0.0f;
-    if (sweeps >= 20) {
+    if (sweeps >= H) {
 #pragma unroll
         for (int p = 0; p < NP; ++p)
 #pragma unroll
-            for (int a = 0; a < 3; ++a) hn[p][a] = hist[19][p][a];
+            for (int a = 0; a < 3; ++a) hn[p][a] = hist[H - 1][p][a];
     }
 #pragma unroll
-    for (int s = 19; s >= 0; --s) {
+    for (int s = H - 1; s >= 0; --s) {
         if (s >= sweeps) {
             if (s - 1 >= 0 && s - 1 < sweeps) {
 #pragma unroll
@@ -486,11 +563,21 @@ __global__ __launch_bounds__(kAdjThreads) void adj_xi_kernel(VolumeLines L, uint
         const uint32_t lane = threadIdx.x % kWave;
         const bool nx[3] = {vx.n + 1 < ch.len, ch.nr != kNone, ch.nc != kNone};
         const uint32_t wn[3] = {vx.w + 1, ch.nr + lane, ch.nc + lane};
-        float full[K][3];
-        for (int i = 0; i < K; ++i) {
-            const float own = GQ[uint64_t(i) * wspan + vx.w];
-            for (int a = 0; a < 3; ++a) full[i][a] = nx[a] ? -tau1 * (GQ[uint64_t(i) * wspan + wn[a]] - own) : 0.0f;
+        // GQ at the voxel and its +e_a neighbours, the last label's as minus
+        // the sum of the others (GQ sums to 0 over the labels)
+        float own[K], nbq[3][K], full[K][3];
+        own[KM] = 0.0f;
+        for (int a = 0; a < 3; ++a) nbq[a][KM] = 0.0f;
+        for (int i = 0; i < KM; ++i) {
+            own[i] = GQ[uint64_t(i) * wspan + vx.w];
+            own[KM] -= own[i];
+            for (int a = 0; a < 3; ++a) {
+                nbq[a][i] = nx[a] ? GQ[uint64_t(i) * wspan + wn[a]] : 0.0f;
+                nbq[a][KM] -= nbq[a][i];
+            }
         }
+        for (int i = 0; i < K; ++i)
+            for (int a = 0; a < 3; ++a) full[i][a] = nx[a] ? -tau1 * (nbq[a][i] - own[i]) : 0.0f;
         for (int a = 0; a < 3; ++a) {
             for (int i = 0; i < KM; ++i) g[i][a] = gXi[uint64_t(3 * i + a) * wspan + vx.w] + full[i][a] - full[KM][a];
             g[KM][a] = 0.0f;
@@ -551,26 +638,34 @@ __global__ __launch_bounds__(kAdjThreads) void adj_xi_replay_kernel(uint64_t wsp
                 x[i][a] = it[3 * i + a];
                 g[i][a] = i < KM ? it[3 * K + 3 * i + a] : 0.0f;
             }
-        if constexpr (NP <= 3) {
-            // whole history in registers: both sweep loops fully unrolled
-            float hist[20][NP][3], r[NP][3];
+        // the item's bucket: the sweeps the forward's Dykstra took
+        uint32_t bk = kBuckets - 1;
+        while (bk > 1 && t >= start[bk - 1]) --bk;
+        bool replayed = false;
+        if constexpr (NP <= 3)
+            if (bk <= uint32_t(kShortSweeps)) replayed = replay_short<K>(x, g, lim);
+        if (!replayed) {
+            if constexpr (NP <= 3) {
+                // whole history unrolled (held in private memory)
+                float hist[20][NP][3], r[NP][3];
 #pragma unroll
-            for (int pp = 0; pp < NP; ++pp) r[pp][0] = r[pp][1] = r[pp][2] = 0.0f;
-            int sw = 0;
-            bool run = true;
+                for (int pp = 0; pp < NP; ++pp) r[pp][0] = r[pp][1] = r[pp][2] = 0.0f;
+                int sw = 0;
+                bool run = true;
 #pragma unroll
-            for (int s = 0; s < 20; ++s) {
-                if (run) {
-                    const float m2 = dykstra_sweep<K>(x, r, lim, hist[s]);
-                    sw = s + 1;
-                    run = !(m2 < lim.stol);
+                for (int s = 0; s < 20; ++s) {
+                    if (run) {
+                        const float m2 = dykstra_sweep<K>(x, r, lim, hist[s]);
+                        sw = s + 1;
+                        run = !(m2 < lim.stol);
+                    }
                 }
+                dykstra_back_unrolled<K>(sw, hist, g, lim);
+            } else {
+                float hist[20][NP][3];
+                const int sw = dykstra_replay<K>(x, hist, lim);
+                dykstra_back<K>(sw, hist, g, lim);
             }
-            dykstra_back_unrolled<K>(sw, hist, g, lim);
-        } else {
-            float hist[20][NP][3];
-            const int sw = dykstra_replay<K>(x, hist, lim);
-            dykstra_back<K>(sw, hist, g, lim);
         }
         xi_outputs<K>(wspan, __float_as_uint(it[6 * K - 3]), tau2, g, gXi, GP);
     }
@@ -724,7 +819,7 @@ struct Tape : TapeBase {
         for (auto &p : B.ck) alloc(p, state);
         for (auto &p : B.seg) alloc(p, state);
         alloc(B.gs, 4 * uint64_t(K - 1) * S);   // gV | gXi
-        alloc(B.GQ, uint64_t(K) * S);
+        alloc(B.GQ, uint64_t(K - 1) * S);
         alloc(B.GP, 3 * uint64_t(K) * S);
         alloc(B.gu, S);
         agrid = uint32_t((Ly.one.count() * kWave + kAdjThreads - 1) / kAdjThreads);
