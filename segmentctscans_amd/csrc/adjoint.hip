@@ -18,8 +18,8 @@
 //      step is back-propagated: the excess map f(h) = h (1 - 1/|h|) for
 //      |h| > 1 has Jacobian (1 - 1/|h|) I + h h^T / |h|^3;
 //   C  grad's adjoint is -div, into V-bar; evaluated inside the next
-//      step's A (it needs only B's output at the voxel and its -e_a
-//      neighbours), so the V-bar adjoint is never stored.
+//      step's A from B's Xi adjoint at the voxel and its -e_a neighbours,
+//      so the V-bar adjoint is never stored.
 // The implicit K-th components turn each full-K adjoint g into g_i - g_{K-1}
 // on the stored components.
 //
@@ -146,27 +146,28 @@ __global__ __launch_bounds__(kAdjThreads) void adj_loss_kernel(VolumeLines L, ui
     block_add<1>(part, loss);
 }
 
-// Adjoint of V-bar+ from the previous reverse step's GP = tau2 x (adjoint of
-// Dykstra's input): grad's adjoint is -div, on all K components, then the
-// implicit K-th component folds into the stored ones.
+// Adjoint of V-bar+ from the previous reverse step's B: with g the adjoint of
+// Dykstra's input, grad's adjoint is -div of tau2 g on all K components, and
+// the implicit K-th component folds into the stored ones as
+// -div(tau2 (g_i - g_{K-1})).  g_i - g_{K-1} is gXi, B's output for the Xi
+// state, which the next B1 has not yet overwritten when A runs.
 template <int K>
-__device__ __forceinline__ void vbar_adjoint(const Voxel &vx, uint64_t wspan, const float *GP, float (&gvb)[K - 1])
+__device__ __forceinline__ void vbar_adjoint(const Voxel &vx, uint64_t wspan, const float *gXi, float tau2,
+                                             float (&gvb)[K - 1])
 {
     const Chunk &ch = vx.ch;
     const uint32_t lane = threadIdx.x % kWave;
     const bool px[3] = {vx.n > 0, ch.pr != kNone, ch.pc != kNone};
     const bool nx[3] = {vx.n + 1 < ch.len, ch.nr != kNone, ch.nc != kNone};
     const uint32_t wp[3] = {vx.w - 1, ch.pr + lane, ch.pc + lane};
-    float full[K];
-    for (int i = 0; i < K; ++i) {
+    for (int i = 0; i < K - 1; ++i) {
         float dv = 0.0f;
         for (int a = 0; a < 3; ++a) {
-            if (nx[a]) dv += GP[uint64_t(3 * i + a) * wspan + vx.w];
-            if (px[a]) dv -= GP[uint64_t(3 * i + a) * wspan + wp[a]];
+            if (nx[a]) dv += gXi[uint64_t(3 * i + a) * wspan + vx.w];
+            if (px[a]) dv -= gXi[uint64_t(3 * i + a) * wspan + wp[a]];
         }
-        full[i] = -dv;
+        gvb[i] = -tau2 * dv;
     }
-    for (int i = 0; i < K - 1; ++i) gvb[i] = full[i] - full[K - 1];
 }
 
 // Replay items of one reverse step, ordered by sweep count: hist[z][s] counts
@@ -190,7 +191,7 @@ __device__ __forceinline__ uint32_t stripe_of(uint32_t block, uint32_t blocks)
 template <int K>
 __global__ __launch_bounds__(kAdjThreads) void adj_v_kernel(VolumeLines L, uint64_t wspan, const float *st,
                                                             const float *st1, const uint8_t *rec1, const float *u,
-                                                            SolveParams<K> sp, float tau1, const float *GP,
+                                                            SolveParams<K> sp, float tau1, float tau2,
                                                             float *gs, float *GQ, float *gu, double *acc,
                                                             ReplayBuckets rb)
 {
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_v_kernel(VolumeLines L, uint6
         const Chunk &ch = vx.ch;
         const uint32_t lane = threadIdx.x % kWave;
         float gvb[KM];
-        vbar_adjoint<K>(vx, wspan, GP, gvb);
+        vbar_adjoint<K>(vx, wspan, gs + uint64_t(KM) * wspan, tau2, gvb);
         const float *Xi1 = st1 + uint64_t(2 * KM) * wspan;
         const bool px[3] = {vx.n > 0, ch.pr != kNone, ch.pc != kNone};
         const bool nx[3] = {vx.n + 1 < ch.len, ch.nr != kNone, ch.nc != kNone};
@@ -456,15 +457,13 @@ __device__ __forceinline__ void dykstra_back_unrolled(int sweeps, const float (&
     }
 }
 
-// Writes the Xi-step adjoints of a voxel: g = adjoint of Dykstra's input x.
+// Writes the Xi-step adjoint of a voxel, g = adjoint of Dykstra's input x, on
+// the stored components (the next A also reads it, for V-bar's adjoint).
 template <int K>
-__device__ __forceinline__ void xi_outputs(uint64_t wspan, uint32_t w, float tau2, const float (&g)[K][3],
-                                           float *gXi, float *GP)
+__device__ __forceinline__ void xi_outputs(uint64_t wspan, uint32_t w, const float (&g)[K][3], float *gXi)
 {
     for (int i = 0; i < K - 1; ++i)
         for (int a = 0; a < 3; ++a) gXi[uint64_t(3 * i + a) * wspan + w] = g[i][a] - g[K - 1][a];
-    for (int i = 0; i < K; ++i)
-        for (int a = 0; a < 3; ++a) GP[uint64_t(3 * i + a) * wspan + w] = tau2 * g[i][a];
 }
 
 // Voxels whose Dykstra replay is needed, in buckets of equal sweep count,
@@ -544,7 +543,7 @@ constexpr int item_floats() { return 6 * K - 2; }   // x: 3K, g: 3(K - 1) (g_{K-
 template <int K>
 __global__ __launch_bounds__(kAdjThreads) void adj_xi_kernel(VolumeLines L, uint64_t wspan, const float *st,
                                                              const uint8_t *rec1, float tau1, float tau2,
-                                                             const float *GQ, float *gs, float *GP, XiQueue Q,
+                                                             const float *GQ, float *gs, XiQueue Q,
                                                              const uint32_t *hist, const uint32_t *offset)
 {
     constexpr int KM = K - 1;
@@ -597,7 +596,7 @@ __global__ __launch_bounds__(kAdjThreads) void adj_xi_kernel(VolumeLines L, uint
             xi_input<K>(vb, vbn, nx, xo, tau2, x);
             rank = atomicAdd(&cnt[sw], 1u);
         } else {
-            xi_outputs<K>(wspan, vx.w, tau2, g, gXi, GP);
+            xi_outputs<K>(wspan, vx.w, g, gXi);
         }
     }
     if (threadIdx.x < kBuckets) base[threadIdx.x] = offset[uint64_t(blockIdx.x) * kStripeWords + threadIdx.x];
@@ -619,9 +618,8 @@ __global__ __launch_bounds__(kAdjThreads) void adj_xi_kernel(VolumeLines L, uint
 // B2: the bucketed voxels, one per lane (longest first): Dykstra replayed from
 // x recording each pair's h, then back-propagated.
 template <int K>
-__global__ __launch_bounds__(kAdjThreads) void adj_xi_replay_kernel(uint64_t wspan, float tau2, DykstraLimits lim,
-                                                                    float *gs, float *GP, XiQueue Q,
-                                                                    const uint32_t *hist)
+__global__ __launch_bounds__(kAdjThreads) void adj_xi_replay_kernel(uint64_t wspan, DykstraLimits lim, float *gs,
+                                                                    XiQueue Q, const uint32_t *hist)
 {
     constexpr int KM = K - 1, NP = K * (K - 1) / 2;
     __shared__ uint32_t start[kBuckets + 1];
@@ -667,13 +665,13 @@ __global__ __launch_bounds__(kAdjThreads) void adj_xi_replay_kernel(uint64_t wsp
                 dykstra_back<K>(sw, hist, g, lim);
             }
         }
-        xi_outputs<K>(wspan, __float_as_uint(it[6 * K - 3]), tau2, g, gXi, GP);
+        xi_outputs<K>(wspan, __float_as_uint(it[6 * K - 3]), g, gXi);
     }
 }
 
 struct Buffers {
     std::vector<float *> ck, seg;
-    float *gs = nullptr, *GQ = nullptr, *GP = nullptr, *gu = nullptr, *items = nullptr;
+    float *gs = nullptr, *GQ = nullptr, *gu = nullptr, *items = nullptr;
     uint32_t *buckets = nullptr;   // ReplayBuckets: the two histograms
     uint32_t *offset = nullptr;    // ReplayBuckets: kStripeWords per workgroup of A and B1
     double *acc = nullptr;   // dc[K], dlambda, loss
@@ -683,7 +681,7 @@ void free_buffers(sct_ctx *ctx, Buffers &b)
 {
     for (float *p : b.ck) sct_free(ctx, p);
     for (float *p : b.seg) sct_free(ctx, p);
-    for (void *p : {(void *)b.gs, (void *)b.GQ, (void *)b.GP, (void *)b.gu, (void *)b.items, (void *)b.buckets, (void *)b.offset,
+    for (void *p : {(void *)b.gs, (void *)b.GQ, (void *)b.gu, (void *)b.items, (void *)b.buckets, (void *)b.offset,
                     (void *)b.acc})
         sct_free(ctx, p);
     b = Buffers{};
@@ -820,7 +818,6 @@ struct Tape : TapeBase {
         for (auto &p : B.seg) alloc(p, state);
         alloc(B.gs, 4 * uint64_t(K - 1) * S);   // gV | gXi
         alloc(B.GQ, uint64_t(K - 1) * S);
-        alloc(B.GP, 3 * uint64_t(K) * S);
         alloc(B.gu, S);
         agrid = uint32_t((Ly.one.count() * kWave + kAdjThreads - 1) / kAdjThreads);
         // per-workgroup sums: dc[K], dlambda of every A launch, then the loss
@@ -910,7 +907,6 @@ struct Tape : TapeBase {
         }
         chk(hipMemsetAsync(B.gs, 0, sizeof(float) * 4 * (K - 1) * S, st), "memset");
         chk(hipMemsetAsync(B.gu, 0, sizeof(float) * S, st), "memset");
-        chk(hipMemsetAsync(B.GP, 0, sizeof(float) * 3 * K * S, st), "memset");
         chk(hipMemsetAsync(B.acc, 0, sizeof(double) * std::max<uint64_t>(1, nacc), st), "memset");
         chk(hipMemsetAsync(B.buckets, 0, sizeof(uint32_t) * kBucketWords, st), "memset");
         if (target) {
@@ -941,11 +937,11 @@ struct Tape : TapeBase {
                 const float *s0 = sg[size_t(t - t0)], *s1 = sg[size_t(t - t0) + 1];
                 uint32_t *const bw = B.buckets;
                 const ReplayBuckets rb{bw + par * kCounters, bw + (1 - par) * kCounters, B.offset};
-                adj_v_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, s1, rec(s1), u(), prm.p, tau1, B.GP,
+                adj_v_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, s1, rec(s1), u(), prm.p, tau1, tau2,
                                                                 B.gs, B.GQ, B.gu, B.acc, rb);
-                adj_xi_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, rec(s1), tau1, tau2, B.GQ, B.gs, B.GP,
-                                                                 Q, rb.hist_cur, rb.offset);
-                adj_xi_replay_kernel<K><<<rgrid, kAdjThreads, 0, st>>>(S, tau2, lim, B.gs, B.GP, Q, rb.hist_cur);
+                adj_xi_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, s0, rec(s1), tau1, tau2, B.GQ, B.gs, Q,
+                                                                 rb.hist_cur, rb.offset);
+                adj_xi_replay_kernel<K><<<rgrid, kAdjThreads, 0, st>>>(S, lim, B.gs, Q, rb.hist_cur);
                 par = 1 - par;
             }
         }
