@@ -670,7 +670,9 @@ __global__ __launch_bounds__(kAdjThreads) void adj_xi_replay_kernel(uint64_t wsp
 }
 
 struct Buffers {
-    std::vector<float *> ck, seg;
+    // seg: the forward's working state (seg[0], the final state) and the
+    // recompute states of even segments; seg2[1..]: those of odd segments
+    std::vector<float *> ck, seg, seg2;
     float *gs = nullptr, *GQ = nullptr, *gu = nullptr, *items = nullptr;
     uint32_t *buckets = nullptr;   // ReplayBuckets: the two histograms
     uint32_t *offset = nullptr;    // ReplayBuckets: kStripeWords per workgroup of A and B1
@@ -681,6 +683,7 @@ void free_buffers(sct_ctx *ctx, Buffers &b)
 {
     for (float *p : b.ck) sct_free(ctx, p);
     for (float *p : b.seg) sct_free(ctx, p);
+    for (float *p : b.seg2) sct_free(ctx, p);
     for (void *p : {(void *)b.gs, (void *)b.GQ, (void *)b.gu, (void *)b.items, (void *)b.buckets, (void *)b.offset,
                     (void *)b.acc})
         sct_free(ctx, p);
@@ -748,6 +751,10 @@ struct Tape : TapeBase {
     float tau1, tau2;
     uint32_t fgrid = 0, xgrid = 0, agrid = 0, rgrid = 0;
     float *final = nullptr;   // the state after the last iteration (seg[0], never recomputed into)
+    // The reverse pass recomputes segment k - 1 on `side` while segment k's
+    // reverse kernels run on the context's stream; events order the two.
+    hipStream_t side = nullptr;
+    hipEvent_t recomputed[2] = {}, reversed[2] = {}, started = nullptr;
     int rc = SCT_OK;
 
     Tape(sct_ctx *c, const sct_field *gf, const float *cv, float lambda, float t1, float t2, int it, int ckp)
@@ -756,7 +763,14 @@ struct Tape : TapeBase {
         for (int i = 0; i < K; ++i) prm.p.c[i] = cv[i];
         prm.p.lambda = lambda;
     }
-    ~Tape() override { free_buffers(ctx, B); }
+    ~Tape() override
+    {
+        if (side) (void)hipStreamSynchronize(side);
+        free_buffers(ctx, B);
+        for (hipEvent_t e : {recomputed[0], recomputed[1], reversed[0], reversed[1], started})
+            if (e) (void)hipEventDestroy(e);
+        if (side) (void)hipStreamDestroy(side);
+    }
     const sct_field *input() const override { return g; }
 
     void chk(hipError_t e, const char *what)
@@ -777,9 +791,9 @@ struct Tape : TapeBase {
     }
     // One iteration from state src into state dst (src unchanged): the Xi
     // step writes dst's Xi, the V step reads it with src's V.
-    void step_into(const float *src, float *dst)
+    void step_into(const float *src, float *dst) { step_into(src, dst, ctx->stream); }
+    void step_into(const float *src, float *dst, hipStream_t st)
     {
-        hipStream_t st = ctx->stream;
         const auto in = state_args<K>(const_cast<float *>(src), u(), S);
         const auto out = state_args<K>(dst, u(), S);
         auto ax = in;
@@ -814,8 +828,19 @@ struct Tape : TapeBase {
         // seg[1 .. seg_len - 1]: the states a segment recomputes between its
         // two checkpoints
         B.seg.assign(size_t(seg_len), nullptr);
+        B.seg2.assign(size_t(nck > 1 ? seg_len : 1), nullptr);   // [0] unused
         for (auto &p : B.ck) alloc(p, state);
         for (auto &p : B.seg) alloc(p, state);
+        for (size_t j = 1; j < B.seg2.size(); ++j) alloc(B.seg2[j], state);
+        if (!rc && hipStreamCreateWithFlags(&side, hipStreamNonBlocking) != hipSuccess) {
+            side = nullptr;
+            rc = sct_fail(ctx, SCT_ERR_HIP, "hipStreamCreate failed");
+        }
+        for (hipEvent_t *e : {&recomputed[0], &recomputed[1], &reversed[0], &reversed[1], &started})
+            if (!rc && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
+                *e = nullptr;
+                rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventCreate failed");
+            }
         alloc(B.gs, 4 * uint64_t(K - 1) * S);   // gV | gXi
         alloc(B.GQ, uint64_t(K - 1) * S);
         alloc(B.gu, S);
@@ -918,20 +943,46 @@ struct Tape : TapeBase {
             for (int i = 0; i < K; ++i) d.p[i] = (const float *)dV[i]->data;
             adj_seed_kernel<K><<<agrid, kAdjThreads, 0, st>>>(Ly.one, S, d, B.gs);
         }
-        // reverse pass, one checkpoint segment at a time
+        // reverse pass, one checkpoint segment at a time; segment k's states
+        // t0 .. t1 are checkpoint k, the states between (recomputed from it
+        // into the buffer set of k's parity), and t1 = checkpoint k + 1 or
+        // the final state.  Segment k - 1 is recomputed on the side stream
+        // while segment k's reverse kernels run.
         chk(hipEventRecord(ev[0], st), "hipEventRecord");
+        auto set_of = [&](int k) -> const std::vector<float *> & { return (k % 2) ? B.seg2 : B.seg; };
+        // states a segment recomputes (none when checkpoints are every iteration)
+        auto inner = [&](int k) { return std::min(iters, k * ck + ck) - k * ck - 1; };
+        auto recompute = [&](int k) {
+            if (inner(k) < 1) return;
+            const int t0 = k * ck, m = std::min(iters, t0 + ck) - t0;
+            const std::vector<float *> &set = set_of(k);
+            const float *prev = B.ck[size_t(k)];
+            for (int j = 1; j < m; ++j) {
+                step_into(prev, set[size_t(j)], side);
+                prev = set[size_t(j)];
+            }
+            chk(hipEventRecord(recomputed[k % 2], side), "hipEventRecord");
+        };
+        // the side stream starts after this call's set-up (and any earlier
+        // reverse pass that read the recompute buffers)
+        if (inner(0) >= 1) {
+            chk(hipEventRecord(started, st), "hipEventRecord");
+            chk(hipStreamWaitEvent(side, started, 0), "hipStreamWaitEvent");
+        }
+        if (!rc) recompute(nck - 1);
         int par = 0;   // which bucket histogram the current reverse step uses
         for (int k = nck - 1; k >= 0 && !rc; --k) {
-            const int t0 = k * ck, t1 = std::min(iters, t0 + ck);
-            // states t0 .. t1 of the segment: checkpoint k, the states between
-            // recomputed from it, and t1 = checkpoint k + 1 or the final state
-            const int m = t1 - t0;
+            if (k >= 1 && inner(k - 1) >= 1) {
+                // segment k - 1 reuses segment k + 1's buffers: wait for its reverse
+                if (k + 1 < nck && inner(k + 1) >= 1)
+                    chk(hipStreamWaitEvent(side, reversed[(k + 1) % 2], 0), "hipStreamWaitEvent");
+                recompute(k - 1);
+            }
+            if (inner(k) >= 1) chk(hipStreamWaitEvent(st, recomputed[k % 2], 0), "hipStreamWaitEvent");
+            const int t0 = k * ck, t1 = std::min(iters, t0 + ck), m = t1 - t0;
             std::vector<const float *> sg(size_t(m) + 1);
             sg[0] = B.ck[size_t(k)];
-            for (int j = 1; j < m; ++j) {
-                step_into(sg[size_t(j) - 1], B.seg[size_t(j)]);
-                sg[size_t(j)] = B.seg[size_t(j)];
-            }
+            for (int j = 1; j < m; ++j) sg[size_t(j)] = set_of(k)[size_t(j)];
             sg[size_t(m)] = k + 1 < nck ? B.ck[size_t(k) + 1] : final;
             for (int t = t1 - 1; t >= t0; --t) {
                 const float *s0 = sg[size_t(t - t0)], *s1 = sg[size_t(t - t0) + 1];
@@ -944,6 +995,7 @@ struct Tape : TapeBase {
                 adj_xi_replay_kernel<K><<<rgrid, kAdjThreads, 0, st>>>(S, lim, B.gs, Q, rb.hist_cur);
                 par = 1 - par;
             }
+            if (inner(k) >= 1) chk(hipEventRecord(reversed[k % 2], st), "hipEventRecord");
         }
         chk(hipEventRecord(ev[1], st), "hipEventRecord");
         chk(hipGetLastError(), "adjoint launch");
