@@ -270,8 +270,11 @@ int sct_potts_batch_set_tiles(sct_ctx *ctx, const int *waves);
  * c[0..K-1], lambda and u, by reverse-mode differentiation of the unrolled
  * iteration (FP32).  States are checkpointed every checkpoint_every
  * iterations (0: round(sqrt(iters))) and the states between two checkpoints
- * recomputed segment by segment; checkpoint_every = 1 keeps all iters + 1
- * states (5 (K - 1) floats per padded voxel each) and recomputes none.
+ * recomputed segment by segment, the next segment on a second stream while
+ * the current one is differentiated (memory: iters / checkpoint_every + 2
+ * checkpoint_every states); checkpoint_every = 1 keeps all iters + 1 states
+ * and recomputes none.  A state is 5 (K - 1) floats and one byte per padded
+ * voxel.
  * *grad_u (optional) receives a new field; times_ms[2] (optional) the event
  * times of the forward pass and of the reverse pass.  Synchronises. */
 int sct_potts_adjoint(sct_ctx *ctx, const sct_field *g, const sct_field *target, int K, const float *c,
