@@ -258,11 +258,15 @@ def crop_agreement(S, ph, c, lam, tau, iters: int, size: int) -> dict:
 def reductions_workload(S, reps: int = 5) -> dict:
     """The reductions on the 512^3 phantom (85,370,880 voxels): min_max,
     calc_hist with 1024 and 65536 bins over [min, max], grad_magn,
-    grad_min_max and the 512 x 512 uv_hist.  Each call is timed with HIP
-    events on the library's stream (median of reps); achieved bandwidth is
-    the algorithmic bytes per voxel (4 for one read, 8 for two arrays) over
-    that time.  Calls include their set-up and, where the result goes to the
-    host, the copy back."""
+    grad_min_max and the 512 x 512 uv_hist; and the ingest kernels on the
+    same voxels as u16 (phantom x 30000 + 12000): find_intensity_range (its
+    65536-bin histogram and tail cut) and convert_to_fp to a float field.
+    Each call is timed with HIP events on the library's stream (median of
+    reps); achieved bandwidth is the algorithmic bytes per voxel (4 for one
+    float read, 8 for two arrays, 2 for a u16 read, 6 for u16 in and float
+    out) over that time.  Calls include their set-up and, where the result
+    goes to the host, the copy back."""
+    from segmentctscans_amd import Field
     from segmentctscans_amd.hiptime import EventTimeline
     from segmentctscans_amd.phantom import make_phantom
     ph = make_phantom(512, 512, 4.0, 20.0, seed=7)
@@ -272,6 +276,10 @@ def reductions_workload(S, reps: int = 5) -> dict:
     v = S.grad_magn(ctx, u)
     mn, mx = S.min_max(ctx, u)
     v0, v1 = S.grad_min_max(ctx, v)
+    valid = ph.field.mapping.valid_mask()
+    raw = np.zeros(ph.field.mapping.span, np.uint16)
+    raw[valid] = np.clip(np.round(ph.field.data[valid] * 30000 + 12000), 0, 65535).astype(np.uint16)
+    r16 = S.upload(ctx, Field(ph.field.mapping, raw))
     ops = {
         "min_max": (4, lambda: S.min_max(ctx, u)),
         "calc_hist_1024": (4, lambda: S.calc_hist(ctx, u, mn, mx, 1024, 0.0, counts=True)),
@@ -279,6 +287,8 @@ def reductions_workload(S, reps: int = 5) -> dict:
         "grad_magn": (8, lambda: S.grad_magn(ctx, u).close()),
         "grad_min_max": (4, lambda: S.grad_min_max(ctx, v)),
         "uv_hist_512": (8, lambda: S.uv_hist(ctx, u, v, mn, mx, v0, v1, 512)),
+        "find_intensity_range_u16": (2, lambda: S.find_intensity_range(ctx, r16, 0.001)),
+        "convert_to_fp_u16": (6, lambda: S.convert_to_fp(ctx, r16, 14000.0, 40000.0).close()),
     }
     out = {"workload": f"512^3 cylinder r=230.4 ({vox} voxels)"}
     for name, (bpv, fn) in ops.items():
@@ -295,6 +305,7 @@ def reductions_workload(S, reps: int = 5) -> dict:
         t = float(np.median(ms))
         out[name] = {"ms": t, "bytes_per_voxel": bpv, "achieved_GBs": bpv * vox / (t / 1e3) / 1e9,
                      "frac": bpv * vox / (t / 1e3) / 1e9 / HBM_PEAK_GBS}
+    r16.close()
     v.close()
     u.close()
     return out

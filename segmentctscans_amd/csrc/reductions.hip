@@ -765,33 +765,61 @@ __global__ __launch_bounds__(256) void convert_kernel(VoxGeom g, const T *__rest
 // u16 value.  Two 16-bit counters per LDS word; a workgroup counts at most
 // 65535 voxels (lines_per_block lines) so no half-word carries, then adds its
 // non-zero counters to the device histogram.  PACKED = false: device atomics.
-template <bool PACKED>
+// find_intensity_range's 65536-bin histogram by bin partitions, as
+// hist_part_kernel: workgroup (slice of lines, partition of 32768 bins) counts
+// its slice's voxels whose value (the bin) falls in its partition in LDS and
+// stores the counts as its slice's row of `partial`; hist_sum_kernel adds the
+// rows.  A wave walks a line at a time, each lane 4 values per trip (one
+// 8-byte load when the stride is a multiple of 4).
+__global__ __launch_bounds__(kPartThreads) void u16_part_kernel(VoxGeom g, const uint16_t *__restrict__ in,
+                                                                uint32_t lines_per_slice, uint32_t *__restrict__ partial)
+{
+    extern __shared__ uint32_t sh[];
+    const uint32_t part = blockIdx.x % 2, slice = blockIdx.x / 2;
+    const uint32_t b0 = part * kPartBins;
+    for (uint32_t i = threadIdx.x; i < kPartBins; i += blockDim.x) sh[i] = 0;
+    __syncthreads();
+    const uint32_t l0 = slice * lines_per_slice, l1 = min(g.lines, l0 + lines_per_slice);
+    const uint32_t lane = threadIdx.x % kWave, nw = blockDim.x / kWave;
+    auto count = [&](uint32_t v) {
+        const uint32_t b = v - b0;
+        if (b < kPartBins) atomicAdd(&sh[b], 1u);
+    };
+    for (uint32_t line = l0 + threadIdx.x / kWave; line < l1; line += nw) {
+        const uint16_t *row = in + uint64_t(line) * g.stride;
+        if (g.stride % 4 == 0) {
+            for (uint32_t n0 = lane * 4; n0 < g.len; n0 += kWave * 4) {
+                if (n0 + 4 <= g.len) {
+                    const uint2 q = *reinterpret_cast<const uint2 *>(row + n0);
+                    count(q.x & 0xffffu);
+                    count(q.x >> 16);
+                    count(q.y & 0xffffu);
+                    count(q.y >> 16);
+                } else {
+                    for (uint32_t n = n0; n < g.len; ++n) count(row[n]);
+                }
+            }
+        } else {
+            for (uint32_t n = lane; n < g.len; n += kWave) count(row[n]);
+        }
+    }
+    __syncthreads();
+    uint32_t *out = partial + uint64_t(slice) * 65536u + b0;
+    for (uint32_t i = threadIdx.x; i < kPartBins; i += blockDim.x) out[i] = sh[i];
+}
+
+// The same histogram with device atomics, for a device without 128 KB of LDS
+// per workgroup.
 __global__ __launch_bounds__(256) void u16_hist_kernel(VoxGeom g, const uint16_t *__restrict__ in,
                                                        uint32_t lines_per_block, uint32_t *__restrict__ counts)
 {
-    extern __shared__ uint32_t sh[];
-    constexpr uint32_t nwords = 32768;
-    if constexpr (PACKED) {
-        for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) sh[i] = 0;
-        __syncthreads();
-    }
     const uint32_t l0 = blockIdx.x * lines_per_block, l1 = min(g.lines, l0 + lines_per_block);
     const uint32_t lane = threadIdx.x % kWave, nw = blockDim.x / kWave;
     const uint32_t pairs = l0 < l1 ? (l1 - l0) * g.chunks : 0;
     for (uint32_t p = threadIdx.x / kWave; p < pairs; p += nw) {
         const uint32_t line = l0 + p / g.chunks, n = (p % g.chunks) * kWave + lane;
         if (n >= g.len) continue;
-        const uint32_t b = in[uint64_t(line) * g.stride + n];
-        if constexpr (PACKED) atomicAdd(&sh[b >> 1], 1u << (16 * (b & 1)));
-        else atomicAdd(&counts[b], 1u);
-    }
-    if constexpr (PACKED) {
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) {
-            const uint32_t c = sh[i];
-            if (c & 0xffffu) atomicAdd(&counts[2 * i], c & 0xffffu);
-            if (c >> 16) atomicAdd(&counts[2 * i + 1], c >> 16);
-        }
+        atomicAdd(&counts[in[uint64_t(line) * g.stride + n]], 1u);
     }
 }
 
@@ -837,32 +865,38 @@ extern "C" int sct_find_intensity_range(sct_ctx *ctx, const sct_field *f, double
     if (!ctx || !f || !lo || !hi) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
     if (f->dtype != SCT_U16) return sct_fail(ctx, SCT_ERR_INVALID, "u16 field required");
     constexpr uint32_t nbins = 65536;
-    uint32_t *d = nullptr, *h = nullptr;
-    int rc = sct_ctx_scratch(ctx, sizeof(uint32_t) * nbins, (void **)&d);
+    const VoxGeom g = vox_geom(f);
+    // counts d, then a row of partial counts per slice of lines (two
+    // partitions of 32768 bins), in the context's scratch
+    const uint32_t slices = std::max<uint32_t>(1, std::min<uint32_t>(std::max<uint32_t>(g.lines, 1), ctx->cu_count / 2));
+    const uint32_t lps = std::max<uint32_t>(1, (g.lines + slices - 1) / slices);
+    const uint32_t used = (g.lines + lps - 1) / lps;
+    const size_t dbytes = sizeof(uint32_t) * nbins;
+    uint32_t *h = nullptr;
+    void *mem = nullptr;
+    int rc = sct_ctx_scratch(ctx, dbytes + sizeof(uint32_t) * uint64_t(used) * nbins, &mem);
     if (!rc) rc = sct_ctx_host_scratch(ctx, sizeof(uint32_t) * nbins, (void **)&h);
     if (rc) return rc;
-    const VoxGeom g = vox_geom(f);
+    uint32_t *d = (uint32_t *)mem, *partial = (uint32_t *)((char *)mem + dbytes);
     hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t) * nbins, ctx->stream);
     if (e == hipSuccess && g.lines && g.len) {
-        const size_t lds = sizeof(uint32_t) * nbins / 2;
-        bool packed = g.len <= 65535 &&
-                      hipFuncSetAttribute((const void *)u16_hist_kernel<true>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) == hipSuccess;
-        if (packed) {
-            const uint32_t lpb = std::max<uint32_t>(1, 65535u / g.len);
-            u16_hist_kernel<true><<<(g.lines + lpb - 1) / lpb, 256, lds, ctx->stream>>>(g, (const uint16_t *)f->data,
-                                                                                         lpb, d);
-            if (hipGetLastError() != hipSuccess) packed = false;
-        }
-        if (!packed) {
+        const size_t lds = sizeof(uint32_t) * kPartBins;
+        if (hipFuncSetAttribute((const void *)u16_part_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                int(lds)) == hipSuccess) {
+            u16_part_kernel<<<used * 2, kPartThreads, lds, ctx->stream>>>(g, (const uint16_t *)f->data, lps, partial);
+            e = hipGetLastError();
+            if (e == hipSuccess) {
+                hist_sum_kernel<<<nbins / 256, 256, 0, ctx->stream>>>(partial, used, nbins, d);
+                e = hipGetLastError();
+            }
+        } else {
+            // no 128 KB of LDS: device atomics
             (void)hipGetLastError();
             const uint32_t blocks = std::max<uint32_t>(1, ctx->cu_count * 4);
             const uint32_t lpb = std::max<uint32_t>(1, (g.lines + blocks - 1) / blocks);
-            e = hipMemsetAsync(d, 0, sizeof(uint32_t) * nbins, ctx->stream);
-            if (e == hipSuccess)
-                u16_hist_kernel<false><<<(g.lines + lpb - 1) / lpb, 256, 0, ctx->stream>>>(
-                    g, (const uint16_t *)f->data, lpb, d);
-            if (e == hipSuccess) e = hipGetLastError();
+            u16_hist_kernel<<<(g.lines + lpb - 1) / lpb, 256, 0, ctx->stream>>>(
+                g, (const uint16_t *)f->data, lpb, d);
+            e = hipGetLastError();
         }
     }
     if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(uint32_t) * nbins, hipMemcpyDeviceToHost, ctx->stream);
