@@ -745,27 +745,54 @@ namespace {
 
 // convert_to_fp_kernel, convert_to_fp.cpp:11-46: v <= lo -> 0, v >= hi -> 1,
 // else (v - lo) / (hi - lo) in float.
+// A line per wave; each lane converts 4 consecutive voxels per trip, with one
+// vector load and one 16-byte store when the stride is a multiple of 4.
+__device__ __forceinline__ float convert_one(float v, float lo, float hi)
+{
+    return v <= lo ? 0.0f : (v >= hi ? 1.0f : (v - lo) / (hi - lo));
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void convert_kernel(VoxGeom g, const T *__restrict__ in, float *__restrict__ out,
                                                       float lo, float hi)
 {
-    const uint64_t total = uint64_t(g.lines) * g.chunks;
-    const uint64_t nwaves = uint64_t(gridDim.x) * (blockDim.x / kWave);
     const uint32_t lane = threadIdx.x % kWave;
-    for (uint64_t w = uint64_t(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; w < total; w += nwaves) {
-        const uint32_t line = uint32_t(w / g.chunks), n = uint32_t(w % g.chunks) * kWave + lane;
-        if (n >= g.len) continue;
-        const uint64_t o = uint64_t(line) * g.stride + n;
-        const float v = float(in[o]);
-        out[o] = v <= lo ? 0.0f : (v >= hi ? 1.0f : (v - lo) / (hi - lo));
+    const uint32_t nwaves = gridDim.x * (blockDim.x / kWave);
+    for (uint32_t line = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; line < g.lines; line += nwaves) {
+        const uint64_t o = uint64_t(line) * g.stride;
+        const T *r = in + o;
+        float *w = out + o;
+        if (g.stride % 4 != 0) {
+            for (uint32_t n = lane; n < g.len; n += kWave) w[n] = convert_one(float(r[n]), lo, hi);
+            continue;
+        }
+        for (uint32_t n0 = lane * 4; n0 < g.len; n0 += kWave * 4) {
+            if (n0 + 4 <= g.len) {
+                float v[4];
+                if constexpr (sizeof(T) == 2) {
+                    const uint2 q = *reinterpret_cast<const uint2 *>(r + n0);
+                    v[0] = float(q.x & 0xffffu);
+                    v[1] = float(q.x >> 16);
+                    v[2] = float(q.y & 0xffffu);
+                    v[3] = float(q.y >> 16);
+                } else {
+                    const float4 q = *reinterpret_cast<const float4 *>(r + n0);
+                    v[0] = float(q.x);
+                    v[1] = float(q.y);
+                    v[2] = float(q.z);
+                    v[3] = float(q.w);
+                }
+                *reinterpret_cast<float4 *>(w + n0) = make_float4(convert_one(v[0], lo, hi), convert_one(v[1], lo, hi),
+                                                                  convert_one(v[2], lo, hi), convert_one(v[3], lo, hi));
+            } else {
+                for (uint32_t n = n0; n < g.len; ++n) w[n] = convert_one(float(r[n]), lo, hi);
+            }
+        }
     }
 }
 
-// histogram_kernel of find_intensity_range.cpp:22-46: 65536 bins, one per
-// u16 value.  Two 16-bit counters per LDS word; a workgroup counts at most
-// 65535 voxels (lines_per_block lines) so no half-word carries, then adds its
-// non-zero counters to the device histogram.  PACKED = false: device atomics.
-// find_intensity_range's 65536-bin histogram by bin partitions, as
+// histogram_kernel of find_intensity_range.cpp:22-46 (65536 bins, one per u16
+// value) by bin partitions, as
 // hist_part_kernel: workgroup (slice of lines, partition of 32768 bins) counts
 // its slice's voxels whose value (the bin) falls in its partition in LDS and
 // stores the counts as its slice's row of `partial`; hist_sum_kernel adds the
