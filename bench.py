@@ -68,26 +68,91 @@ def log(msg: str) -> None:
     print(f"[bench {time.time() - T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(ph, c, tau, iters: int, size: int) -> dict:
-    """FP32 host transcription (the oracle's FP32 mirror, OpenMP over lines) on a
-    size^3 crop of the same phantom."""
-    import oracle as O
-    from segmentctscans_amd.phantom import crop_box
+def host_cores() -> dict:
+    """The host cores the CPU baseline runs on: every core this process may
+    use (sched_getaffinity), at most OMP_NUM_THREADS when that is set (the
+    GPU's share of a shared host); with nproc and the CPU model."""
+    import subprocess
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    use = min(aff, int(share)) if share.isdigit() and int(share) > 0 else aff
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        nproc = int(subprocess.run(["nproc", "--all"], capture_output=True, text=True).stdout.strip())
+    except (OSError, ValueError):
+        nproc = os.cpu_count() or 0
+    return {"threads": use, "affinity_cores": aff, "nproc_all": nproc, "omp_num_threads": share or None,
+            "cpu_model": model}
 
+
+def _median_time(fn, reps: int) -> float:
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts))
+
+
+def cpu_baseline(ph, c, tau, iters: int, size: int, tv_ph, tv_out, hist_range, fp64, reps: int = 3) -> dict:
+    """The reference CPU path: the oracle's FP32 host transcription of the
+    reference kernels (OpenMP over lines), timed with a monotonic clock
+    around the iteration loop, median of reps runs:
+      - the headline: the 256^3 central crop of the 1426 x 1024^2 phantom,
+        K = 3, iters iterations (voxel-iters/s);
+      - the 64^3 phantom (K = 3, lambda 30, 500 iterations), a full run;
+      - Chambolle TV on the 256^3 central crop of the 512^3 phantom, 20 iterations;
+      - calc_hist at 65536 bins over the GPU's 512^3 TV result (one thread);
+    and the FP64 oracle's time on the agreement box (not a baseline)."""
+    import oracle as O
+    from segmentctscans_amd.phantom import crop_box, make_phantom
+
+    hc = host_cores()
+    nt = hc["threads"]
     m = ph.field.mapping
-    n0 = (m.n_images - size) // 2
-    r0 = c0 = (m.height - size) // 2
-    crop = crop_box(ph, n0, r0, c0, size)
+    crop = crop_box(ph, (m.n_images - size) // 2, (m.height - size) // 2, (m.width - size) // 2, size)
     lay = O.Layout.from_mapping(crop.mapping)
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    cores = min(cores, 16)
-    t = time.perf_counter()
-    O.potts(lay, crop.data, c, 30.0, tau, tau, iters, precision="f32", nthreads=cores)
-    dt = time.perf_counter() - t
     vox = crop.mapping.data_size
-    return {"value": vox * iters / dt, "unit": "voxel-iters/s", "cores": cores, "kind": "port",
-            "sample": f"{size}^3 central crop of the same phantom, K=3, {iters} iterations, "
-                      f"FP32 host mirror of the reference update ({dt:.1f} s)"}
+    dt = _median_time(lambda: O.potts(lay, crop.data, c, 30.0, tau, tau, iters, precision="f32", nthreads=nt), reps)
+    out = {"value": vox * iters / dt, "unit": "voxel-iters/s", "cores": nt, "kind": "port",
+           "sample": f"{size}^3 central crop of the same phantom ({vox} voxels), K=3, {iters} iterations, "
+                     f"FP32 host mirror of the reference update, {nt} threads, median of {reps} "
+                     f"({dt:.2f} s each)",
+           "achieved_GBs_at_84B": vox * iters * 84 / dt / 1e9, **hc}
+    c1 = make_phantom(64, 64, 4.0, 12.0, seed=1)
+    l1 = O.Layout.from_mapping(c1.field.mapping)
+    v1 = c1.field.mapping.data_size
+    c1c = c1.normalized_labels()
+    d1 = _median_time(lambda: O.potts(l1, c1.field.data, c1c, 30.0, tau, tau, 500, precision="f32", nthreads=nt),
+                      reps)
+    out["c1_full_run"] = {"workload": f"64^3 phantom ({v1} voxels), K=3, lambda 30, 500 iterations",
+                          "seconds": d1, "voxel_iters_per_s": v1 * 500 / d1, "threads": nt}
+    tm = tv_ph.field.mapping
+    tcrop = crop_box(tv_ph, (tm.n_images - size) // 2, (tm.height - size) // 2, (tm.width - size) // 2, size)
+    tl = O.Layout.from_mapping(tcrop.mapping)
+    tv_tau = float(np.float32(1.0 / 6.0) * np.float32(0.99))
+    dtv = _median_time(lambda: O.chambolle(tl, tcrop.data, 0.08, tv_tau, 20, precision="f32", nthreads=nt), reps)
+    tvox = tcrop.mapping.data_size
+    out["tv_crop"] = {"workload": f"{size}^3 central crop of the 512^3 TV phantom, lambda 0.08, 20 iterations",
+                      "seconds": dtv, "voxel_iters_per_s": tvox * 20 / dtv, "threads": nt,
+                      "achieved_GBs_at_28B": tvox * 20 * 28 / dtv / 1e9}
+    if tv_out is not None:
+        hl = O.Layout.from_mapping(tm)
+        dh = _median_time(lambda: O.calc_hist(hl, tv_out, hist_range[0], hist_range[1], 65536), reps)
+        out["calc_hist_65536"] = {"workload": "65536 bins over the 512^3 TV result (lambda 0.12), "
+                                              f"{tm.data_size} voxels", "seconds": dh,
+                                  "voxels_per_s": tm.data_size / dh, "threads": 1}
+    if fp64 is not None:
+        out["fp64_oracle"] = fp64
+    return out
 
 
 def adjoint_ratio(S) -> dict:
@@ -127,7 +192,7 @@ TV_BYTES = 28        # R xi 3, R g, W xi 3 (FP32) per voxel and iteration
 TV_UPDATE_BYTES = 20  # final g - lambda div xi: R xi 3, R+W g
 
 
-def tv_workload(S, iters: int, cpu_iters: int) -> dict:
+def tv_workload(S, iters: int, keep: dict | None = None) -> dict:
     """Chambolle TV then the 65536-bin histogram on the 512^3 phantom (cylinder
     radius 230.4, spheres U[4, 20]): for lambda = 0.04, 0.05, ..., 0.12, TV
     from the same input (tau = 0.99 / 6, iters iterations), then calc_hist of
@@ -136,8 +201,9 @@ def tv_workload(S, iters: int, cpu_iters: int) -> dict:
     includes zeroing both xi buffers (24 B per stored element) and the final
     update, so the time of one iteration is (call - call with 0 iterations)
     / iters, and its achieved bandwidth is 28 B per voxel over that time.
-    voxel_iters_per_s is over the whole call."""
-    import oracle as O
+    voxel_iters_per_s is over the whole call.  keep (optional) receives the
+    phantom ("ph"), the input's range ("range") and the result of the last
+    weight ("out", host array) for the CPU baseline."""
     from segmentctscans_amd.hiptime import EventTimeline
     from segmentctscans_amd.phantom import make_phantom
 
@@ -165,6 +231,8 @@ def tv_workload(S, iters: int, cpu_iters: int) -> dict:
         tl.mark()
         d = tl.durations()
         tl.close()
+        if keep is not None and lam == lams[-1]:
+            keep["out"] = g.to_host()
         g.close()
         tv_ms.append(float(d[0]))
         hist_ms.append(float(d[1]))
@@ -194,17 +262,9 @@ def tv_workload(S, iters: int, cpu_iters: int) -> dict:
            "bytes_per_voxel_iter": TV_BYTES}
     traffic, src = committed_traffic("tv_step_kernel", "tv_512")
     out["traffic"], out["traffic_source"] = traffic, src
-    if cpu_iters:
-        lay = O.Layout.from_mapping(m)
-        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        cores = min(cores, 16)
-        t0 = time.perf_counter()
-        O.chambolle(lay, ph.field.data, 0.08, tau, cpu_iters, precision="f32", nthreads=cores)
-        dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": vox * cpu_iters / dt, "unit": "voxel-iters/s", "cores": cores,
-                               "kind": "port",
-                               "sample": f"the same 512^3 input, lambda 0.08, {cpu_iters} iterations, FP32 "
-                                         f"host mirror of the reference's kernels ({dt:.1f} s)"}
+    if keep is not None:
+        keep["ph"] = ph
+        keep["range"] = (mn, mx)
     return out
 
 
@@ -225,33 +285,77 @@ def long_run(S, ctx, g, c, lam, tau, iters: int, gap_every: int, storage: str = 
             "ms_per_iteration": ms / iters, "relative_gap_trace": [float(x) for x in gaps]}
 
 
-def crop_agreement(S, ph, c, lam, tau, iters: int, size: int) -> dict:
-    """Label agreement and ||V - V_fp64||_inf of FP32 / BF16 / FP16 against
-    the oracle's FP64 solve on a size^3 box cut from the centre of the
-    workload, solved on its own (box domain, Neumann edges)."""
-    import oracle as O
-    from segmentctscans_amd.phantom import crop_box
-    m = ph.field.mapping
-    box = crop_box(ph, (m.n_images - size) // 2, (m.height - size) // 2, (m.width - size) // 2, size)
-    lay = O.Layout.from_mapping(box.mapping)
-    t = time.perf_counter()
-    r64 = O.potts(lay, box.data, c, lam, tau, tau, iters, precision="f64", want_state=True,
-                  nthreads=min(16, os.cpu_count() or 1))
-    log(f"FP64 oracle on the {size}^3 box: {time.perf_counter() - t:.1f} s")
+class Fp64Box:
+    """The oracle's FP64 solve of a size^3 box cut from the centre of the
+    workload (box domain, Neumann edges), run on a host thread while the GPU
+    legs run (the oracle releases the GIL)."""
+
+    def __init__(self, ph, c, lam, tau, iters: int, size: int, threads: int):
+        import threading
+        from segmentctscans_amd.phantom import crop_box
+        m = ph.field.mapping
+        self.box = crop_box(ph, (m.n_images - size) // 2, (m.height - size) // 2, (m.width - size) // 2, size)
+        self.args = (c, lam, tau, iters)
+        self.size, self.threads, self.iters = size, threads, iters
+        self.result, self.seconds, self.error = None, None, None
+        self.thread = threading.Thread(target=self._run, daemon=True)
+        self.thread.start()
+
+    def _run(self):
+        import oracle as O
+        try:
+            c, lam, tau, iters = self.args
+            lay = O.Layout.from_mapping(self.box.mapping)
+            t = time.perf_counter()
+            self.result = O.potts(lay, self.box.data, c, lam, tau, tau, iters, precision="f64", want_state=True,
+                                  nthreads=self.threads)
+            self.seconds = time.perf_counter() - t
+            log(f"FP64 oracle on the {self.size}^3 box: {self.seconds:.1f} s")
+        except Exception as e:   # reported in the JSON line
+            self.error = repr(e)
+
+    def join(self):
+        self.thread.join()
+        if self.error:
+            raise RuntimeError(self.error)
+        return self.result
+
+    def timing(self) -> dict:
+        vox = self.box.mapping.data_size
+        return {"workload": f"{self.size}^3 box, K=3, {self.iters} iterations, FP64", "seconds": self.seconds,
+                "voxel_iters_per_s": vox * self.iters / self.seconds, "threads": self.threads}
+
+
+def agreement_gpu(S, job: Fp64Box, c, lam, tau) -> dict:
+    """V and labels of the agreement box after job.iters iterations on the GPU
+    with FP32, BF16 and FP16 storage (in-mask voxels only)."""
+    box, iters = job.box, job.iters
     valid = box.mapping.valid_mask()
     ctx = S.make_context(0)
-    out = {"box": size, "iterations": iters}
+    got = {}
     for st in ("fp32", "bf16", "fp16"):
         gb = S.upload(ctx, box)
         sol = S.PottsSolver(ctx, gb, c, lam, tau, tau, storage=st)
         sol.iterate(iters)
         V, _ = sol.state()
-        res = sol.labels().to_host()
+        got[st] = (V[:, valid], sol.labels().to_host()[valid])
         sol.close()
         gb.close()
-        out[st] = {"label_agreement": float((res[valid] == r64["res"][valid]).mean()),
-                   "v_inf": float(np.abs(V[:, valid] - r64["V"][:, valid]).max())}
-        log(f"{size}^3 box, {st}: agreement {out[st]['label_agreement']:.6f}, ||V - V64||inf {out[st]['v_inf']:.3e}")
+    log(f"agreement box solved on the GPU ({', '.join(got)})")
+    return got
+
+
+def crop_agreement(job: Fp64Box, got: dict) -> dict:
+    """Label agreement and ||V - V_fp64||_inf of each storage type against the
+    oracle's FP64 solve of the same box."""
+    valid = job.box.mapping.valid_mask()
+    r64 = job.join()
+    out = {"box": job.size, "iterations": job.iters, "voxels": int(valid.sum()), "fp64_seconds": job.seconds}
+    for st, (V, res) in got.items():
+        out[st] = {"label_agreement": float((res == r64["res"][valid]).mean()),
+                   "v_inf": float(np.abs(V - r64["V"][:, valid]).max())}
+        log(f"{job.size}^3 box, {st}: agreement {out[st]['label_agreement']:.6f}, "
+            f"||V - V64||inf {out[st]['v_inf']:.3e}")
     return out
 
 
@@ -330,39 +434,24 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
-def batch_workload(args) -> int:
+def batch_workload(S, seed: int, iters: int, autotune: bool = True) -> dict:
     """576 solves: 64 sub-volumes with edges U[48, 192] per axis (cylinder or
     box, K ~ U{3, 4, 5}, extra phases evenly spaced in [0.030, 0.060] raw)
-    x lambda in {20, 25, ..., 60}, 300 iterations each: one batched call
+    x lambda in {20, 25, ..., 60}, iters iterations each: one batched call
     (one captured hipGraph launch for the iterations) against the same solves
-    looped through potts_min_partition."""
-    import segmentctscans_amd as S
-    from segmentctscans_amd.phantom import make_phantom
+    looped through potts_min_partition.  The batched call runs twice: with
+    the default Xi tile, then with the autotuned one (its trial iterations
+    lie outside the event-timed iterations)."""
+    from segmentctscans_amd.phantom import C4_LAMBDAS, c4_volumes
 
-    rng = np.random.default_rng(args.seed)
     tau = float(np.float32(1.0 / np.sqrt(12.0)))
-    iters = args.batch_iters
     ctx = S.make_context(0)
-    vols = []
-    for i in range(args.batch_volumes):
-        n, h, w = (int(x) for x in rng.integers(48, 193, 3))
-        box = bool(rng.integers(0, 2))
-        K = int(rng.integers(3, 6))
-        r = 4.0 * max(h, w) if box else 0.45 * min(h, w)
-        ph = make_phantom(n, max(h, w), 4.0, 12.0, seed=1000 + i, height=h, width=w,
-                          xc=(w - 1) / 2, yc=(h - 1) / 2, r=r)
-        lo, hi = ph.field.lo, ph.field.hi
-        extra = np.linspace(0.030, 0.060, K - 1)[1:-1] if K > 3 else np.zeros(0)
-        c = np.sort(np.concatenate([ph.normalized_labels(), (extra - lo) / (hi - lo)])).astype(np.float32)
-        c = c[:K] if c.size >= K else np.linspace(0.1, 0.9, K).astype(np.float32)
-        vols.append((S.upload(ctx, ph.field), c, n, h, w, box))
-    lams = [20.0 + 5.0 * k for k in range(9)]
-    probs = [(g, c, lam) for (g, c, *_) in vols for lam in lams]
+    vols = c4_volumes(seed=seed)
+    dev = [S.upload(ctx, v.field) for v in vols]
+    probs = [(g, v.c, lam) for v, g in zip(vols, dev) for lam in C4_LAMBDAS]
     vox = sum(g.mapping.data_size for (g, _, _) in probs)
-    log(f"{len(probs)} solves, {vox} voxels in all, {iters} iterations")
+    log(f"batch: {len(probs)} solves, {vox} voxels in all, {iters} iterations")
     S.potts_batch(ctx, probs[:4], tau, tau, 2)   # warm-up (code objects)
-    # the default tile, then the autotuned one (its trial iterations are
-    # outside the timed region; the results are the same)
     t = time.perf_counter()
     outs0, ms_default = S.potts_batch(ctx, probs, tau, tau, iters, graph=True)
     wall_default = time.perf_counter() - t
@@ -370,10 +459,11 @@ def batch_workload(args) -> int:
         o.close()
     log(f"batched, default tile: iterations {ms_default:.1f} ms (events), call {wall_default:.2f} s")
     t = time.perf_counter()
-    outs, ms = S.potts_batch(ctx, probs, tau, tau, iters, graph=True, autotune=not args.no_autotune)
+    outs, ms = S.potts_batch(ctx, probs, tau, tau, iters, graph=True, autotune=autotune)
     wall_batch = time.perf_counter() - t
     tiles = S.potts_batch_tiles(ctx)
-    log(f"batched, autotuned tiles {tiles}: iterations {ms:.1f} ms (events), call {wall_batch:.2f} s")
+    log(f"batched, {'autotuned' if autotune else 'default'} tiles {tiles}: iterations {ms:.1f} ms (events), "
+        f"call {wall_batch:.2f} s")
     t = time.perf_counter()
     refs = [S.potts_min_partition(ctx, g, c, lam, tau, tau, iters) for (g, c, lam) in probs]
     ctx.synchronize()
@@ -384,18 +474,19 @@ def batch_workload(args) -> int:
         agree += int(np.array_equal(ref.to_host()[m], o.to_host()[m]))
         ref.close()
         o.close()
+    for g in dev:
+        g.close()
     log(f"looped single solves: {wall_loop:.2f} s; identical labels {agree}/{len(probs)}")
-    print(json.dumps({
-        "metric": "batched Potts voxel-iters/s", "value": vox * iters / (ms / 1e3), "unit": "voxel-iters/s",
-        "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "potts_batch_576", "volumes": args.batch_volumes, "lambdas": lams,
-                   "iterations": iters, "solves": len(probs), "voxels": vox,
-                   "shapes": [[n, h, w, "box" if box else "cylinder", int(c.size)] for (_, c, n, h, w, box) in vols]},
-        "iterations_ms": ms, "iterations_ms_default_tile": ms_default,
-        "xi_tile_waves": {str(k): v for k, v in tiles.items()},
-        "batched_call_s": wall_batch, "looped_calls_s": wall_loop,
-        "identical_labels": agree}), flush=True)
-    return 0
+    return {"workload": "64 sub-volumes (edges U[48,192], box or cylinder, K~U{3,4,5}) x 9 lambdas, "
+                        f"{iters} iterations, FP32",
+            "solves": len(probs), "voxels": vox, "iterations": iters,
+            "voxel_iters_per_s": vox * iters / (ms / 1e3), "iterations_ms": ms,
+            "iteration_ms": ms / iters, "iterations_ms_default_tile": ms_default,
+            "xi_tile_waves": {str(k): v for k, v in tiles.items()},
+            "batched_call_s": wall_batch, "looped_calls_s": wall_loop,
+            "speedup_vs_looped": wall_loop / wall_batch,
+            "identical_labels": agree,
+            "shapes": [[v.n, v.height, v.width, "box" if v.box else "cylinder", int(v.c.size)] for v in vols]}
 
 
 def main() -> int:
@@ -410,7 +501,7 @@ def main() -> int:
     ap.add_argument("--radius", type=float, default=460.0)
     ap.add_argument("--rmax", type=float, default=40.0)
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--cpu-iters", type=int, default=100)
+    ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--cpu-size", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-bf16", action="store_true", help="skip the BF16-storage comparison run")
@@ -420,19 +511,19 @@ def main() -> int:
     ap.add_argument("--no-long", action="store_true", help="skip the 1000-iteration solve with the FP64 gap")
     ap.add_argument("--long-iters", type=int, default=1000)
     ap.add_argument("--gap-every", type=int, default=50)
-    ap.add_argument("--agreement", action="store_true",
-                    help="also solve a 256^3 central box for --long-iters iterations in FP32 / BF16 / FP16 "
-                         "and compare with the oracle's FP64 solve (minutes of host time)")
+    ap.add_argument("--no-agreement", action="store_true",
+                    help="skip solving a 256^3 central box for --long-iters iterations in FP32 / BF16 / FP16 "
+                         "and comparing with the oracle's FP64 solve (minutes of host time, overlapped with "
+                         "the GPU legs)")
     ap.add_argument("--tv-iters", type=int, default=200)
-    ap.add_argument("--tv-cpu-iters", type=int, default=10)
     ap.add_argument("--tv-only", action="store_true",
                     help="run only the Chambolle TV + histogram workload (512^3) and print its JSON")
     ap.add_argument("--no-reductions", action="store_true", help="skip the reductions workload")
     ap.add_argument("--reductions-only", action="store_true",
                     help="run only the reductions on the 512^3 phantom and print their JSON")
     ap.add_argument("--batch", action="store_true",
-                    help="run the batched workload (576 solves) instead of the single-volume one")
-    ap.add_argument("--batch-volumes", type=int, default=64)
+                    help="run only the batched workload (576 solves) and print its JSON")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--batch-iters", type=int, default=300)
     ap.add_argument("--no-autotune", action="store_true", help="batched workload without the tile autotuner")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -440,7 +531,12 @@ def main() -> int:
                          "kernel), to calibrate the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.batch:
-        return batch_workload(args)
+        import segmentctscans_amd as S
+        r = batch_workload(S, args.seed, args.batch_iters, not args.no_autotune)
+        print(json.dumps({"metric": "batched Potts voxel-iters/s", "value": r["voxel_iters_per_s"],
+                          "unit": "voxel-iters/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+                          "data": "synthetic", "config": {"workload": "potts_batch_576"}, **r}), flush=True)
+        return 0
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
     if args.reductions_only:
@@ -451,7 +547,7 @@ def main() -> int:
         return 0
     if args.tv_only:
         import segmentctscans_amd as S
-        r = tv_workload(S, args.tv_iters, 0 if args.no_cpu_baseline else args.tv_cpu_iters)
+        r = tv_workload(S, args.tv_iters)
         log(f"TV: {r['ms_per_iteration']:.3f} ms/iteration, {r['frac']:.3f} of HBM peak")
         print(json.dumps(r, default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)
         return 0
@@ -471,6 +567,8 @@ def main() -> int:
     ph = make_phantom(args.n, args.side, 4.0, args.rmax, seed=args.seed, r=args.radius)
     c = ph.normalized_labels()
     vox = ph.field.mapping.data_size
+    # the FP64 oracle's solve of the agreement box runs on host threads while the GPU legs run
+    fp64 = None if args.no_agreement else Fp64Box(ph, c, lam, tau, args.long_iters, 256, host_cores()["threads"])
     log(f"phantom done: {vox} in-mask voxels, {ph.n_spheres} spheres, porosity {ph.porosity:.3f}")
 
     ctx = S.make_context(0)
@@ -573,15 +671,15 @@ def main() -> int:
                 f"{lr['ms_total'] / 1e3:.2f} s, relative gap {lr['relative_gap_trace'][0]:.3e} -> "
                 f"{lr['relative_gap_trace'][-1]:.3e}")
     g.close()
-    if args.agreement:
-        result["agreement"] = crop_agreement(S, ph, c, lam, tau, args.long_iters, 256)
+    box_states = agreement_gpu(S, fp64, c, lam, tau) if fp64 is not None else None
     if not args.no_adjoint:
         result["adjoint"] = adjoint_ratio(S)
         log(f"adjoint: forward {result['adjoint']['forward_ms']:.2f} ms, reverse "
             f"{result['adjoint']['reverse_ms']:.2f} ms, ratio {result['adjoint']['ratio']:.2f}; every state "
             f"stored: ratio {result['adjoint']['every_state_stored']['ratio']:.2f}")
+    tv_keep = {}
     if not args.no_tv:
-        result["tv"] = tv_workload(S, args.tv_iters, 0 if args.no_cpu_baseline else args.tv_cpu_iters)
+        result["tv"] = tv_workload(S, args.tv_iters, tv_keep)
         log(f"TV: {result['tv']['ms_per_iteration']:.3f} ms/iteration, "
             f"{result['tv']['voxel_iters_per_s']:.3e} voxel-iters/s, {result['tv']['frac']:.3f} of HBM peak; "
             f"65536-bin histogram {np.median(result['tv']['hist_ms']):.3f} ms")
@@ -589,10 +687,20 @@ def main() -> int:
         result["reductions"] = reductions_workload(S)
         log("reductions: " + ", ".join(f"{k} {v['ms']:.3f} ms ({v['frac']:.2f} of peak)"
                                        for k, v in result["reductions"].items() if isinstance(v, dict)))
+    if not args.no_batch:
+        result["batch"] = batch_workload(S, args.seed, args.batch_iters, not args.no_autotune)
+    if fp64 is not None:
+        result["agreement"] = crop_agreement(fp64, box_states)   # waits for the FP64 solve
+        box_states = None
     if not args.no_cpu_baseline:
         log("CPU baseline")
-        result["cpu_baseline"] = cpu_baseline(ph, c, tau, args.cpu_iters, args.cpu_size)
-        log(f"CPU baseline {result['cpu_baseline']['value']:.3e} voxel-iters/s")
+        if "ph" not in tv_keep:
+            tv_keep["ph"] = make_phantom(512, 512, 4.0, 20.0, seed=7, r=230.4)
+        result["cpu_baseline"] = cpu_baseline(ph, c, tau, args.cpu_iters, args.cpu_size, tv_keep["ph"],
+                                              tv_keep.get("out"), tv_keep.get("range"),
+                                              fp64.timing() if fp64 is not None else None)
+        log(f"CPU baseline {result['cpu_baseline']['value']:.3e} voxel-iters/s on "
+            f"{result['cpu_baseline']['cores']} threads ({result['cpu_baseline']['cpu_model']})")
     print(json.dumps(result, default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)
     return 0
 

@@ -91,3 +91,48 @@ def crop_box(ph: Phantom, n0: int, r0: int, c0: int, size: int) -> Field:
         blk = src[line0 * m.nstride:(line0 + size) * m.nstride].reshape(size, m.nstride)[:, n0:n0 + size]
         out[col * size * box.nstride:(col + 1) * size * box.nstride].reshape(size, box.nstride)[:, :size] = blk
     return Field(box, out, lo=ph.field.lo, hi=ph.field.hi)
+
+
+# The batched workload: 64 sub-volumes x 9 lambdas, 300 iterations.
+C4_LAMBDAS = tuple(20.0 + 5.0 * k for k in range(9))
+
+
+@dataclass
+class SubVolume:
+    field: Field
+    c: np.ndarray           # K normalised label intensities, ascending
+    n: int
+    height: int
+    width: int
+    box: bool               # box domain (True) or cylinder of radius 0.45 min(h, w)
+
+
+def c4_volumes(seed: int = 3, count: int = 64, workers: int = 8) -> list[SubVolume]:
+    """The sub-volumes of the batched workload: edges n, height, width ~
+    U[48, 192] (seeded), a box or a cylinder domain with p = 1/2, spheres
+    U[4, 12], K ~ U{3, 4, 5}.  Labels are the three phase intensities plus,
+    for K > 3, extra phases evenly spaced in [0.030, 0.060] (raw units),
+    all normalised and sorted.  The phantoms are generated on `workers`
+    host threads (sphere placement is serial per phantom)."""
+    import concurrent.futures as cf
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        n, h, w = (int(x) for x in rng.integers(48, 193, 3))
+        box = bool(rng.integers(0, 2))
+        K = int(rng.integers(3, 6))
+        specs.append((i, n, h, w, box, K))
+
+    def one(spec):
+        i, n, h, w, box, K = spec
+        r = 4.0 * max(h, w) if box else 0.45 * min(h, w)
+        ph = make_phantom(n, max(h, w), 4.0, 12.0, seed=1000 + i, nthreads=1, height=h, width=w,
+                          xc=(w - 1) / 2, yc=(h - 1) / 2, r=r)
+        lo, hi = ph.field.lo, ph.field.hi
+        extra = np.linspace(0.030, 0.060, K - 1)[1:-1] if K > 3 else np.zeros(0)
+        c = np.sort(np.concatenate([ph.normalized_labels(), (extra - lo) / (hi - lo)])).astype(np.float32)
+        c = c[:K] if c.size >= K else np.linspace(0.1, 0.9, K).astype(np.float32)
+        return SubVolume(ph.field, c, n, h, w, box)
+
+    with cf.ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        return list(ex.map(one, specs))

@@ -1,0 +1,84 @@
+"""The batched workload at its full size: 64 sub-volumes (edges U[48, 192] per
+axis, box or cylinder domain, K ~ U{3, 4, 5}) x lambda in {20, 25, ..., 60}
+= 576 FP32 solves of 300 iterations, run as one batched call (the iterations
+captured into one hipGraph launch, Xi tiles chosen by the autotuner).
+
+Checks:
+  - every solve's labels equal those of its own potts_min_partition call
+    (potts-min-partition.cpp:540-679 per problem, as potts-model.cpp:164
+    calls it);
+  - five sampled solves (the smallest volume of each K, at the lowest and
+    highest lambda for K = 3 and 5, the middle lambda for K = 4) equal the
+    oracle's FP32 mirror of the reference update bit for bit."""
+import numpy as np
+import pytest
+
+import oracle as O
+import segmentctscans_amd as S
+from segmentctscans_amd.phantom import C4_LAMBDAS, c4_volumes
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
+
+TAU = float(np.float32(1.0 / np.sqrt(12.0)))
+ITERS = 300
+
+
+@pytest.fixture(scope="module")
+def c4(ctx):
+    vols = c4_volumes(seed=3)
+    dev = [S.upload(ctx, v.field) for v in vols]
+    probs = [(g, v.c, lam) for v, g in zip(vols, dev) for lam in C4_LAMBDAS]
+    print(f"C4: {len(probs)} solves, {sum(g.mapping.data_size for (g, _, _) in probs)} voxels", flush=True)
+    outs, ms = S.potts_batch(ctx, probs, TAU, TAU, ITERS, graph=True, autotune=True)
+    labels = [o.to_host() for o in outs]
+    for o in outs:
+        o.close()
+    print(f"C4 batched iterations: {ms:.1f} ms, tiles {S.potts_batch_tiles(ctx)}", flush=True)
+    yield vols, probs, labels
+    for g in dev:
+        g.close()
+
+
+def test_c4_shape(c4):
+    vols, probs, _ = c4
+    assert len(vols) == 64 and len(probs) == 576
+    for v in vols:
+        assert 48 <= min(v.n, v.height, v.width) and max(v.n, v.height, v.width) <= 192
+        assert v.c.size in (3, 4, 5) and np.all(np.diff(v.c) > 0)
+    assert {v.c.size for v in vols} == {3, 4, 5}
+    assert {v.box for v in vols} == {True, False}
+
+
+def test_c4_every_solve_equals_its_single_solve(ctx, c4):
+    _, probs, labels = c4
+    bad = []
+    for i, ((g, c, lam), got) in enumerate(zip(probs, labels)):
+        ref = S.potts_min_partition(ctx, g, c, lam, TAU, TAU, ITERS)
+        want = ref.to_host()
+        ref.close()
+        m = g.mapping.valid_mask()
+        if not np.array_equal(got[m], want[m]):
+            bad.append((i, g.mapping.n_images, c.size, lam))
+    assert not bad, f"{len(bad)} of {len(probs)} batched solves differ from their single solves: {bad[:5]}"
+
+
+def _samples(vols):
+    """(volume index, lambda) of the sampled solves."""
+    out = []
+    for K, lams in ((3, (C4_LAMBDAS[0], C4_LAMBDAS[-1])), (4, (C4_LAMBDAS[4],)), (5, (C4_LAMBDAS[0], C4_LAMBDAS[-1]))):
+        i = min((v.field.mapping.data_size, i) for i, v in enumerate(vols) if v.c.size == K)[1]
+        out += [(i, lam) for lam in lams]
+    return out
+
+
+def test_c4_samples_bitwise_vs_fp32_mirror(c4):
+    vols, probs, labels = c4
+    for i, lam in _samples(vols):
+        v = vols[i]
+        lay = O.Layout.from_mapping(v.field.mapping)
+        want = O.potts(lay, v.field.data, v.c, lam, TAU, TAU, ITERS, precision="f32")["res"]
+        got = labels[i * len(C4_LAMBDAS) + C4_LAMBDAS.index(lam)]
+        m = v.field.mapping.valid_mask()
+        assert np.array_equal(got[m].view(np.uint32), want[m].view(np.uint32)), (i, v.c.size, lam)
+        print(f"volume {i} ({v.n}x{v.height}x{v.width}, {'box' if v.box else 'cylinder'}, K={v.c.size}), "
+              f"lambda {lam}: labels == FP32 mirror")
