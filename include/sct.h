@@ -174,6 +174,19 @@ int sct_min_max(sct_ctx *ctx, const sct_field *u, float *mn, float *mx);
 int sct_calc_hist(sct_ctx *ctx, const sct_field *u, float range_start, float range_end,
                   uint32_t nbins, float delta, uint64_t seed, uint32_t *counts,
                   double *density);
+/* sct_calc_hist without allocating.  No counterpart in the reference: for
+ * callers that cannot allocate device or pinned memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as sct_calc_hist, for nbins up to
+ * SCT_NOALLOC_MAX_BINS (SCT_ERR_INVALID beyond).  Its device temporaries
+ * live in a fixed module-scope buffer of 17 MiB per GPU, held until the
+ * process ends; it calls no hipMalloc, hipFree or hipHostMalloc and copies
+ * the counts straight into `counts`.  Every call on one GPU, from any context
+ * or thread, shares that buffer: calls that overlap on one GPU race.
+ * Synchronises. */
+#define SCT_NOALLOC_MAX_BINS 262144u
+int sct_calc_hist_noalloc(sct_ctx *ctx, const sct_field *u, float range_start, float range_end,
+                          uint32_t nbins, float delta, uint64_t seed, uint32_t *counts, double *density);
 /* calc_hist.h:8-9: range from min_max (calc_hist.cpp:83-92). */
 int sct_calc_hist_auto(sct_ctx *ctx, const sct_field *u, uint32_t nbins, float delta,
                        uint64_t seed, uint32_t *counts, double *density,

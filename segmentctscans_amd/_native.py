@@ -115,6 +115,8 @@ SIGNATURES = {
     "sct_min_max": (C.c_int, [C.c_void_p, C.c_void_p, f32p, f32p]),
     "sct_calc_hist": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_uint32, C.c_float,
                                 C.c_uint64, u32p, f64p]),
+    "sct_calc_hist_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_uint32, C.c_float,
+                                        C.c_uint64, u32p, f64p]),
     "sct_calc_hist_auto": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_uint64, u32p,
                                      f64p, f32p, f32p]),
     "sct_grad_magn": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),

@@ -288,6 +288,11 @@ extern "C" int sct_ctx_create(int device, sct_ctx **out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = uint32_t(prop.multiProcessorCount);
     (void)hipDeviceGetDefaultMemPool(&c->pool, device);
+    if (sct_noalloc_scratch_setup(c) != SCT_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return SCT_ERR_HIP;
+    }
     *out = c;
     return SCT_OK;
 }

@@ -11,6 +11,8 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 namespace {
 
@@ -526,14 +528,91 @@ extern "C" int sct_grad_min_max(sct_ctx *ctx, const sct_field *v, float *mn, flo
     return run_minmax(ctx, v, true, mn, mx);
 }
 
-// calc_hist(ctx, in, range_start, range_end, nbins, delta), calc_hist.cpp:94-138.
-extern "C" int sct_calc_hist(sct_ctx *ctx, const sct_field *f, float rs, float re, uint32_t nbins, float delta,
-                             uint64_t seed, uint32_t *counts, double *density)
+namespace {
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(uint32_t *__restrict__ p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// How calc_hist counts nbins bins of g: mode 0 (LDS counters, device atomic
+// flush), 3 (bin partitions, per-slice partial rows, then a summing kernel)
+// or 2 (device atomics).  Mode 3 takes one workgroup per CU over all
+// partitions (the LDS holds one), fewer when the partial rows must fit in
+// max_partial words.
+struct HistPlan {
+    int mode = 0;
+    uint32_t parts = 1, used = 0;   // partitions, slices (rows of partial)
+    uint64_t partial_words = 0;
+};
+
+HistPlan hist_plan(const sct_ctx *ctx, const VoxGeom &g, HistArgs &a, uint64_t max_partial)
+{
+    HistPlan p;
+    p.parts = (a.nbins + kPartBins - 1) / kPartBins;
+    p.mode = a.nbins <= kSmallBins ? 0 : (p.parts <= kMaxParts ? 3 : 2);
+    if (!g.lines || !g.len) return p;
+    if (p.mode == 3) {
+        uint64_t slices = std::max<uint32_t>(1, std::min<uint32_t>(g.lines, ctx->cu_count / p.parts));
+        slices = std::max<uint64_t>(1, std::min<uint64_t>(slices, max_partial / a.nbins));
+        a.lines_per_block = uint32_t((g.lines + slices - 1) / slices);
+        p.used = (g.lines + a.lines_per_block - 1) / a.lines_per_block;
+        p.partial_words = uint64_t(p.used) * a.nbins;
+    } else {
+        const uint32_t blocks = std::max<uint32_t>(1, ctx->cu_count * 4);
+        a.lines_per_block = std::max<uint32_t>(1, (g.lines + blocks - 1) / blocks);
+    }
+    return p;
+}
+
+// The kernels of calc_hist on the context's stream: counts d[nbins] (cleared
+// here by a kernel), partial rows in partial (mode 3).
+hipError_t hist_passes(sct_ctx *ctx, const sct_field *f, const VoxGeom &g, const HistArgs &a, const HistPlan &p,
+                       uint32_t *d, uint32_t *partial)
+{
+    const bool dither = a.delta != 0.0f;
+    const uint32_t nbins = a.nbins;
+    if (p.mode != 3 || !g.lines || !g.len) {
+        fill_zero_kernel<<<(nbins + 255) / 256, 256, 0, ctx->stream>>>(d, nbins);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess || !g.lines || !g.len) return e;
+    }
+    if (p.mode == 3) {
+        const size_t lds = sizeof(uint32_t) * std::min(nbins, kPartBins);
+        auto launch = [&](auto kernel) -> hipError_t {
+            hipError_t ea = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+            if (ea != hipSuccess) return ea;
+            kernel<<<p.used * p.parts, kPartThreads, lds, ctx->stream>>>(g, (const float *)f->data, a, p.parts, partial);
+            return hipGetLastError();
+        };
+        hipError_t e = dither ? launch(hist_part_kernel<true>) : launch(hist_part_kernel<false>);
+        if (e != hipSuccess) return e;
+        hist_sum_kernel<<<(nbins + 255) / 256, 256, 0, ctx->stream>>>(partial, p.used, nbins, d);
+        return hipGetLastError();
+    }
+    const size_t lds = p.mode == 0 ? sizeof(uint32_t) * nbins : 0;
+    const uint32_t grid = std::max<uint32_t>(1, (g.lines + a.lines_per_block - 1) / a.lines_per_block);
+    if (p.mode == 0) {
+        if (dither) hist_kernel<0, true><<<grid, 256, lds, ctx->stream>>>(g, (const float *)f->data, a, d);
+        else hist_kernel<0, false><<<grid, 256, lds, ctx->stream>>>(g, (const float *)f->data, a, d);
+    } else {
+        if (dither) hist_kernel<2, true><<<grid, 256, 0, ctx->stream>>>(g, (const float *)f->data, a, d);
+        else hist_kernel<2, false><<<grid, 256, 0, ctx->stream>>>(g, (const float *)f->data, a, d);
+    }
+    return hipGetLastError();
+}
+
+int hist_check(sct_ctx *ctx, const sct_field *f, uint32_t nbins, const uint32_t *counts)
 {
     if (!ctx || !f || !counts) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
     if (f->dtype != SCT_F32) return sct_fail(ctx, SCT_ERR_INVALID, "float field required");
     if (nbins == 0) return sct_fail(ctx, SCT_ERR_INVALID, "nbins must be positive");
-    const VoxGeom g = vox_geom(f);
+    return SCT_OK;
+}
+
+HistArgs hist_args(float rs, float re, uint32_t nbins, float delta, uint64_t seed)
+{
     HistArgs a;
     a.rs = rs;
     a.re = re;
@@ -541,65 +620,107 @@ extern "C" int sct_calc_hist(sct_ctx *ctx, const sct_field *f, float rs, float r
     a.nbins = nbins;
     a.k0 = uint32_t(seed);
     a.k1 = uint32_t(seed >> 32);
-    const bool dither = delta != 0.0f;
-    const uint32_t parts = (nbins + kPartBins - 1) / kPartBins;
-    int mode = nbins <= kSmallBins ? 0 : (parts <= kMaxParts ? 3 : 2);
+    a.lines_per_block = 1;
+    return a;
+}
+
+// density = count / (data_size * bin_width), in FP64 (calc_hist.cpp:131-137).
+void hist_density(const sct_field *f, float rs, float re, uint32_t nbins, const uint32_t *counts, double *density)
+{
+    if (!density) return;
+    const double bin_width = (double(re) - double(rs)) / nbins;
+    const double scale = 1.0 / (double(f->geo.data_size) * bin_width);
+    for (uint32_t i = 0; i < nbins; ++i) density[i] = counts[i] * scale;
+}
+
+}  // namespace
+
+// calc_hist(ctx, in, range_start, range_end, nbins, delta), calc_hist.cpp:94-138.
+extern "C" int sct_calc_hist(sct_ctx *ctx, const sct_field *f, float rs, float re, uint32_t nbins, float delta,
+                             uint64_t seed, uint32_t *counts, double *density)
+{
+    int rc = hist_check(ctx, f, nbins, counts);
+    if (rc) return rc;
+    const VoxGeom g = vox_geom(f);
+    HistArgs a = hist_args(rs, re, nbins, delta, seed);
+    const HistPlan p = hist_plan(ctx, g, a, ~uint64_t(0));
     // counts d, then (partitioned mode) a row of partial counts per slice of
     // lines, in the context's scratch
     const size_t dbytes = (sizeof(uint32_t) * size_t(nbins) + 255) & ~size_t(255);
-    uint32_t slices = 0, used = 0;
-    if (mode == 3 && g.lines && g.len) {
-        // slices: one workgroup per CU over all partitions (the LDS holds one)
-        slices = std::max<uint32_t>(1, std::min<uint32_t>(g.lines, ctx->cu_count / parts));
-        a.lines_per_block = (g.lines + slices - 1) / slices;
-        used = (g.lines + a.lines_per_block - 1) / a.lines_per_block;
-    }
     void *mem = nullptr, *host = nullptr;
-    int rc = sct_ctx_scratch(ctx, dbytes + sizeof(uint32_t) * uint64_t(used) * nbins, &mem);
+    rc = sct_ctx_scratch(ctx, dbytes + sizeof(uint32_t) * p.partial_words, &mem);
     if (!rc) rc = sct_ctx_host_scratch(ctx, sizeof(uint32_t) * size_t(nbins), &host);
     if (rc) return rc;
-    uint32_t *d = (uint32_t *)mem, *partial = used ? (uint32_t *)((char *)mem + dbytes) : nullptr;
-    hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t) * nbins, ctx->stream);
-    if (mode == 3 && g.lines && g.len) {
-        const size_t lds = sizeof(uint32_t) * std::min(nbins, kPartBins);
-        auto launch = [&](auto kernel) -> hipError_t {
-            hipError_t ea = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-            if (ea != hipSuccess) return ea;
-            kernel<<<used * parts, kPartThreads, lds, ctx->stream>>>(g, (const float *)f->data, a, parts, partial);
-            return hipGetLastError();
-        };
-        if (e == hipSuccess) e = dither ? launch(hist_part_kernel<true>) : launch(hist_part_kernel<false>);
-        if (e == hipSuccess) {
-            hist_sum_kernel<<<(nbins + 255) / 256, 256, 0, ctx->stream>>>(partial, used, nbins, d);
-            e = hipGetLastError();
-        }
-    } else if (g.lines && g.len) {
-        const size_t lds = mode == 0 ? sizeof(uint32_t) * nbins : 0;
-        const uint32_t blocks = std::max<uint32_t>(1, ctx->cu_count * 4);
-        a.lines_per_block = std::max<uint32_t>(1, (g.lines + blocks - 1) / blocks);
-        const uint32_t grid = std::max<uint32_t>(1, (g.lines + a.lines_per_block - 1) / a.lines_per_block);
-        if (e == hipSuccess) {
-            if (mode == 0) {
-                if (dither) hist_kernel<0, true><<<grid, 256, lds, ctx->stream>>>(g, (const float *)f->data, a, d);
-                else hist_kernel<0, false><<<grid, 256, lds, ctx->stream>>>(g, (const float *)f->data, a, d);
-            } else {
-                if (dither) hist_kernel<2, true><<<grid, 256, 0, ctx->stream>>>(g, (const float *)f->data, a, d);
-                else hist_kernel<2, false><<<grid, 256, 0, ctx->stream>>>(g, (const float *)f->data, a, d);
-            }
-            e = hipGetLastError();
-        }
-    }
+    uint32_t *d = (uint32_t *)mem, *partial = p.used ? (uint32_t *)((char *)mem + dbytes) : nullptr;
+    hipError_t e = hist_passes(ctx, f, g, a, p, d, partial);
     if (e == hipSuccess) e = hipMemcpyAsync(host, d, sizeof(uint32_t) * nbins, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) (void)hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return sct_hip_fail(ctx, e, "calc_hist");
     std::memcpy(counts, host, sizeof(uint32_t) * nbins);
-    if (density) {
-        // calc_hist.cpp:131-137
-        const double bin_width = (double(re) - double(rs)) / nbins;
-        const double scale = 1.0 / (double(f->geo.data_size) * bin_width);
-        for (uint32_t i = 0; i < nbins; ++i) density[i] = counts[i] * scale;
+    hist_density(f, rs, re, nbins, counts, density);
+    return SCT_OK;
+}
+
+// ---- calc_hist with no allocation ------------------------------------------
+// The device temporaries of sct_calc_hist_noalloc live in one module-scope
+// array per GPU, `scratch`: the counts (SCT_NOALLOC_MAX_BINS words) and then
+// the partial rows of the partitioned mode (kNoAllocPartial words).  Each
+// GPU holds its own copy at its own address, found once per device with
+// hipGetSymbolAddress when a context is created on it; host code uses that
+// address, never the array's name.
+constexpr uint64_t kNoAllocPartial = uint64_t(4) << 20;   // 16 MiB of partial rows
+static_assert(SCT_NOALLOC_MAX_BINS <= kNoAllocPartial, "one partial row must fit");
+static __device__ __align__(256) uint32_t scratch[SCT_NOALLOC_MAX_BINS + kNoAllocPartial];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];
+}  // namespace
+
+int sct_noalloc_scratch_setup(sct_ctx *ctx)
+{
+    if (ctx->device < 0 || ctx->device >= kMaxDevices)
+        return sct_fail(ctx, SCT_ERR_INVALID, "device index beyond the no-allocation table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        void *ptr = nullptr;
+        // the context's device is current (sct_ctx_create set it)
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return sct_hip_fail(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol,
+                                                         "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[ctx->device] = ptr;
     }
+    ctx->noalloc_scratch = g_scratch_ptr[ctx->device];
+    return SCT_OK;
+}
+
+// sct_calc_hist with its device temporaries in the module's fixed scratch:
+// no hipMalloc, hipFree or pinned staging (the counts are copied straight
+// into the caller's array).
+extern "C" int sct_calc_hist_noalloc(sct_ctx *ctx, const sct_field *f, float rs, float re, uint32_t nbins,
+                                     float delta, uint64_t seed, uint32_t *counts, double *density)
+{
+    int rc = hist_check(ctx, f, nbins, counts);
+    if (rc) return rc;
+    if (nbins > SCT_NOALLOC_MAX_BINS)
+        return sct_fail(ctx, SCT_ERR_INVALID, "sct_calc_hist_noalloc: nbins " + std::to_string(nbins) +
+                                                  " exceeds SCT_NOALLOC_MAX_BINS (" +
+                                                  std::to_string(SCT_NOALLOC_MAX_BINS) + ")");
+    if (!ctx->noalloc_scratch) return sct_fail(ctx, SCT_ERR_HIP, "no-allocation scratch not set up");
+    const VoxGeom g = vox_geom(f);
+    HistArgs a = hist_args(rs, re, nbins, delta, seed);
+    const HistPlan p = hist_plan(ctx, g, a, kNoAllocPartial);
+    if (p.partial_words > kNoAllocPartial) return sct_fail(ctx, SCT_ERR_INVALID, "partial rows exceed the scratch");
+    uint32_t *d = (uint32_t *)ctx->noalloc_scratch;
+    uint32_t *partial = p.used ? d + SCT_NOALLOC_MAX_BINS : nullptr;
+    hipError_t e = hist_passes(ctx, f, g, a, p, d, partial);
+    if (e == hipSuccess) e = hipMemcpyAsync(counts, d, sizeof(uint32_t) * nbins, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) (void)hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return sct_hip_fail(ctx, e, "calc_hist_noalloc");
+    hist_density(f, rs, re, nbins, counts, density);
     return SCT_OK;
 }
 

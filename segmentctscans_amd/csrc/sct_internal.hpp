@@ -95,6 +95,7 @@ struct sct_ctx {
     size_t scratch_bytes = 0;
     void *host_scratch = nullptr;    // pinned host staging of their results
     size_t host_scratch_bytes = 0;
+    void *noalloc_scratch = nullptr; // device address of this GPU's fixed scratch (sct_calc_hist_noalloc)
 };
 
 struct sct_field {
@@ -124,6 +125,10 @@ void sct_free(sct_ctx *ctx, void *p);
 // synchronised before it reuses host scratch).
 int sct_ctx_scratch(sct_ctx *ctx, size_t bytes, void **p);
 int sct_ctx_host_scratch(sct_ctx *ctx, size_t bytes, void **p);
+
+// Finds the device address of the fixed scratch of sct_calc_hist_noalloc on
+// the context's (current) device, once per device, into ctx->noalloc_scratch.
+int sct_noalloc_scratch_setup(sct_ctx *ctx);
 
 // ---- geometry construction ------------------------------------------------
 int sct_geometry_build(sct_ctx *ctx, const sct_layout *l, Geometry &g);
