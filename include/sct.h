@@ -337,6 +337,11 @@ int sct_gmm1d_assign(const sct_samples *s, int K, const double *pi, const double
  * *bad_sqrt and *bad_div receive the numbers of differing results. */
 int sct_selftest_cr_math(sct_ctx *ctx, uint64_t n, uint64_t seed, uint64_t *bad_sqrt, uint64_t *bad_div);
 
+/* Rounds x[n] to the stored element of Xi and V-bar for storage
+ * SCT_STORAGE_BF16 or SCT_STORAGE_F16 with the solver's own device code;
+ * out[n] receives the 16-bit patterns.  Synchronises. */
+int sct_selftest_store_round(sct_ctx *ctx, int storage, const float *x, uint64_t n, uint16_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -140,7 +140,10 @@ static inline uint16_t f32_to_f16_bits(float f)
     uint32_t x;
     memcpy(&x, &f, 4);
     const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
-    if (ax >= 0x7f800000u) return (uint16_t)(sign | (ax > 0x7f800000u ? 0x7e00u : 0x7c00u));
+    /* NaN: quiet, with the top 9 payload bits kept (what the hardware
+     * conversion of the solver's half storage does) */
+    if (ax > 0x7f800000u) return (uint16_t)(sign | 0x7e00u | ((ax >> 13) & 0x1ffu));
+    if (ax == 0x7f800000u) return (uint16_t)(sign | 0x7c00u);
     if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);          /* rounds to infinity */
     if (ax < 0x38800000u) {                                             /* half subnormal or zero */
         const int e = (int)(ax >> 23);

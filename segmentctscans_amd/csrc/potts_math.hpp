@@ -379,13 +379,9 @@ struct StorageF32 {
 struct StorageBF16 {
     using T = uint16_t;   // bfloat16 bit pattern
     static __device__ __forceinline__ float ld(T x) { return __uint_as_float(uint32_t(x) << 16); }
-    static __device__ __forceinline__ T st(float f)
-    {
-        uint32_t x = __float_as_uint(f);
-        if ((x & 0x7fffffffu) > 0x7f800000u) return T((x | 0x00400000u) >> 16);   // quiet NaN
-        x += 0x7fffu + ((x >> 16) & 1u);
-        return T(x >> 16);
-    }
+    // v_cvt_pk_bf16_f32: round to nearest even, NaN quietened with its top
+    // payload bits kept; two adjacent conversions pack into one instruction
+    static __device__ __forceinline__ T st(float f) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f)); }
 };
 
 struct StorageF16 {

@@ -57,3 +57,47 @@ extern "C" int sct_selftest_cr_math(sct_ctx *ctx, uint64_t n, uint64_t seed, uin
     *bad_div = h[1];
     return SCT_OK;
 }
+
+namespace {
+
+// The solver's rounding of a float to the stored Xi / V-bar element.
+template <typename St>
+__global__ __launch_bounds__(256) void store_round_kernel(const float *__restrict__ x, uint64_t n,
+                                                          uint16_t *__restrict__ out)
+{
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        out[i] = __builtin_bit_cast(uint16_t, St::st(x[i]));
+}
+
+}  // namespace
+
+extern "C" int sct_selftest_store_round(sct_ctx *ctx, int storage, const float *x, uint64_t n, uint16_t *out)
+{
+    if (!ctx || (n && (!x || !out))) return sct_fail(ctx, SCT_ERR_INVALID, "null argument");
+    if (storage != SCT_STORAGE_BF16 && storage != SCT_STORAGE_F16)
+        return sct_fail(ctx, SCT_ERR_INVALID, "storage must be SCT_STORAGE_BF16 or SCT_STORAGE_F16");
+    if (!n) return SCT_OK;
+    float *dx = nullptr;
+    uint16_t *dy = nullptr;
+    int rc = sct_alloc(ctx, (void **)&dx, n * sizeof(float));
+    if (!rc) rc = sct_alloc(ctx, (void **)&dy, n * sizeof(uint16_t));
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMemcpyAsync(dx, x, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
+    if (!rc && e == hipSuccess) {
+        const uint32_t grid = uint32_t(std::min<uint64_t>((n + 255) / 256, uint64_t(ctx->cu_count) * 16));
+        if (storage == SCT_STORAGE_BF16)
+            store_round_kernel<sct::StorageBF16><<<grid, 256, 0, ctx->stream>>>(dx, n, dy);
+        else
+            store_round_kernel<sct::StorageF16><<<grid, 256, 0, ctx->stream>>>(dx, n, dy);
+        e = hipGetLastError();
+    }
+    if (!rc && e == hipSuccess) e = hipMemcpyAsync(out, dy, n * sizeof(uint16_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (!rc && e != hipSuccess) {
+        (void)hipStreamSynchronize(ctx->stream);
+        rc = sct_hip_fail(ctx, e, "store_round self-test");
+    }
+    sct_free(ctx, dx);
+    sct_free(ctx, dy);
+    return rc;
+}
