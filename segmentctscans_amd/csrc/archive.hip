@@ -154,7 +154,7 @@ extern "C" size_t sct_bin_bound(const sct_layout *l, int dtype)
     return size_t((header + vox + 7) / 8);
 }
 
-extern "C" int sct_bin_write(const sct_layout *l, int dtype, float lo, float hi, const void *data, void *out,
+static int sct_bin_write_impl(const sct_layout *l, int dtype, float lo, float hi, const void *data, void *out,
                              size_t cap, size_t *written)
 {
     if (!l || !data || !out || !written || !l->base || (l->width && !l->start_row) || l->nstride == 0 ||
@@ -230,7 +230,7 @@ extern "C" int sct_bin_write(const sct_layout *l, int dtype, float lo, float hi,
     return SCT_OK;
 }
 
-extern "C" int sct_bin_parse(const void *in, size_t len, int dtype, sct_bin_info *info, uint32_t *start_row,
+static int sct_bin_parse_impl(const void *in, size_t len, int dtype, sct_bin_info *info, uint32_t *start_row,
                              uint32_t *base)
 {
     if (!in || !info || (dtype != SCT_F32 && dtype != SCT_U16)) return SCT_ERR_INVALID;
@@ -247,6 +247,11 @@ extern "C" int sct_bin_parse(const void *in, size_t len, int dtype, sct_bin_info
     I.nstride = uint32_t(ns);
     I.col_begin = uint32_t(cb);
     I.col_end = uint32_t(ce);
+    // start_row[width] and base[width + 1] follow, one Exp-Golomb code (at
+    // least one bit) each, plus the base count: a width the remaining bits
+    // cannot hold is rejected before anything is sized by it
+    if (2 * uint64_t(I.width) + 2 > uint64_t(len) * 8 - std::min<uint64_t>(r.bit, uint64_t(len) * 8))
+        return SCT_ERR_SIZE;
     std::vector<uint32_t> sr(I.width), bs(size_t(I.width) + 1);
     if (I.width) {
         sr[0] = uint32_t(r.ue());
@@ -280,7 +285,7 @@ extern "C" int sct_bin_parse(const void *in, size_t len, int dtype, sct_bin_info
     return SCT_OK;
 }
 
-extern "C" int sct_bin_read(const void *in, size_t len, int dtype, const sct_bin_info *info, void *data)
+static int sct_bin_read_impl(const void *in, size_t len, int dtype, const sct_bin_info *info, void *data)
 {
     if (!in || !info || !data || (dtype != SCT_F32 && dtype != SCT_U16) || info->nstride == 0) return SCT_ERR_INVALID;
     const unsigned nb = elem_bits(dtype);
@@ -311,4 +316,35 @@ extern "C" int sct_bin_read(const void *in, size_t len, int dtype, const sct_bin
         }
     });
     return SCT_OK;
+}
+
+// The C ABI lets no exception through: a failed allocation (or any other
+// exception) of the host code above is an error status.
+extern "C" int sct_bin_parse(const void *in, size_t len, int dtype, sct_bin_info *info, uint32_t *start_row,
+                             uint32_t *base)
+{
+    try {
+        return sct_bin_parse_impl(in, len, dtype, info, start_row, base);
+    } catch (...) {
+        return SCT_ERR_INVALID;
+    }
+}
+
+extern "C" int sct_bin_read(const void *in, size_t len, int dtype, const sct_bin_info *info, void *data)
+{
+    try {
+        return sct_bin_read_impl(in, len, dtype, info, data);
+    } catch (...) {
+        return SCT_ERR_INVALID;
+    }
+}
+
+extern "C" int sct_bin_write(const sct_layout *layout, int dtype, float lo, float hi, const void *data, void *buf,
+                             size_t cap, size_t *written)
+{
+    try {
+        return sct_bin_write_impl(layout, dtype, lo, hi, data, buf, cap, written);
+    } catch (...) {
+        return SCT_ERR_INVALID;
+    }
 }

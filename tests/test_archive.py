@@ -2,11 +2,14 @@
 host only.  The oracle's bit-level restatement (oracle/archive.py) is pinned
 by the textbook Exp-Golomb codewords and a hand-assembled stream; the library
 writer must produce the oracle's bytes and the reader must invert both."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
 from oracle import archive as OA
 from segmentctscans_amd import Field, Mapping, SctError
+from segmentctscans_amd import _native as N
 from segmentctscans_amd import archive as A
 from segmentctscans_amd.phantom import make_phantom
 
@@ -89,6 +92,19 @@ def test_truncated_and_invalid():
         A.load_bin(b[:-3])
     with pytest.raises(SctError):
         A.load_bin(b"\x00\x00\x00\x00")
+
+
+def test_header_width_beyond_the_stream_is_rejected():
+    """A few header bytes can claim a width of about 2^32 columns; the parser
+    rejects it from the stream length before sizing any table by it."""
+    w = OA.BitWriter()
+    for v in (10, 10, 0xFFFFFFF0, 16, 0, 0xFFFFFFF0, 0xFFFFFFF0):   # extents, nstride, col range, |start_row|
+        w.ue(v)
+    buf = np.frombuffer(w.tobytes() + b"\xff" * 16, np.uint8).copy()
+    rc = N.lib().sct_bin_parse(buf.ctypes.data, buf.size, N.SCT_F32, C.byref(N.SctBinInfo()), None, None)
+    assert rc == N.SCT_ERR_SIZE
+    with pytest.raises(SctError):
+        A.load_bin(buf.tobytes())
 
 
 def test_cpp_save_load_bin_matches_oracle_bytes(tmp_path):
