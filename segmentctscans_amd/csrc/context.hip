@@ -287,7 +287,12 @@ extern "C" int sct_ctx_create(int device, sct_ctx **out)
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = uint32_t(prop.multiProcessorCount);
-    (void)hipDeviceGetDefaultMemPool(&c->pool, device);
+    // The default pool keeps up to 8 GiB of freed stream-ordered memory
+    // (per-call temporaries) instead of releasing it at every synchronisation.
+    if (hipDeviceGetDefaultMemPool(&c->pool, device) == hipSuccess && c->pool) {
+        uint64_t keep = uint64_t(8) << 30;
+        (void)hipMemPoolSetAttribute(c->pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     if (sct_noalloc_scratch_setup(c) != SCT_OK) {
         (void)hipStreamDestroy(c->stream);
         delete c;

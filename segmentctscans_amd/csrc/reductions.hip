@@ -792,15 +792,25 @@ extern "C" int sct_uv_hist(sct_ctx *ctx, const sct_field *u, const sct_field *v,
         lps = (g.lines + slices - 1) / slices;
         used = (g.lines + lps - 1) / lps;
     }
-    // with many partitions, the bins are computed once into a third region
-    const bool bin_pass = part_mode && parts > kBinPassParts;
     const size_t pbytes = (sizeof(uint32_t) * uint64_t(used) * total + 255) & ~size_t(255);
     void *mem = nullptr, *host = nullptr;
-    int rc = sct_ctx_scratch(ctx, dbytes + pbytes + (bin_pass ? sizeof(uint32_t) * u->geo.span() : 0), &mem);
+    int rc = sct_ctx_scratch(ctx, dbytes + pbytes, &mem);
     if (!rc) rc = sct_ctx_host_scratch(ctx, sizeof(uint32_t) * total, &host);
     if (rc) return rc;
     uint32_t *d = (uint32_t *)mem, *partial = used ? (uint32_t *)((char *)mem + dbytes) : nullptr;
-    uint32_t *bins = bin_pass ? (uint32_t *)((char *)mem + dbytes + pbytes) : nullptr;
+    // With many partitions the joint bins are computed once, into 4 bytes per
+    // voxel slot taken from the device's memory pool for this call only (the
+    // pool keeps it for the next call and the memory query counts it as
+    // available); without that memory the partitions compute the bins
+    // themselves.
+    uint32_t *bins = nullptr;
+    if (part_mode && parts > kBinPassParts) {
+        if (hipMallocAsync((void **)&bins, sizeof(uint32_t) * u->geo.span(), ctx->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            bins = nullptr;
+        }
+    }
+    const bool bin_pass = bins != nullptr;
     hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t) * total, ctx->stream);
     if (e == hipSuccess && part_mode) {
         // counters in LDS, by partitions of the bins (see uv_part_kernel)
@@ -835,6 +845,10 @@ extern "C" int sct_uv_hist(sct_ctx *ctx, const sct_field *u, const sct_field *v,
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(host, d, sizeof(uint32_t) * total, hipMemcpyDeviceToHost, ctx->stream);
+    if (bins) {
+        const hipError_t ef = hipFreeAsync(bins, ctx->stream);
+        if (e == hipSuccess) e = ef;
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) (void)hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return sct_hip_fail(ctx, e, "uv_hist");
