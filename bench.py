@@ -42,7 +42,8 @@ def single_pass_bytes(K: int, storage: str) -> int:
     return kernel_bytes(K, storage)["potts_fused_kernel"]
 
 
-KERNEL_NAMES = {1: ["potts_fused_kernel"], 2: ["potts_xi_kernel", "potts_v_kernel"]}
+KERNEL_NAMES = {"fused": ["potts_fused_kernel"], "twopass": ["potts_xi_kernel", "potts_v_kernel"]}
+KERNEL_NAMES["auto"] = KERNEL_NAMES["twopass"]
 
 
 def committed_traffic(kernel: str, workload: str):
@@ -590,7 +591,8 @@ def main() -> int:
     tl.close()
     ms_step = total_ms / args.steps
     value = vox * args.steps / (total_ms / 1e3)
-    names = KERNEL_NAMES[kms.shape[1]]
+    names = KERNEL_NAMES[args.kernel]
+    assert len(names) == kms.shape[1]
     kavg = [float(x) for x in kms.astype(np.float64).mean(axis=0)]
     log(f"timed {args.steps} steps: {ms_step:.3f} ms/step (events), wall {wall:.3f} s; per kernel "
         + ", ".join(f"{n} {a:.3f} ms" for n, a in zip(names, kavg)))

@@ -186,7 +186,7 @@ def test_iterate_timed_matches_iterate(ctx, kernel):
     b = S.PottsSolver(ctx, g, c, 30.0, TAU, TAU, kernel=kernel)
     a.iterate(7)
     t = b.iterate_timed(7)
-    assert t.shape == (7, 2 if kernel == "twopass" else 1)
+    assert t.shape == (7, 1 if kernel == "fused" else 2)
     assert np.all(t > 0)
     Va, xa = a.state()
     Vb, xb = b.state()
