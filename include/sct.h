@@ -121,6 +121,12 @@ typedef struct sct_potts_opts {
     int32_t  gap_every;     /* iterations between gap evaluations (0: never)      */
     int32_t  gap_capacity;  /* entries of gap_trace; sct_potts_min_partition
                                takes 0 as iters / gap_every                       */
+    double   gap_tol;       /* > 0: sct_potts_min_partition stops a slab at the
+                               first gap evaluation whose relative gap is below
+                               gap_tol (needs gap_every > 0; extension, the
+                               reference runs a fixed number of iterations)       */
+    int32_t *iters_done;    /* optional host out: iterations performed (the most
+                               over the slabs)                                    */
 } sct_potts_opts;
 
 void sct_potts_opts_init(sct_potts_opts *opts);

@@ -268,8 +268,10 @@ def download(ctx: Context, dfield: DeviceField, out: Field) -> None:
 # Potts
 # ---------------------------------------------------------------------------
 def _opts(storage="fp32", kernel="auto", slab_w=None, mem_available=None, dq=None,
-          gap=None, gap_every=0) -> N.SctPottsOpts:
-    """gap: host float64 array receiving the relative gap every gap_every iterations."""
+          gap=None, gap_every=0, gap_tol=0.0, iters_done=None) -> N.SctPottsOpts:
+    """gap: host float64 array receiving the relative gap every gap_every iterations;
+    gap_tol > 0 stops at the first evaluation below it, iters_done (c_int32)
+    receiving the iterations performed."""
     o = N.SctPottsOpts()
     N.lib().sct_potts_opts_init(C.byref(o))
     o.storage = N.STORAGE[storage]
@@ -280,21 +282,26 @@ def _opts(storage="fp32", kernel="auto", slab_w=None, mem_available=None, dq=Non
         o.dq_trace = dq.ctypes.data_as(N.f32p)
     if gap is not None:
         o.gap_trace = gap.ctypes.data_as(N.f64p)
-        o.gap_every = int(gap_every)
         o.gap_capacity = int(gap.size)
+    o.gap_every = int(gap_every)
+    o.gap_tol = float(gap_tol)
+    if iters_done is not None:
+        o.iters_done = C.pointer(iters_done)
     return o
 
 
 def potts_min_partition(ctx: Context, g: DeviceField, ci, lam: float, tau1: float, tau2: float,
                         iters: int, *, storage="fp32", kernel="auto", slab_w=None,
-                        mem_available=None, dq_trace=False, gap_every=0):
+                        mem_available=None, dq_trace=False, gap_every=0, gap_tol=0.0):
     """potts_min_partition<K>(ctx, g, cpu_map, ci, lambda, tau1, tau2, iters).
 
     Returns a DeviceField of c[argmax] labels, or None when the slab cannot
     hold the halo (the reference's null return, potts-min-partition.cpp:571-577).
     With dq_trace=True returns (field, dq[iters]); with gap_every > 0 the
     relative FP64 gap after every gap_every-th iteration is appended to the
-    returned tuple.
+    returned tuple.  gap_tol > 0 (with gap_every > 0) stops at the first
+    evaluation whose relative gap is below gap_tol and appends the number of
+    iterations performed.
     """
     c = np.ascontiguousarray(ci, np.float32)
     K = c.size
@@ -302,11 +309,16 @@ def potts_min_partition(ctx: Context, g: DeviceField, ci, lam: float, tau1: floa
         raise ValueError("K must be in [2, 6]")
     dq = np.zeros(max(iters, 1), np.float32) if dq_trace else None
     gap = np.full(max(iters // gap_every, 1), np.nan) if gap_every > 0 else None
-    o = _opts(storage, kernel, slab_w, mem_available, dq, gap, gap_every)
+    done = C.c_int32(-1)
+    o = _opts(storage, kernel, slab_w, mem_available, dq, gap, gap_every, gap_tol, done if gap_tol > 0 else None)
     h = C.c_void_p()
     rc = N.lib().sct_potts_min_partition(ctx.handle, g.handle, K, c.ctypes.data_as(N.f32p), lam, tau1,
                                          tau2, iters, C.byref(o), C.byref(h))
-    extra = ([dq[:iters]] if dq_trace else []) + ([gap[:iters // gap_every]] if gap is not None else [])
+    n_gap = iters // gap_every if gap is not None else 0
+    if gap is not None and gap_tol > 0 and done.value >= 0:
+        n_gap = done.value // gap_every
+    extra = ([dq[:iters]] if dq_trace else []) + ([gap[:n_gap]] if gap is not None else []) + \
+        ([done.value] if gap_tol > 0 else [])
     if rc == N.SCT_ERR_NOMEM:
         return (None, *[None] * len(extra)) if extra else None
     ctx.check(rc)

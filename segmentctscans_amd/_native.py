@@ -48,7 +48,7 @@ class SctPottsOpts(C.Structure):
         ("size", C.c_uint32), ("storage", C.c_int32), ("kernel", C.c_int32),
         ("slab_w", C.c_int32), ("mem_available", C.c_int64),
         ("dq_trace", f32p), ("gap_trace", f64p), ("gap_every", C.c_int32),
-        ("gap_capacity", C.c_int32),
+        ("gap_capacity", C.c_int32), ("gap_tol", C.c_double), ("iters_done", C.POINTER(C.c_int32)),
     ]
 
 

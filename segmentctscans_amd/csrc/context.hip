@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <set>
 
 // ---- errors ------------------------------------------------------------------
 int sct_fail(sct_ctx *ctx, int code, const std::string &msg)
@@ -38,6 +40,61 @@ int sct_alloc(sct_ctx *ctx, void **p, size_t bytes)
 void sct_free(sct_ctx *, void *p)
 {
     if (p) (void)hipFree(p);
+}
+
+// ---- field voxels ----------------------------------------------------------------
+// A field's voxels come from the device's stream-ordered memory pool, in the
+// order of its context's stream: a call that makes and drops an output field
+// (grad_magn, convert_to_fp) then costs no hipMalloc / hipFree, and the pool
+// keeps up to 8 GiB of freed memory for the next field (counted as available
+// by sct_device_memory).  A field outliving its context is freed after the
+// device has drained.  The live contexts are tracked for that.
+namespace {
+std::mutex g_live_mu;
+std::set<const sct_ctx *> g_live;
+
+int field_alloc(sct_ctx *ctx, sct_field *f, size_t bytes)
+{
+    f->device = ctx->device;
+    if (bytes == 0) bytes = 256;
+    if (hipMallocAsync(&f->data, bytes, ctx->stream) == hipSuccess) {
+        f->pooled = true;
+        return SCT_OK;
+    }
+    (void)hipGetLastError();
+    f->data = nullptr;
+    f->pooled = false;
+    return sct_alloc(ctx, &f->data, bytes);
+}
+
+void field_free(sct_field *f)
+{
+    if (!f->data) return;
+    if (!f->pooled) {
+        (void)hipFree(f->data);   // waits for the work that uses it
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    if (g_live.count(f->ctx)) {
+        (void)hipFreeAsync(f->data, f->ctx->stream);
+    } else {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(f->device);
+        (void)hipDeviceSynchronize();
+        (void)hipFreeAsync(f->data, nullptr);
+        (void)hipSetDevice(cur);
+    }
+    f->data = nullptr;
+}
+}  // namespace
+
+// Returns idle pool memory to the device before a large plain allocation.
+void sct_pool_trim(sct_ctx *ctx)
+{
+    if (!ctx || !ctx->pool) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipMemPoolTrimTo(ctx->pool, 0);
 }
 
 // Sizes round up to 1 MiB (device) and 64 KiB (host), so that calls of
@@ -298,6 +355,10 @@ extern "C" int sct_ctx_create(int device, sct_ctx **out)
         delete c;
         return SCT_ERR_HIP;
     }
+    {
+        std::lock_guard<std::mutex> lk(g_live_mu);
+        g_live.insert(c);
+    }
     *out = c;
     return SCT_OK;
 }
@@ -305,6 +366,10 @@ extern "C" int sct_ctx_create(int device, sct_ctx **out)
 extern "C" void sct_ctx_destroy(sct_ctx *ctx)
 {
     if (!ctx) return;
+    {
+        std::lock_guard<std::mutex> lk(g_live_mu);
+        g_live.erase(ctx);
+    }
     (void)hipStreamSynchronize(ctx->stream);
     sct_free(ctx, ctx->scratch);
     if (ctx->host_scratch) (void)hipHostFree(ctx->host_scratch);
@@ -352,7 +417,7 @@ extern "C" int sct_field_create(sct_ctx *ctx, const sct_layout *layout, int dtyp
     f->ctx = ctx;
     f->dtype = dtype;
     int rc = sct_geometry_build(ctx, layout, f->geo);
-    if (!rc) rc = sct_alloc(ctx, &f->data, f->geo.span() * elem_size(dtype));
+    if (!rc) rc = field_alloc(ctx, f, f->geo.span() * elem_size(dtype));
     if (rc) {
         sct_geometry_release(ctx, f->geo);
         delete f;
@@ -372,7 +437,7 @@ int sct_field_like(sct_ctx *ctx, const sct_field *src, int dtype, sct_field **ou
     f->ctx = ctx;
     f->dtype = dtype;
     int rc = sct_geometry_copy(ctx, src->geo, f->geo);
-    if (!rc) rc = sct_alloc(ctx, &f->data, std::max<size_t>(1, f->geo.span()) * elem_size(dtype));
+    if (!rc) rc = field_alloc(ctx, f, std::max<size_t>(1, f->geo.span()) * elem_size(dtype));
     if (rc) {
         sct_geometry_release(ctx, f->geo);
         delete f;
@@ -411,8 +476,7 @@ extern "C" int sct_download(sct_ctx *ctx, const sct_field *f, void *host, size_t
 extern "C" void sct_field_destroy(sct_field *f)
 {
     if (!f) return;
-    // hipFree waits for work using the buffer; the context may already be gone.
-    sct_free(nullptr, f->data);
+    field_free(f);
     sct_geometry_release(nullptr, f->geo);
     delete f;
 }

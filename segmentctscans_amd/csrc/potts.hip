@@ -12,6 +12,7 @@
 #include "twopass.hpp"
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 
@@ -171,10 +172,14 @@ int do_iterate(sct_potts_solver *s, int iters, float *dq) { SCT_DISPATCH_K(s->K,
 
 // do_iterate with the FP64 gap evaluated after every gap_every-th iteration of
 // the slab (count: iterations already done on it) into gap_trace[count / gap_every - 1].
-int iterate_with_gap(sct_potts_solver *s, int iters, float *dq, int &count, int capacity)
+// With stop, the iterations end at the first evaluation whose relative gap
+// (E - D) / E is below opts.gap_tol (the stopping test; count then holds the
+// iterations performed).
+int iterate_with_gap(sct_potts_solver *s, int iters, float *dq, int &count, int capacity, bool stop = false)
 {
     const int every = s->opts.gap_every;
-    if (every <= 0 || !s->opts.gap_trace) {
+    stop = stop && s->opts.gap_tol > 0.0;
+    if (every <= 0 || (!s->opts.gap_trace && !stop)) {
         count += iters;
         return do_iterate(s, iters, dq);
     }
@@ -185,10 +190,13 @@ int iterate_with_gap(sct_potts_solver *s, int iters, float *dq, int &count, int 
         if (rc) return rc;
         done += chunk;
         count += chunk;
-        if (count % every == 0 && count / every <= capacity) {
+        const bool record = s->opts.gap_trace && count / every <= capacity;
+        if (count % every == 0 && (record || stop)) {
             double E = 0.0, D = 0.0;
             if ((rc = potts_gap(s, &E, &D))) return rc;
-            s->opts.gap_trace[count / every - 1] = E != 0.0 ? (E - D) / E : 0.0;
+            const double rel = E != 0.0 ? (E - D) / E : 0.0;
+            if (record) s->opts.gap_trace[count / every - 1] = rel;
+            if (stop && rel < s->opts.gap_tol) break;
         }
     }
     return SCT_OK;
@@ -223,6 +231,7 @@ void release_buffers(sct_potts_solver *s)
 int alloc_buffers(sct_potts_solver *s, uint32_t slab_len)
 {
     const Geometry &geo = s->g->geo;
+    sct_pool_trim(s->ctx);   // the plan counted the pool's idle memory as available
     s->wstride = sct_get_stride(slab_len);
     s->wspan = uint64_t(geo.lines) * s->wstride;
     const size_t comp = std::max<uint64_t>(1, s->wspan);
@@ -248,6 +257,10 @@ int validate(sct_ctx *ctx, const sct_field *g, int K, const float *c, int iters,
     if (iters < 0) return sct_fail(ctx, SCT_ERR_INVALID, "iters must be >= 0");
     if (opts && (opts->gap_every < 0 || opts->gap_capacity < 0))
         return sct_fail(ctx, SCT_ERR_INVALID, "gap_every and gap_capacity must be >= 0");
+    if (opts && opts->size >= offsetof(sct_potts_opts, iters_done) && !(opts->gap_tol >= 0.0))
+        return sct_fail(ctx, SCT_ERR_INVALID, "gap_tol must be >= 0");
+    if (opts && opts->size >= offsetof(sct_potts_opts, iters_done) && opts->gap_tol > 0.0 && opts->gap_every <= 0)
+        return sct_fail(ctx, SCT_ERR_INVALID, "gap_tol needs gap_every > 0");
     return SCT_OK;
 }
 
@@ -520,6 +533,7 @@ extern "C" int sct_potts_min_partition(sct_ctx *ctx, const sct_field *g, int K, 
     if (!rc && o.dq_trace) {
         rc = sct_alloc(ctx, (void **)&d_dq, sizeof(float) * size_t(std::max(iters, 1)));
     }
+    int done_max = 0;
     for (int k = 0; !rc && k < plan.n_slabs; ++k) {
         s.s0 = uint32_t(k * plan.res_w);
         s.s1 = uint32_t(std::min(n, k * plan.res_w + plan.slab_w));
@@ -530,7 +544,9 @@ extern "C" int sct_potts_min_partition(sct_ctx *ctx, const sct_field *g, int K, 
         if (!rc) rc = do_init(&s);
         int count = 0;
         if (!rc) rc = iterate_with_gap(&s, iters, d_dq, count, o.gap_capacity > 0 ? o.gap_capacity
-                                                                                 : iters / std::max(1, o.gap_every));
+                                                                                 : iters / std::max(1, o.gap_every),
+                                       true);
+        done_max = std::max(done_max, count);
         const uint32_t r0 = s.s0 == 0 ? s.s0 : s.s0 + uint32_t(plan.halo);
         const uint32_t r1 = int(s.s1) == n ? s.s1 : s.s1 - uint32_t(plan.halo);
         if (!rc) rc = do_labels(&s, (float *)res->data, r0, r1);
@@ -549,6 +565,7 @@ extern "C" int sct_potts_min_partition(sct_ctx *ctx, const sct_field *g, int K, 
         sct_field_destroy(res);
         return rc;
     }
+    if (o.iters_done) *o.iters_done = done_max;
     *out = res;
     return SCT_OK;
 }

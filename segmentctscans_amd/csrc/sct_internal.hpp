@@ -103,6 +103,8 @@ struct sct_field {
     int dtype = SCT_F32;
     void *data = nullptr;
     Geometry geo;
+    int device = 0;
+    bool pooled = false;   // data from the device's stream-ordered pool (else hipMalloc)
 };
 
 // ---- error handling --------------------------------------------------------
@@ -118,6 +120,9 @@ int sct_hip_fail(sct_ctx *ctx, hipError_t e, const char *what);
 // ---- memory (hipMalloc / hipFree) -------------------------------------------
 int sct_alloc(sct_ctx *ctx, void **p, size_t bytes);
 void sct_free(sct_ctx *ctx, void *p);
+// Waits for the context's stream and returns the pool's idle memory (kept for
+// field allocations) to the device, before a large sct_alloc.
+void sct_pool_trim(sct_ctx *ctx);
 // Device / pinned host scratch of at least `bytes`, owned by the context and
 // grown on demand (kept until the context is destroyed).  For temporaries
 // within one call: the next call on the context's stream reuses the same

@@ -53,3 +53,28 @@ def test_gap_trace_c1(ctx, c1_phantom):
     assert np.array_equal(s.gap_trace(), gaps)
     s.close()
     g.close()
+
+
+def test_gap_stopping_test_on_c1(ctx, c1_phantom):
+    """The FP64 stopping test: with gap_tol the solve of the 64^3 phantom ends
+    at the first gap evaluation (every 10 iterations) whose relative gap is
+    below the tolerance, and its labels equal a fixed-iteration run of that
+    many iterations."""
+    c = c1_phantom.normalized_labels()
+    g = S.upload(ctx, c1_phantom.field)
+    res, gaps, done = S.potts_min_partition(ctx, g, c, 30.0, TAU, TAU, 500, gap_every=10, gap_tol=1e-4)
+    print(f"stopped after {done} iterations, relative gap {gaps[-1]:.3e}")
+    assert 0 < done < 500 and done % 10 == 0
+    assert gaps.size == done // 10
+    assert gaps[-1] < 1e-4 and np.all(gaps[:-1] >= 1e-4)
+    fixed = S.potts_min_partition(ctx, g, c, 30.0, TAU, TAU, done)
+    m = c1_phantom.field.mapping.valid_mask()
+    assert np.array_equal(res.to_host()[m], fixed.to_host()[m])
+    # never below the tolerance: all the iterations run
+    _, _, full = S.potts_min_partition(ctx, g, c, 30.0, TAU, TAU, 30, gap_every=10, gap_tol=1e-12)
+    assert full == 30
+    with pytest.raises(S.SctError):
+        S.potts_min_partition(ctx, g, c, 30.0, TAU, TAU, 30, gap_tol=1e-4)   # no gap_every
+    res.close()
+    fixed.close()
+    g.close()
