@@ -813,8 +813,8 @@ struct Tape : TapeBase {
     {
         const Geometry &geo = g->geo;
         const uint32_t n = geo.n_images, ws = sct_get_stride(n);
-        Ly.fwd = VolumeLines{geo.d_nbr, geo.lines, n, (n + 127) / 128, ws, geo.nstride, 0};
-        Ly.one = VolumeLines{geo.d_nbr, geo.lines, n, (n + 63) / 64, ws, geo.nstride, 0};
+        Ly.fwd = VolumeLines::make(geo.d_nbr, geo.lines, n, (n + 127) / 128, ws, geo.nstride, 0);
+        Ly.one = VolumeLines::make(geo.d_nbr, geo.lines, n, (n + 63) / 64, ws, geo.nstride, 0);
         Ly.wspan = uint64_t(geo.lines) * ws;
         S = std::max<uint64_t>(1, Ly.wspan);
         state = 5 * uint64_t(K - 1) * S + (S + 3) / 4;   // components, then the SweepRecord

@@ -48,15 +48,8 @@ VolumeLines volume_lines(const sct_potts_solver *s)
 {
     const Geometry &geo = s->g->geo;
     constexpr uint32_t cw = kWave * vpl_of<St>();
-    VolumeLines L;
-    L.nbr = geo.d_nbr;
-    L.lines = geo.lines;
-    L.len = s->s1 - s->s0;
-    L.vchunks = (L.len + cw - 1) / cw;
-    L.wstride = s->wstride;
-    L.in_stride = geo.nstride;
-    L.s0 = s->s0;
-    return L;
+    const uint32_t len = s->s1 - s->s0;
+    return VolumeLines::make(geo.d_nbr, geo.lines, len, (len + cw - 1) / cw, s->wstride, geo.nstride, s->s0);
 }
 
 // Base pointers of the state of Xi / V-bar buffer set b.

@@ -45,6 +45,23 @@ __device__ __forceinline__ float div_cr_normal(float a, float b)
     return __fmaf_rn(__fmaf_rn(-b, q1, a), r, q1);
 }
 
+// x / D, correctly rounded, for D in {3, 5, 7}: q0 = x RN(1/D), then one
+// residual correction, r = x - D q0 (exact in one FMA), q = q0 + r RN(1/D).
+// Equal bit for bit to the IEEE quotient for every finite x except -0 (which
+// gives +0); checked exhaustively over all float patterns on the host and by
+// sct_selftest_cr_math on the device.  Other D divide (exact for powers of 2).
+template <int D>
+__host__ __device__ __forceinline__ float div_small(float x)
+{
+    if constexpr (D == 3 || D == 5 || D == 7) {
+        constexpr float c = 1.0f / float(D);
+        const float q0 = x * c;
+        return __fmaf_rn(__fmaf_rn(-float(D), q0, x), c, q0);
+    } else {
+        return x / float(D);
+    }
+}
+
 struct DykstraLimits {
     float t1, stol;
 };
@@ -85,12 +102,22 @@ __device__ __forceinline__ void project_simplex(float (&q)[K], bool (*act)[K] = 
     for (int i = 0; i < K; ++i) u[i] = q[i];
     sort_desc<K>(u);
     float sum = 0.0f, theta = 0.0f;
+    auto candidate = [](float x, int i) {
+        switch (i) {   // i is a constant once unrolled; sum - 1 is never -0
+        case 0: return div_small<1>(x);
+        case 1: return div_small<2>(x);
+        case 2: return div_small<3>(x);
+        case 3: return div_small<4>(x);
+        case 4: return div_small<5>(x);
+        default: return div_small<6>(x);
+        }
+    };
 #pragma unroll
     for (int i = 0; i < K; ++i) {
         sum += u[i];
         // (rho_sum - 1) / rho equals the candidate computed at index rho - 1,
         // so the final division of :340 is the one already done here.
-        const float th = (sum - 1.0f) / float(i + 1);
+        const float th = candidate(sum - 1.0f, i);
         if (u[i] > th) theta = th;
     }
 #pragma unroll

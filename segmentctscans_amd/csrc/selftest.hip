@@ -1,5 +1,6 @@
 // selftest.hip -- GPU check of the range-specific correctly rounded helpers
-// of the Dykstra projection against the compiler's IEEE sqrtf and division.
+// of the Dykstra projection and of the simplex projection's small-constant
+// quotients against the compiler's IEEE sqrtf and division.
 #include "sct_internal.hpp"
 #include "potts_math.hpp"
 
@@ -29,6 +30,13 @@ __global__ __launch_bounds__(256) void cr_math_kernel(uint64_t n, uint64_t seed,
         const float a = __uint_as_float(a_lo + uint32_t(h2 % (b_bits - a_lo)));
         bs += __float_as_uint(sct::sqrt_cr_ge1(x)) != __float_as_uint(sqrtf(x));
         bd += __float_as_uint(sct::div_cr_normal(a, b)) != __float_as_uint(a / b);
+        // the small-constant quotients on any finite operand other than -0
+        const float y = __uint_as_float(uint32_t(h2 >> 32));
+        if (isfinite(y) && __float_as_uint(y) != 0x80000000u) {
+            bd += __float_as_uint(sct::div_small<3>(y)) != __float_as_uint(y / 3.0f);
+            bd += __float_as_uint(sct::div_small<5>(y)) != __float_as_uint(y / 5.0f);
+            bd += __float_as_uint(sct::div_small<7>(y)) != __float_as_uint(y / 7.0f);
+        }
     }
     if (bs) atomicAdd(&bad[0], bs);
     if (bd) atomicAdd(&bad[1], bd);

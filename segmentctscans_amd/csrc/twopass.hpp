@@ -28,16 +28,46 @@ struct Chunk {
     uint32_t solve;
 };
 
+// Division by a run-time constant d (1 <= d < 2^31) as a multiply-high and
+// shifts (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, figure 4.1), exact for every 32-bit n.
+struct FastDiv {
+    uint32_t d = 1, m = 1, s1 = 0, s2 = 0;
+    FastDiv() = default;
+    explicit FastDiv(uint32_t dv) : d(dv)
+    {
+        uint32_t l = 0;
+        while ((uint64_t(1) << l) < dv) ++l;   // l = ceil(log2 d)
+        m = uint32_t((uint64_t(1) << 32) * ((uint64_t(1) << l) - dv) / dv + 1);
+        s1 = l < 1 ? l : 1;
+        s2 = l > 1 ? l - 1 : 0;
+    }
+    __host__ __device__ __forceinline__ uint32_t div(uint32_t n) const
+    {
+        const uint32_t t = uint32_t((uint64_t(m) * n) >> 32);
+        return (t + ((n - t) >> s1)) >> s2;
+    }
+};
+
 struct VolumeLines {
     const LineNbr *nbr;
     uint32_t lines, len, vchunks, wstride, in_stride, s0;
+    FastDiv vdiv;   // division by vchunks
 
+    static VolumeLines make(const LineNbr *nbr, uint32_t lines, uint32_t len, uint32_t vchunks, uint32_t wstride,
+                            uint32_t in_stride, uint32_t s0)
+    {
+        VolumeLines L{nbr, lines, len, vchunks, wstride, in_stride, s0, FastDiv(vchunks ? vchunks : 1)};
+        return L;
+    }
     __host__ __device__ __forceinline__ uint64_t count() const { return uint64_t(lines) * vchunks; }
     template <int VPL>
-    __device__ __forceinline__ Chunk chunk(uint64_t gw) const
+    __device__ __forceinline__ Chunk chunk(uint64_t gw64) const
     {
-        const uint32_t line = uint32_t(gw / vchunks);
-        const uint32_t n0 = uint32_t(gw % vchunks) * (kWave * VPL);
+        // chunk counts stay below 2^32 (lines < 2^32 / vchunks is checked on the host)
+        const uint32_t gw = uint32_t(gw64);
+        const uint32_t line = vdiv.div(gw);
+        const uint32_t n0 = (gw - line * vchunks) * (kWave * VPL);
         const LineNbr nb = nbr[line];
         const auto at = [&](int32_t l) { return l >= 0 ? uint32_t(l) * wstride + n0 : kNone; };
         const uint32_t u = line * in_stride + s0 + n0;
@@ -173,7 +203,8 @@ template <int VPL>
 __device__ __forceinline__ uint64_t chunk_of_wave()
 {
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x, 64 / VPL);
-    return (uint64_t(b) * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    return uint64_t(b) * (blockDim.x / kWave) + w;
 }
 
 // init_v_kernel, potts-min-partition.cpp:56-94.
@@ -242,7 +273,7 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<
     __shared__ int lt[kXiLong];            // parked item: index | sweeps << 16
     __shared__ uint8_t sq[REC ? kXiItems : 1];   // sweeps of the work-list items
     __shared__ int nq, next, nl;
-    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave, w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     if (threadIdx.x == 0) {
         nq = 0;
         next = 0;
