@@ -336,6 +336,23 @@ int sct_gmm1d_pass(const sct_samples *s, int K, const double *pi, const double *
 int sct_gmm1d_assign(const sct_samples *s, int K, const double *pi, const double *mu, const double *var,
                      int32_t *clusters);
 
+/* ---- bimm-peaks: the sample likelihood of the bivariate intensity /
+ * gradient-magnitude mixture, bimm-peaks/puv.h:42-130 ------------------------
+ * Samples (u, v) (doubles) live on the device.  The mixture has K in [1, 6]
+ * pure phases with intensities I[K] and weights w[0..K-1], and the interfaces
+ * of the pairs i < j (strict-upper order) with weights w[K..]; noise
+ * deviation exp(log_sn), blur exp(log_sb), interface half-width exp(log_ds),
+ * noise correlation rho < 1. */
+typedef struct sct_uv_samples sct_uv_samples;
+int sct_uv_samples_create(sct_ctx *ctx, const double *u, const double *v, uint64_t n, sct_uv_samples **out);
+void sct_uv_samples_destroy(sct_uv_samples *s);
+/* *nll = sum over samples [first, first + count) of -log full_p(u, v)
+ * (bimm-peaks.cpp:458-472 before the division by the count), each interface
+ * density's integral over t by adaptive Gauss-Kronrod to 1e-10 relative.
+ * Deterministic.  Synchronises. */
+int sct_bimm_nll(const sct_uv_samples *s, int K, const double *w, const double *I, double log_sn, double log_sb,
+                 double log_ds, double rho, uint64_t first, uint64_t count, double *nll);
+
 /* ---- diagnostics (no counterpart in the reference) ------------------------ */
 /* Checks on the GPU that the Dykstra projection's range-specific correctly
  * rounded square root (x in [1, 2^64)) and quotient (b in [1, 2^64),

@@ -22,7 +22,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "_lib"
 OBJ = PKG / "_build"
 
-HIP_SOURCES = ["context.hip", "potts.hip", "potts_fused.hip", "chambolle.hip", "reductions.hip", "selftest.hip", "gap.hip", "batch.hip", "adjoint.hip", "archive.hip", "gmm.hip"]
+HIP_SOURCES = ["context.hip", "potts.hip", "potts_fused.hip", "chambolle.hip", "reductions.hip", "selftest.hip", "gap.hip", "batch.hip", "adjoint.hip", "archive.hip", "gmm.hip", "bimm.hip"]
 
 # -ffp-contract=off: the only fused multiply-adds are the explicit __fmaf_rn
 # calls, so results match the FP32 host mirror bit for bit.  Packed FP32

@@ -104,6 +104,10 @@ SIGNATURES = {
     "sct_potts_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
                                   C.POINTER(C.c_void_p), C.POINTER(C.c_float)]),
     "sct_potts_solver_gap": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "sct_uv_samples_create": (C.c_int, [C.c_void_p, f64p, f64p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "sct_uv_samples_destroy": (None, [C.c_void_p]),
+    "sct_bimm_nll": (C.c_int, [C.c_void_p, C.c_int, f64p, f64p, C.c_double, C.c_double, C.c_double, C.c_double,
+                               C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]),
     "sct_selftest_store_round": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p]),
     "sct_selftest_cr_math": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_uint64)]),
