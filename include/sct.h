@@ -249,7 +249,8 @@ int sct_bin_write(const sct_layout *layout, int dtype, float lo, float hi, const
 int sct_convert_to_fp(sct_ctx *ctx, const sct_field *in, float lo, float hi, sct_field **out);
 /* find_intensity_range(ctx, field, tail_thr), find_intensity_range.h:8-9: the
  * 65536-bin histogram of a u16 field and its tail cut; hist (optional)
- * receives the 65536 counts.  Synchronises. */
+ * receives the 65536 counts (64-bit; SCT_ERR_SIZE if 1/128 of the field's
+ * lines hold 2^32 or more voxels).  Synchronises. */
 int sct_find_intensity_range(sct_ctx *ctx, const sct_field *field, double tail_thr, uint16_t *lo,
                              uint16_t *hi, uint64_t *hist);
 

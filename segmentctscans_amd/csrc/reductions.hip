@@ -288,6 +288,18 @@ __global__ __launch_bounds__(256) void hist_sum_kernel(const uint32_t *__restric
     counts[b] = s;
 }
 
+// hist_sum_kernel into 64-bit counts: a slice's row stays 32-bit (a slice holds
+// far fewer than 2^32 voxels), the sum over slices need not.
+__global__ __launch_bounds__(256) void hist_sum64_kernel(const uint32_t *__restrict__ partial, uint32_t slices,
+                                                         uint32_t nbins, uint64_t *__restrict__ counts)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbins) return;
+    uint64_t s = 0;
+    for (uint32_t k = 0; k < slices; ++k) s += partial[uint64_t(k) * nbins + b];
+    counts[b] = s;
+}
+
 // grad_kernel of grad_magn.cpp:17-58 for one voxel: x its value's
 // neighbours along n (xm, xp), along rows (rm, rp) and columns (cm, cp).
 __device__ __forceinline__ float grad_value(bool inner, float xm, float xp, float rm, float rp, float cm, float cp)
@@ -973,7 +985,7 @@ __global__ __launch_bounds__(kPartThreads) void u16_part_kernel(VoxGeom g, const
 // The same histogram with device atomics, for a device without 128 KB of LDS
 // per workgroup.
 __global__ __launch_bounds__(256) void u16_hist_kernel(VoxGeom g, const uint16_t *__restrict__ in,
-                                                       uint32_t lines_per_block, uint32_t *__restrict__ counts)
+                                                       uint32_t lines_per_block, unsigned long long *__restrict__ counts)
 {
     const uint32_t l0 = blockIdx.x * lines_per_block, l1 = min(g.lines, l0 + lines_per_block);
     const uint32_t lane = threadIdx.x % kWave, nw = blockDim.x / kWave;
@@ -981,7 +993,7 @@ __global__ __launch_bounds__(256) void u16_hist_kernel(VoxGeom g, const uint16_t
     for (uint32_t p = threadIdx.x / kWave; p < pairs; p += nw) {
         const uint32_t line = l0 + p / g.chunks, n = (p % g.chunks) * kWave + lane;
         if (n >= g.len) continue;
-        atomicAdd(&counts[in[uint64_t(line) * g.stride + n]], 1u);
+        atomicAdd(&counts[in[uint64_t(line) * g.stride + n]], 1ull);
     }
 }
 
@@ -1020,7 +1032,10 @@ extern "C" int sct_convert_to_fp(sct_ctx *ctx, const sct_field *in, float lo, fl
 
 // find_intensity_range(ctx, field, tail_thr), find_intensity_range.cpp:63-110:
 // the 65536-bin histogram of a u16 field and the tail cut.  hist (optional)
-// receives the 65536 counts.  Synchronises.
+// receives the 65536 counts.  Counts are 64-bit, as the reference's
+// (find_intensity_range.cpp:48-60): per-slice rows are 32-bit, so a slice of
+// lines must hold fewer than 2^32 voxels (SCT_ERR_SIZE otherwise; a slice is
+// 1/128 of the volume on MI355X).  Synchronises.
 extern "C" int sct_find_intensity_range(sct_ctx *ctx, const sct_field *f, double tail_thr, uint16_t *lo,
                                         uint16_t *hi, uint64_t *hist)
 {
@@ -1033,14 +1048,17 @@ extern "C" int sct_find_intensity_range(sct_ctx *ctx, const sct_field *f, double
     const uint32_t slices = std::max<uint32_t>(1, std::min<uint32_t>(std::max<uint32_t>(g.lines, 1), ctx->cu_count / 2));
     const uint32_t lps = std::max<uint32_t>(1, (g.lines + slices - 1) / slices);
     const uint32_t used = (g.lines + lps - 1) / lps;
-    const size_t dbytes = sizeof(uint32_t) * nbins;
-    uint32_t *h = nullptr;
+    if (uint64_t(lps) * g.len > 0xffffffffull)
+        return sct_fail(ctx, SCT_ERR_SIZE, "find_intensity_range: 2^32 or more voxels in a slice of lines");
+    const size_t dbytes = sizeof(uint64_t) * nbins;
+    uint64_t *h = nullptr;
     void *mem = nullptr;
     int rc = sct_ctx_scratch(ctx, dbytes + sizeof(uint32_t) * uint64_t(used) * nbins, &mem);
-    if (!rc) rc = sct_ctx_host_scratch(ctx, sizeof(uint32_t) * nbins, (void **)&h);
+    if (!rc) rc = sct_ctx_host_scratch(ctx, sizeof(uint64_t) * nbins, (void **)&h);
     if (rc) return rc;
-    uint32_t *d = (uint32_t *)mem, *partial = (uint32_t *)((char *)mem + dbytes);
-    hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t) * nbins, ctx->stream);
+    uint64_t *d = (uint64_t *)mem;
+    uint32_t *partial = (uint32_t *)((char *)mem + dbytes);
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(uint64_t) * nbins, ctx->stream);
     if (e == hipSuccess && g.lines && g.len) {
         const size_t lds = sizeof(uint32_t) * kPartBins;
         if (hipFuncSetAttribute((const void *)u16_part_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1048,7 +1066,7 @@ extern "C" int sct_find_intensity_range(sct_ctx *ctx, const sct_field *f, double
             u16_part_kernel<<<used * 2, kPartThreads, lds, ctx->stream>>>(g, (const uint16_t *)f->data, lps, partial);
             e = hipGetLastError();
             if (e == hipSuccess) {
-                hist_sum_kernel<<<nbins / 256, 256, 0, ctx->stream>>>(partial, used, nbins, d);
+                hist_sum64_kernel<<<nbins / 256, 256, 0, ctx->stream>>>(partial, used, nbins, d);
                 e = hipGetLastError();
             }
         } else {
@@ -1057,11 +1075,11 @@ extern "C" int sct_find_intensity_range(sct_ctx *ctx, const sct_field *f, double
             const uint32_t blocks = std::max<uint32_t>(1, ctx->cu_count * 4);
             const uint32_t lpb = std::max<uint32_t>(1, (g.lines + blocks - 1) / blocks);
             u16_hist_kernel<<<(g.lines + lpb - 1) / lpb, 256, 0, ctx->stream>>>(
-                g, (const uint16_t *)f->data, lpb, d);
+                g, (const uint16_t *)f->data, lpb, (unsigned long long *)d);
             e = hipGetLastError();
         }
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(uint32_t) * nbins, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(uint64_t) * nbins, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) (void)hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return sct_hip_fail(ctx, e, "find_intensity_range");
