@@ -91,6 +91,8 @@ int init_st(sct_potts_solver *s)
 template <int K>
 int launch_init(sct_potts_solver *s)
 {
+    s->cur = 0;
+    if (s->Vb[0]) s->V = s->Vb[0];
     if (!slab_empty(s)) {
         int rc;
         switch (s->storage) {
@@ -197,11 +199,11 @@ int iterate_with_gap(sct_potts_solver *s, int iters, float *dq, int &count, int 
 int do_labels(sct_potts_solver *s, float *res, uint32_t r0, uint32_t r1) { SCT_DISPATCH_K(s->K, launch_labels, s, res, r0, r1); }
 
 // Bytes of solver state per voxel of a slab: Xi 3(K-1) and V-bar (K-1) in the
-// storage type (doubled for the fused kernel's ping-pong), V (K-1) in FP32.
+// storage type, V (K-1) in FP32; the fused kernel double-buffers Xi and V.
 uint64_t bytes_per_voxel(int K, int kernel, int storage)
 {
     const uint64_t copies = kernel == SCT_KERNEL_FUSED ? 2 : 1;
-    return (copies * 4 * storage_bytes(storage) + 4) * uint64_t(K - 1);
+    return ((copies * 3 + 1) * storage_bytes(storage) + copies * 4) * uint64_t(K - 1);
 }
 
 int pick_kernel(const sct_potts_opts &o)
@@ -214,9 +216,10 @@ void release_buffers(sct_potts_solver *s)
     for (int b = 0; b < 2; ++b) {
         sct_free(s->ctx, s->xi[b]);
         sct_free(s->ctx, s->vbar[b]);
+        sct_free(s->ctx, s->Vb[b]);
         s->xi[b] = s->vbar[b] = nullptr;
+        s->Vb[b] = nullptr;
     }
-    sct_free(s->ctx, s->V);
     s->V = nullptr;
     fused_release(s);
 }
@@ -233,9 +236,10 @@ int alloc_buffers(sct_potts_solver *s, uint32_t slab_len)
     int rc;
     for (int b = 0; b < nb; ++b) {
         if ((rc = sct_alloc(s->ctx, &s->xi[b], sb * comp * 3 * (s->K - 1)))) return rc;
-        if ((rc = sct_alloc(s->ctx, &s->vbar[b], sb * comp * (s->K - 1)))) return rc;
+        if ((rc = sct_alloc(s->ctx, (void **)&s->Vb[b], sizeof(float) * comp * (s->K - 1)))) return rc;
     }
-    if ((rc = sct_alloc(s->ctx, (void **)&s->V, sizeof(float) * comp * (s->K - 1)))) return rc;
+    if ((rc = sct_alloc(s->ctx, &s->vbar[0], sb * comp * (s->K - 1)))) return rc;
+    s->V = s->Vb[0];
     if (s->kernel == SCT_KERNEL_FUSED && (rc = fused_prepare(s))) return rc;
     return SCT_OK;
 }

@@ -21,11 +21,12 @@ struct ColInfo {
     uint32_t pad;
 };
 
-// A band of rows marched over the columns [cs, ce) where any of its rows is live.
-struct BandInfo {
-    uint32_t r0;      // first row of the band
-    uint32_t cs, ce;  // column range
-    uint32_t pad;
+// One workgroup of the fused kernel: the n-range [n0, n0 + TN) of the rows
+// [r0, r0 + R), marched over the columns [cs, ce) (V is updated there; Xi+ is
+// also recomputed at the row r0 - 1, at n0 - 1 and at the column cs - 1).
+struct FusedTile {
+    uint32_t n0, r0;
+    uint32_t cs, ce;
 };
 
 DykstraLimits dykstra_limits();
@@ -42,13 +43,15 @@ struct sct_potts_solver {
     int kernel = SCT_KERNEL_FUSED;
     uint32_t s0 = 0, s1 = 0, wstride = 0;
     uint64_t wspan = 0;
-    // Xi and V-bar are double-buffered for the fused kernel (read [cur], write
-    // [cur ^ 1]); the two-pass kernels update [0] in place.
+    // Xi and V are double-buffered for the fused kernel (read [cur], write
+    // [cur ^ 1]; V points at Vb[cur]); the two-pass kernels update [0] in
+    // place.  V-bar has one buffer (vbar[1] is unused).
     // Element type of Xi and V-bar follows opts.storage (float, bfloat16 or half).
     void *xi[2] = {nullptr, nullptr};
     void *vbar[2] = {nullptr, nullptr};
     int storage = SCT_STORAGE_F32;
     float *V = nullptr;
+    float *Vb[2] = {nullptr, nullptr};
     int cur = 0;
     float *d_dq = nullptr;
     int dq_cap = 0;
@@ -59,8 +62,9 @@ struct sct_potts_solver {
     int n_marks = 0;
     // fused-kernel tables (device)
     sct::ColInfo *d_cols = nullptr;
-    sct::BandInfo *d_bands = nullptr;
-    uint32_t n_bands = 0;
+    sct::FusedTile *d_tiles = nullptr;
+    uint32_t n_tiles = 0, tiles_len = 0;   // tiles of a slab of length tiles_len
+    void *d_fptrs = nullptr;   // the fused kernel's two component pointer tables
 };
 
 namespace sct {
