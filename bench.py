@@ -445,9 +445,10 @@ def batch_workload(S, seed: int, iters: int, autotune: bool = True) -> dict:
     box, K ~ U{3, 4, 5}, extra phases evenly spaced in [0.030, 0.060] raw)
     x lambda in {20, 25, ..., 60}, iters iterations each: one batched call
     (one captured hipGraph launch for the iterations) against the same solves
-    looped through potts_min_partition.  The batched call runs twice: with
-    the default Xi tile, then with the autotuned one (its trial iterations
-    lie outside the event-timed iterations)."""
+    looped through potts_min_partition.  The batched call runs three times:
+    with the default Xi tile, with the autotuned one (its trial iterations
+    lie outside the event-timed iterations), and as one persistent launch per
+    K group (autotuned tiles and workgroups per CU)."""
     from segmentctscans_amd.phantom import C4_LAMBDAS, c4_volumes
 
     tau = float(np.float32(1.0 / np.sqrt(12.0)))
@@ -470,6 +471,18 @@ def batch_workload(S, seed: int, iters: int, autotune: bool = True) -> dict:
     tiles = S.potts_batch_tiles(ctx)
     log(f"batched, {'autotuned' if autotune else 'default'} tiles {tiles}: iterations {ms:.1f} ms (events), "
         f"call {wall_batch:.2f} s")
+    # one persistent launch per K group (tiles from a work queue, grid barriers)
+    t = time.perf_counter()
+    outs_p, ms_p = S.potts_batch(ctx, probs, tau, tau, iters, graph=True, autotune=autotune, persistent=True)
+    wall_p = time.perf_counter() - t
+    tiles_p, occ_p = S.potts_batch_tiles(ctx), S.potts_batch_occupancy(ctx)
+    same_p = 0
+    for (g, _, _), a_, b_ in zip(probs, outs, outs_p):
+        m = g.mapping.valid_mask()
+        same_p += int(np.array_equal(a_.to_host()[m], b_.to_host()[m]))
+        b_.close()
+    log(f"persistent, tiles {tiles_p}, workgroups per CU {occ_p}: iterations {ms_p:.1f} ms (events), "
+        f"call {wall_p:.2f} s; labels equal to the graph's {same_p}/{len(probs)}")
     t = time.perf_counter()
     refs = [S.potts_min_partition(ctx, g, c, lam, tau, tau, iters) for (g, c, lam) in probs]
     ctx.synchronize()
@@ -492,6 +505,11 @@ def batch_workload(S, seed: int, iters: int, autotune: bool = True) -> dict:
             "batched_call_s": wall_batch, "looped_calls_s": wall_loop,
             "speedup_vs_looped": wall_loop / wall_batch,
             "identical_labels": agree,
+            "persistent": {"iterations_ms": ms_p, "voxel_iters_per_s": vox * iters / (ms_p / 1e3),
+                           "call_s": wall_p, "speedup_vs_looped": wall_loop / wall_p,
+                           "xi_tile_waves": {str(k): v for k, v in tiles_p.items()},
+                           "workgroups_per_cu": {str(k): v for k, v in occ_p.items()},
+                           "labels_equal_to_graph": same_p},
             "shapes": [[v.n, v.height, v.width, "box" if v.box else "cylinder", int(v.c.size)] for v in vols]}
 
 

@@ -344,14 +344,17 @@ def find_intensity_range(ctx: Context, f: DeviceField, tail_thr: float, *, hist:
 
 
 def potts_batch(ctx: Context, problems, tau1: float, tau2: float, iters: int, *, storage="fp32",
-                graph=True, autotune=False) -> tuple[list, float]:
+                graph=True, autotune=False, persistent=False) -> tuple[list, float]:
     """Many Potts solves in shared launches (no counterpart in the reference).
 
     problems: sequence of (DeviceField g, labels c, lambda).  Returns the
     label fields (one per problem, as potts_min_partition would return them)
     and the event time of the iterations in ms; graph=True runs all
-    iterations as one captured hipGraph launch; autotune=True first picks
-    each K group's Xi workgroup tile by timing (potts_batch_tiles reports it)."""
+    iterations as one captured hipGraph launch; persistent=True runs each K
+    group's iterations in one persistent launch (tiles from a work queue, a
+    grid barrier after each half-step); autotune=True first picks each K
+    group's Xi workgroup tile and, persistent, its workgroups per CU by timing
+    (potts_batch_tiles and potts_batch_occupancy report them)."""
     n = len(problems)
     arr = (N.SctPottsProblem * max(n, 1))()
     for i, (g, c, lam) in enumerate(problems):
@@ -366,7 +369,8 @@ def potts_batch(ctx: Context, problems, tau1: float, tau2: float, iters: int, *,
     outs = (C.c_void_p * max(n, 1))()
     ms = C.c_float()
     ctx.check(N.lib().sct_potts_batch(ctx.handle, n, arr, tau1, tau2, iters, N.STORAGE[storage],
-                                      (N.SCT_BATCH_GRAPH if graph else 0) | (N.SCT_BATCH_AUTOTUNE if autotune else 0),
+                                      (N.SCT_BATCH_GRAPH if graph else 0) | (N.SCT_BATCH_AUTOTUNE if autotune else 0)
+                                      | (N.SCT_BATCH_PERSISTENT if persistent else 0),
                                       outs, C.byref(ms)))
     return [DeviceField(ctx, C.c_void_p(outs[i]), problems[i][0].mapping) for i in range(n)], ms.value
 
@@ -383,6 +387,16 @@ def potts_batch_tiles(ctx: Context) -> dict:
     w = (C.c_int * 5)()
     ctx.check(N.lib().sct_potts_batch_tiles(ctx.handle, w))
     return {K: w[K - 2] for K in range(2, 7) if w[K - 2]}
+
+
+def potts_batch_occupancy(ctx: Context) -> dict:
+    """{K: workgroups per CU} of the persistent launches of the last
+    potts_batch call (0: one fewer than fit, within 24 waves and 8 workgroups per CU, and for every K of a call without
+    persistent=True)."""
+    w = (C.c_int * 5)()
+    ctx.check(N.lib().sct_potts_batch_occupancy(ctx.handle, w))
+    t = potts_batch_tiles(ctx)
+    return {K: w[K - 2] for K in t}
 
 
 def potts_adjoint(ctx: Context, g: DeviceField, target: DeviceField, ci, lam: float, tau1: float, tau2: float,

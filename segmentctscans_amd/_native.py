@@ -41,6 +41,7 @@ class SctPottsProblem(C.Structure):
 
 SCT_BATCH_GRAPH = 1
 SCT_BATCH_AUTOTUNE = 2
+SCT_BATCH_PERSISTENT = 4
 
 
 class SctPottsOpts(C.Structure):
@@ -101,6 +102,7 @@ SIGNATURES = {
     "sct_gmm1d_assign": (C.c_int, [C.c_void_p, C.c_int] + [C.POINTER(C.c_double)] * 3 + [C.POINTER(C.c_int32)]),
     "sct_potts_batch_tiles": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "sct_potts_batch_set_tiles": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "sct_potts_batch_occupancy": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "sct_potts_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
                                   C.POINTER(C.c_void_p), C.POINTER(C.c_float)]),
     "sct_potts_solver_gap": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

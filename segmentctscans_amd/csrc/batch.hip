@@ -9,8 +9,11 @@
 // lambda from a per-solve table.  Problems that share an input field share
 // one gathered copy of it.  With SCT_BATCH_GRAPH the launches of all
 // iterations of all groups are captured once into a hipGraph and run as a
-// single graph launch.  Every solve computes exactly what
-// sct_potts_min_partition computes for it (same kernels, same arithmetic).
+// single graph launch.  With SCT_BATCH_PERSISTENT each group runs all its
+// iterations in one persistent launch: its workgroups take Xi tiles, then V
+// tiles, from a ticket counter, with a grid-wide barrier after each
+// half-step.  Every solve computes exactly what sct_potts_min_partition
+// computes for it (same per-tile code, same arithmetic).
 #include "potts.hpp"
 #include "twopass.hpp"
 
@@ -27,6 +30,7 @@ namespace {
 struct Group {
     int K = 0;
     int bw = kXiBlockWaves;            // Xi workgroup tile: waves per workgroup
+    int occ = 0;                       // persistent launch: workgroups per CU (0: as many as allowed)
     std::vector<int> probs;            // problem indices
     std::vector<uint64_t> r_off;       // result offset of each solve in R
     uint64_t wspan = 0, n_chunks = 0, r_total = 0, u_total = 0;
@@ -136,9 +140,138 @@ TwoPassArgs<K, St> group_args(const Group &G)
     return a;
 }
 
-// Phases of a group: 0 build, 1 init, 2 one iteration, 3 labels.
+// Grid-wide barrier and tile tickets of a persistent launch (zeroed by
+// sync_reset_kernel before it).
+struct GridSync {
+    unsigned count, gen, err, pad;
+    unsigned long long ticket;
+};
+
+// A waiting workgroup gives up after this many polls (of the order of a
+// second; a half-step of the batched workloads takes milliseconds), flags the
+// launch as failed and leaves the kernel, as does every workgroup that then
+// finds the flag set, so a launch whose workgroups are not all resident ends
+// with an error instead of waiting for ever.
+constexpr uint32_t kSpinLimit = 1u << 21;
+
+__global__ void sync_reset_kernel(GridSync *g)
+{
+    if (threadIdx.x == 0) *g = GridSync{0, 0, 0, 0, 0};
+}
+
+// Every workgroup's writes before the barrier are visible to every
+// workgroup after it: each wave's writes are complete within the workgroup
+// (workgroup-scope release), then one thread releases them at agent scope
+// (the workgroups of one launch span all XCDs; one L2 write-back per
+// workgroup rather than one per wave) and, after the barrier, acquires at
+// agent scope for the workgroup.  Returns false (in every thread of the
+// workgroup) when the launch has failed.
+__device__ __forceinline__ bool grid_sync(GridSync *g, unsigned nblocks)
+{
+    __shared__ unsigned ok;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned gen = __hip_atomic_load(&g->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned arrived = __hip_atomic_fetch_add(&g->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived + 1 == nblocks) {
+            __hip_atomic_store(&g->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&g->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t polls = 0;
+            while (__hip_atomic_load(&g->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(2);
+                if (__hip_atomic_load(&g->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                if (++polls == kSpinLimit) {
+                    __hip_atomic_store(&g->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        ok = __hip_atomic_load(&g->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return ok != 0u;
+}
+
+// All iterations of one K group in one launch.  A workgroup of BW waves takes
+// tickets from the work queue, each worth kTilesPerTicket tiles: in the Xi
+// half-step a tile is the two-pass Xi kernel's workgroup of chunks, in the V
+// half-step BW chunks, one per wave.  The tickets of half-step h are
+// [base_h, base_h + tickets_h), and each workgroup draws one more to find the
+// half-step done, so base_{h+1} = base_h + tickets_h + gridDim.x.
+constexpr unsigned kTilesPerTicket = 8;
+
+template <int K, typename St, int BW>
+__global__ __launch_bounds__(64 * BW) void potts_batch_persistent_kernel(BatchLines L, TwoPassArgs<K, St> a,
+                                                                         SolveTable<K> prm, float tau1, float tau2,
+                                                                         DykstraLimits lim, int iters, GridSync *gs)
+{
+    constexpr uint64_t cpt = uint64_t(kXiVoxelsPerLane / vpl_of<St>()) * BW;   // chunks per Xi tile
+    const unsigned long long nxi = (L.count() + cpt - 1) / cpt, nvt = (L.count() + BW - 1) / BW;
+    const unsigned long long kxi = (nxi + kTilesPerTicket - 1) / kTilesPerTicket,
+                             kvt = (nvt + kTilesPerTicket - 1) / kTilesPerTicket;
+    __shared__ unsigned long long tk;
+    unsigned long long base = 0;
+    auto next_ticket = [&]() -> unsigned long long {
+        if (threadIdx.x == 0) tk = atomicAdd(&gs->ticket, 1ull) - base;
+        __syncthreads();
+        const unsigned long long t = tk;
+        __syncthreads();
+        return t;
+    };
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    for (int it = 0; it < iters; ++it) {
+        for (unsigned long long k = next_ticket(); k < kxi; k = next_ticket()) {
+            const unsigned long long t1 = min(nxi, (k + 1) * kTilesPerTicket);
+            for (unsigned long long t = k * kTilesPerTicket; t < t1; ++t)
+                xi_tile<K, St, BatchLines, BW, false>(L, a, tau2, lim, SweepRecord{}, uint32_t(t));
+        }
+        base += kxi + gridDim.x;
+        if (!grid_sync(gs, gridDim.x)) return;
+        for (unsigned long long k = next_ticket(); k < kvt; k = next_ticket()) {
+            const unsigned long long t1 = min(nvt, (k + 1) * kTilesPerTicket);
+            for (unsigned long long t = k * kTilesPerTicket; t < t1; ++t)
+                v_chunk<K, St, BatchLines, SolveTable<K>>(L, a, prm, tau1, t * BW + w);
+        }
+        base += kvt + gridDim.x;
+        if (!grid_sync(gs, gridDim.x)) return;
+    }
+}
+
+constexpr int kMaxPerCu = 8, kMaxWavesPerCu = 24;
+
+template <int K, typename St, int BW>
+int launch_persistent(sct_ctx *ctx, const Group &G, const BatchLines &L, const TwoPassArgs<K, St> &a,
+                      const SolveTable<K> &prm, float tau1, float tau2, int iters, GridSync *gs)
+{
+    const void *fn = (const void *)potts_batch_persistent_kernel<K, St, BW>;
+    int fit = 0;
+    SCT_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, fn, 64 * BW, 0));
+    if (fit < 1) return sct_fail(ctx, SCT_ERR_HIP, "persistent batch kernel does not fit a CU");
+    // One workgroup per CU fewer than the occupancy query reports, at most 24
+    // waves and 8 workgroups per CU.  On MI355X these grids did not all become
+    // resident at once (their barriers timed out): 4-wave workgroups at 5 and
+    // at 8 per CU (every slot the query reports), 2-wave ones at 14 per CU,
+    // 1-wave ones at 31 per CU; within these limits they all did.
+    const int most = std::max(1, std::min(std::min(fit - 1, kMaxWavesPerCu / BW), kMaxPerCu));
+    const int per_cu = G.occ > 0 ? std::min(G.occ, most) : most;
+    const uint32_t grid = uint32_t(per_cu) * std::max<uint32_t>(1, ctx->cu_count);
+    sync_reset_kernel<<<1, 64, 0, ctx->stream>>>(gs);
+    potts_batch_persistent_kernel<K, St, BW><<<grid, 64 * BW, 0, ctx->stream>>>(L, a, prm, tau1, tau2,
+                                                                                dykstra_limits(), iters, gs);
+    SCT_HIP(ctx, hipGetLastError());
+    return SCT_OK;
+}
+
+// Phases of a group: 0 build, 1 init, 2 one iteration, 3 labels; 4 `iters`
+// iterations in one persistent launch.
 template <int K, typename St>
-int group_phase(sct_ctx *ctx, const sct_potts_problem *pr, Group &G, int phase, float tau1, float tau2)
+int group_phase(sct_ctx *ctx, const sct_potts_problem *pr, Group &G, int phase, float tau1, float tau2,
+                int iters = 1, GridSync *gs = nullptr)
 {
     if (phase == 0) return build_group<K, St>(ctx, pr, G);
     if (!G.n_chunks) return SCT_OK;
@@ -158,6 +291,11 @@ int group_phase(sct_ctx *ctx, const sct_potts_problem *pr, Group &G, int phase, 
         else
             potts_xi_kernel<K, St, BatchLines, 4><<<xi_grid<St, 4>(L.count()), 256, 0, st>>>(L, a, tau2, lim);
         potts_v_kernel<K, St, false><<<wave_grid(L.count()), 256, 0, st>>>(L, a, prm, tau1, nullptr);
+    } else if (phase == 4) {
+        if (iters <= 0) return SCT_OK;
+        if (G.bw == 1) return launch_persistent<K, St, 1>(ctx, G, L, a, prm, tau1, tau2, iters, gs);
+        if (G.bw == 2) return launch_persistent<K, St, 2>(ctx, G, L, a, prm, tau1, tau2, iters, gs);
+        return launch_persistent<K, St, 4>(ctx, G, L, a, prm, tau1, tau2, iters, gs);
     } else {
         potts_labels_kernel<K, St><<<wave_grid(L.count()), 256, 0, st>>>(L, a, prm, G.R, 0u, 0xffffffffu);
     }
@@ -166,75 +304,113 @@ int group_phase(sct_ctx *ctx, const sct_potts_problem *pr, Group &G, int phase, 
 }
 
 template <int K>
-int group_phase_k(sct_ctx *ctx, const sct_potts_problem *pr, Group &G, int storage, int phase, float t1, float t2)
+int group_phase_k(sct_ctx *ctx, const sct_potts_problem *pr, Group &G, int storage, int phase, float t1, float t2,
+                  int iters, GridSync *gs)
 {
     switch (storage) {
-    case SCT_STORAGE_BF16: return group_phase<K, StorageBF16>(ctx, pr, G, phase, t1, t2);
-    case SCT_STORAGE_F16: return group_phase<K, StorageF16>(ctx, pr, G, phase, t1, t2);
-    default: return group_phase<K, StorageF32>(ctx, pr, G, phase, t1, t2);
+    case SCT_STORAGE_BF16: return group_phase<K, StorageBF16>(ctx, pr, G, phase, t1, t2, iters, gs);
+    case SCT_STORAGE_F16: return group_phase<K, StorageF16>(ctx, pr, G, phase, t1, t2, iters, gs);
+    default: return group_phase<K, StorageF32>(ctx, pr, G, phase, t1, t2, iters, gs);
     }
 }
 
-int run_phase(sct_ctx *ctx, const sct_potts_problem *pr, Group &G, int storage, int phase, float t1, float t2)
+int run_phase(sct_ctx *ctx, const sct_potts_problem *pr, Group &G, int storage, int phase, float t1, float t2,
+              int iters = 1, GridSync *gs = nullptr)
 {
     switch (G.K) {
-    case 2: return group_phase_k<2>(ctx, pr, G, storage, phase, t1, t2);
-    case 3: return group_phase_k<3>(ctx, pr, G, storage, phase, t1, t2);
-    case 4: return group_phase_k<4>(ctx, pr, G, storage, phase, t1, t2);
-    case 5: return group_phase_k<5>(ctx, pr, G, storage, phase, t1, t2);
-    default: return group_phase_k<6>(ctx, pr, G, storage, phase, t1, t2);
+    case 2: return group_phase_k<2>(ctx, pr, G, storage, phase, t1, t2, iters, gs);
+    case 3: return group_phase_k<3>(ctx, pr, G, storage, phase, t1, t2, iters, gs);
+    case 4: return group_phase_k<4>(ctx, pr, G, storage, phase, t1, t2, iters, gs);
+    case 5: return group_phase_k<5>(ctx, pr, G, storage, phase, t1, t2, iters, gs);
+    default: return group_phase_k<6>(ctx, pr, G, storage, phase, t1, t2, iters, gs);
     }
 }
 
-// The autotuner: for each group, times kTuneIters iterations with each Xi
-// workgroup tile (BW = 4, 2, 1 waves: 1024, 512, 256 voxels and work lists
-// of 512, 256, 128 entries), keeps the fastest, and starts the group again
-// from its initial state, so the result is the untuned one.  Choices are
-// cached in the context per (K, storage, size class of the group).
+// The grid barrier of each group's persistent launches: a launch that gave up
+// waiting flags its GridSync.
+int check_sync(sct_ctx *ctx, const GridSync *gs, size_t n)
+{
+    std::vector<GridSync> h(n);
+    SCT_HIP(ctx, hipMemcpyAsync(h.data(), gs, sizeof(GridSync) * n, hipMemcpyDeviceToHost, ctx->stream));
+    SCT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (const GridSync &g : h)
+        if (g.err) return sct_fail(ctx, SCT_ERR_HIP, "persistent batch launch: a grid barrier timed out");
+    return SCT_OK;
+}
+
+// The autotuner: for each group, times kTuneIters iterations of each
+// configuration, keeps the fastest, and starts the group again from its
+// initial state, so the result is the untuned one.  The configurations are
+// the Xi workgroup tile (BW = 4, 2, 1 waves: 1024, 512, 256 voxels and work
+// lists of 512, 256, 128 entries) and, for the persistent launch, the
+// workgroups per CU (1, 2, or one fewer than fit within 24 waves and 8
+// workgroups per CU).  Choices are cached in the
+// context per (K, storage, launch form, size class of the group).
 constexpr int kTuneIters = 3;
 constexpr int kTiles[3] = {4, 2, 1};
+constexpr int kOcc[3] = {0, 2, 1};
 
-uint64_t tile_key(const Group &G, int storage)
+uint64_t tile_key(const Group &G, int storage, bool persistent)
 {
     uint64_t cls = 0;
     for (uint64_t c = G.n_chunks; c > 1; c >>= 1) ++cls;   // log2 of the chunk count
-    return (uint64_t(G.K) << 16) | (uint64_t(storage) << 8) | cls;
+    return (uint64_t(persistent) << 24) | (uint64_t(G.K) << 16) | (uint64_t(storage) << 8) | cls;
 }
 
-int autotune(sct_ctx *ctx, const sct_potts_problem *pr, std::vector<Group> &groups, int storage, float t1, float t2)
+int autotune(sct_ctx *ctx, const sct_potts_problem *pr, std::vector<Group> &groups, int storage, float t1, float t2,
+             GridSync *gs)
 {
+    const bool persistent = gs != nullptr;
     hipStream_t st = ctx->stream;
-    hipEvent_t ev[4] = {};
+    hipEvent_t ev[2] = {};
     int rc = SCT_OK;
     for (auto &e : ev)
         if (!rc && hipEventCreate(&e) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventCreate failed");
-    for (Group &G : groups) {
-        if (rc || !G.n_chunks || ctx->forced_tiles[G.K - 2]) continue;
-        const uint64_t key = tile_key(G, storage);
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        Group &G = groups[gi];
+        if (rc || !G.n_chunks) continue;
+        const int forced = ctx->forced_tiles[G.K - 2];
+        const uint64_t key = tile_key(G, storage, persistent);
         auto hit = ctx->tiles.find(key);
         if (hit != ctx->tiles.end()) {
-            G.bw = hit->second;
+            G.bw = forced ? forced : (hit->second & 0xff);
+            G.occ = hit->second >> 8;
             continue;
         }
         float best = 0;
-        int best_bw = kXiBlockWaves;
+        int best_bw = G.bw, best_occ = 0;
+        bool first = true;
         for (int i = 0; i < 3 && !rc; ++i) {
-            G.bw = kTiles[i];
-            // one untimed iteration, then kTuneIters timed ones
-            rc = run_phase(ctx, pr, G, storage, 2, t1, t2);
-            if (!rc && hipEventRecord(ev[0], st) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventRecord");
-            for (int it = 0; it < kTuneIters && !rc; ++it) rc = run_phase(ctx, pr, G, storage, 2, t1, t2);
-            if (!rc && hipEventRecord(ev[1], st) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventRecord");
-            float ms = 0;
-            if (!rc && (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess))
-                rc = sct_fail(ctx, SCT_ERR_HIP, "autotune timing failed");
-            if (!rc && (i == 0 || ms < best)) {
-                best = ms;
-                best_bw = kTiles[i];
+            if (forced && kTiles[i] != forced) continue;
+            for (int j = 0; j < (persistent ? 3 : 1) && !rc; ++j) {
+                G.bw = kTiles[i];
+                G.occ = persistent ? kOcc[j] : 0;
+                // one untimed iteration, then kTuneIters timed ones
+                auto run = [&](int n) {
+                    if (persistent) return run_phase(ctx, pr, G, storage, 4, t1, t2, n, gs + gi);
+                    int r = SCT_OK;
+                    for (int it = 0; it < n && !r; ++it) r = run_phase(ctx, pr, G, storage, 2, t1, t2);
+                    return r;
+                };
+                rc = run(1);
+                if (!rc && hipEventRecord(ev[0], st) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventRecord");
+                if (!rc) rc = run(kTuneIters);
+                if (!rc && hipEventRecord(ev[1], st) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventRecord");
+                float ms = 0;
+                if (!rc && (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess))
+                    rc = sct_fail(ctx, SCT_ERR_HIP, "autotune timing failed");
+                if (!rc && persistent) rc = check_sync(ctx, gs + gi, 1);
+                if (!rc && (first || ms < best)) {
+                    best = ms;
+                    best_bw = G.bw;
+                    best_occ = G.occ;
+                    first = false;
+                }
             }
         }
         G.bw = best_bw;
-        if (!rc) ctx->tiles[key] = best_bw;
+        G.occ = best_occ;
+        if (!rc && !forced) ctx->tiles[key] = best_bw | (best_occ << 8);
         if (!rc) rc = run_phase(ctx, pr, G, storage, 1, t1, t2);   // back to the initial state
     }
     for (auto e : ev)
@@ -245,7 +421,7 @@ int autotune(sct_ctx *ctx, const sct_potts_problem *pr, std::vector<Group> &grou
 // Runs the iterations of all groups, directly or as one captured graph, and
 // times them with events on the context's stream.
 int run_iterations(sct_ctx *ctx, const sct_potts_problem *pr, std::vector<Group> &groups, int storage, int iters,
-                   float t1, float t2, bool graph, float *elapsed_ms)
+                   float t1, float t2, bool graph, GridSync *gs, float *elapsed_ms)
 {
     hipStream_t st = ctx->stream;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -253,6 +429,11 @@ int run_iterations(sct_ctx *ctx, const sct_potts_problem *pr, std::vector<Group>
     hipGraphExec_t ge = nullptr;
     int rc = SCT_OK;
     auto launch_all = [&]() {
+        if (gs) {
+            for (size_t gi = 0; gi < groups.size() && !rc; ++gi)
+                rc = run_phase(ctx, pr, groups[gi], storage, 4, t1, t2, iters, gs + gi);
+            return;
+        }
         for (int it = 0; it < iters && !rc; ++it)
             for (Group &G : groups)
                 if (!rc) rc = run_phase(ctx, pr, G, storage, 2, t1, t2);
@@ -281,6 +462,7 @@ int run_iterations(sct_ctx *ctx, const sct_potts_problem *pr, std::vector<Group>
     if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = sct_fail(ctx, SCT_ERR_HIP, "batch iterations failed");
     if (!rc && elapsed_ms && hipEventElapsedTime(elapsed_ms, e0, e1) != hipSuccess)
         rc = sct_fail(ctx, SCT_ERR_HIP, "hipEventElapsedTime failed");
+    if (!rc && gs) rc = check_sync(ctx, gs, groups.size());
     if (ge) (void)hipGraphExecDestroy(ge);
     if (g) (void)hipGraphDestroy(g);
     if (e0) (void)hipEventDestroy(e0);
@@ -304,6 +486,13 @@ extern "C" int sct_potts_batch_tiles(sct_ctx *ctx, int *waves)
 {
     if (!ctx || !waves) return SCT_ERR_INVALID;
     for (int k = 0; k < 5; ++k) waves[k] = ctx->last_tiles[k];
+    return SCT_OK;
+}
+
+extern "C" int sct_potts_batch_occupancy(sct_ctx *ctx, int *per_cu)
+{
+    if (!ctx || !per_cu) return SCT_ERR_INVALID;
+    for (int k = 0; k < 5; ++k) per_cu[k] = ctx->last_occ[k];
     return SCT_OK;
 }
 
@@ -335,11 +524,18 @@ extern "C" int sct_potts_batch(sct_ctx *ctx, int n, const sct_potts_problem *pro
         if (!rc) rc = run_phase(ctx, problems, G, storage, 1, tau1, tau2);
     for (Group &G : groups)
         if (ctx->forced_tiles[G.K - 2]) G.bw = ctx->forced_tiles[G.K - 2];
-    if (!rc && (flags & SCT_BATCH_AUTOTUNE) && iters > 0) rc = autotune(ctx, problems, groups, storage, tau1, tau2);
-    for (int K = 2; K <= 6; ++K) ctx->last_tiles[K - 2] = 0;
-    for (const Group &G : groups) ctx->last_tiles[G.K - 2] = G.n_chunks ? G.bw : 0;
+    const bool persistent = (flags & SCT_BATCH_PERSISTENT) != 0;
+    GridSync *gs = nullptr;
+    if (!rc && persistent) rc = sct_alloc(ctx, (void **)&gs, sizeof(GridSync) * std::max<size_t>(1, groups.size()));
+    if (!rc && (flags & SCT_BATCH_AUTOTUNE) && iters > 0)
+        rc = autotune(ctx, problems, groups, storage, tau1, tau2, gs);
+    for (int K = 2; K <= 6; ++K) ctx->last_tiles[K - 2] = ctx->last_occ[K - 2] = 0;
+    for (const Group &G : groups) {
+        ctx->last_tiles[G.K - 2] = G.n_chunks ? G.bw : 0;
+        if (persistent && G.n_chunks) ctx->last_occ[G.K - 2] = G.occ;
+    }
     if (!rc) rc = run_iterations(ctx, problems, groups, storage, iters, tau1, tau2, (flags & SCT_BATCH_GRAPH) != 0,
-                                 elapsed_ms);
+                                 gs, elapsed_ms);
     for (Group &G : groups)
         if (!rc) rc = run_phase(ctx, problems, G, storage, 3, tau1, tau2);
     for (Group &G : groups) {
@@ -359,6 +555,7 @@ extern "C" int sct_potts_batch(sct_ctx *ctx, int n, const sct_potts_problem *pro
         (void)hipStreamSynchronize(ctx->stream);
     }
     for (Group &G : groups) free_group(ctx, G);
+    sct_free(ctx, gs);
     if (rc)
         for (int p = 0; p < n; ++p) {
             sct_field_destroy(out[p]);

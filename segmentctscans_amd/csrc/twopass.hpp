@@ -256,9 +256,9 @@ __global__ __launch_bounds__(256) void potts_xi_zero_kernel(Lines lines, TwoPass
 // BW: waves per workgroup (launch 64 BW threads); the workgroup covers 256 BW
 // voxels and its work list holds 128 BW of them.  REC: also store the
 // SweepRecord of every voxel (the solver runs without).
-template <int K, typename St, typename Lines, int BW = kXiBlockWaves, bool REC = false>
-__global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<K, St> a, float tau2,
-                                                       DykstraLimits lim, SweepRecord rec = {})
+template <int K, typename St, typename Lines, int BW, bool REC>
+__device__ __forceinline__ void xi_tile(const Lines &lines, const TwoPassArgs<K, St> &a, float tau2,
+                                        DykstraLimits lim, SweepRecord rec, uint32_t blk)
 {
     constexpr int KM = K - 1, NP = K * (K - 1) / 2;
     constexpr int VPL = vpl_of<St>(), CPW = kXiVoxelsPerLane / VPL;
@@ -281,7 +281,6 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<
     }
     __syncthreads();
     const uint64_t total = lines.count();
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, 64 / BW);
     float res[CPW][VPL][KM][3];
     int pos[CPW][VPL];
     int sfull[CPW][VPL];   // REC: sweeps of voxels not on the work list
@@ -470,21 +469,28 @@ __global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<
     }
 }
 
+template <int K, typename St, typename Lines, int BW = kXiBlockWaves, bool REC = false>
+__global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<K, St> a, float tau2,
+                                                       DykstraLimits lim, SweepRecord rec = {})
+{
+    xi_tile<K, St, Lines, BW, REC>(lines, a, tau2, lim, rec, xcd_block(blockIdx.x, gridDim.x, 64 / BW));
+}
+
 // div_kernel, potts-min-partition.cpp:349-488: V <- Pi_S(V + tau1 (div Xi - G)),
 // V-bar <- 2 V+ - V.  With TRACE, the block maximum of sum_k dV_k^2 goes to *dq.
 // One chunk per wave.  V and V-bar are stored non-temporally: the next reads
 // are a whole sweep away, and keeping them out of the L2 leaves it to the
 // neighbour lines (5% faster on 1426 x 1024^2; the Xi step's stores gain
 // nothing from it).
-template <int K, typename St, bool TRACE, typename Lines, typename Params>
-__global__ __launch_bounds__(256) void potts_v_kernel(Lines lines, TwoPassArgs<K, St> a, Params prm, float tau1,
-                                                      float *dq)
+// The V step of chunk gw (one wave); returns the largest sum_k dV_k^2 of its voxels.
+template <int K, typename St, typename Lines, typename Params>
+__device__ __forceinline__ float v_chunk(const Lines &lines, const TwoPassArgs<K, St> &a, const Params &prm,
+                                         float tau1, uint64_t gw)
 {
     constexpr int KM = K - 1, VPL = vpl_of<St>();
     using P = Packed<St, VPL>;
     using PF = Packed<StorageF32, VPL>;
     const int lane = threadIdx.x % kWave;
-    const uint64_t gw = chunk_of_wave<VPL>();
     float d = 0.0f;
     if (gw < lines.count()) {   // uniform over the wave
         const Chunk ch = lines.template chunk<VPL>(gw);
@@ -548,6 +554,14 @@ __global__ __launch_bounds__(256) void potts_v_kernel(Lines lines, TwoPassArgs<K
             }
         }
     }
+    return d;
+}
+
+template <int K, typename St, bool TRACE, typename Lines, typename Params>
+__global__ __launch_bounds__(256) void potts_v_kernel(Lines lines, TwoPassArgs<K, St> a, Params prm, float tau1,
+                                                      float *dq)
+{
+    float d = v_chunk<K, St, Lines, Params>(lines, a, prm, tau1, chunk_of_wave<vpl_of<St>()>());
     if constexpr (TRACE) {
         __shared__ float red[4];
         d = wave_max(d);

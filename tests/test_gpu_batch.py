@@ -100,3 +100,56 @@ def test_autotuned_batch_equals_untuned(ctx, volumes):
         x.close()
     with pytest.raises(S.SctError):
         S.potts_batch_set_tiles(ctx, {3: 3})
+
+
+@pytest.mark.parametrize("storage", ["fp32", "bf16"])
+@pytest.mark.parametrize("graph", [False, True])
+def test_persistent_batch_equals_single_solves(ctx, volumes, storage, graph):
+    """One persistent launch per K group (tiles from a work queue, a grid
+    barrier after each half-step) gives every problem its own
+    potts_min_partition labels."""
+    probs = _problems(volumes)
+    outs, ms = S.potts_batch(ctx, probs, TAU, TAU, 30, storage=storage, graph=graph, persistent=True)
+    assert len(outs) == len(probs) and ms > 0
+    for (g, c, lam), o in zip(probs, outs):
+        want = S.potts_min_partition(ctx, g, c, lam, TAU, TAU, 30, storage=storage).to_host()
+        m = g.mapping.valid_mask()
+        assert np.array_equal(o.to_host()[m], want[m]), (g.mapping.n_images, c.size, lam)
+        o.close()
+
+
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_persistent_every_tile_equals_graph(ctx, volumes, waves):
+    probs = _problems(volumes)
+    S.potts_batch_set_tiles(ctx, {K: waves for K in range(2, 7)})
+    try:
+        a, _ = S.potts_batch(ctx, probs, TAU, TAU, 20, graph=True)
+        b, _ = S.potts_batch(ctx, probs, TAU, TAU, 20, graph=False, persistent=True)
+        assert set(S.potts_batch_tiles(ctx).values()) == {waves}
+    finally:
+        S.potts_batch_set_tiles(ctx, {})
+    for x, y, (g, _, _) in zip(a, b, probs):
+        m = g.mapping.valid_mask()
+        assert np.array_equal(x.to_host()[m], y.to_host()[m]), waves
+        x.close()
+        y.close()
+
+
+def test_persistent_autotune(ctx, volumes):
+    """The persistent autotuner searches tiles x workgroups per CU; its trial
+    launches leave no trace and its choice is cached per context."""
+    probs = _problems(volumes)
+    a, _ = S.potts_batch(ctx, probs, TAU, TAU, 15, graph=True, autotune=True, persistent=True)
+    tiles, occ = S.potts_batch_tiles(ctx), S.potts_batch_occupancy(ctx)
+    assert set(tiles.values()) <= {1, 2, 4} and set(occ.values()) <= {0, 1, 2}
+    assert set(occ) == {c.size for (_, c, _) in probs}
+    b, _ = S.potts_batch(ctx, probs, TAU, TAU, 15, graph=True)
+    for x, y, (g, _, _) in zip(a, b, probs):
+        m = g.mapping.valid_mask()
+        assert np.array_equal(x.to_host()[m], y.to_host()[m])
+        x.close()
+        y.close()
+    c2, _ = S.potts_batch(ctx, probs, TAU, TAU, 15, graph=False, autotune=True, persistent=True)
+    assert S.potts_batch_tiles(ctx) == tiles and S.potts_batch_occupancy(ctx) == occ
+    for x in c2:
+        x.close()

@@ -1,7 +1,8 @@
 """The batched workload at its full size: 64 sub-volumes (edges U[48, 192] per
 axis, box or cylinder domain, K ~ U{3, 4, 5}) x lambda in {20, 25, ..., 60}
 = 576 FP32 solves of 300 iterations, run as one batched call (the iterations
-captured into one hipGraph launch, Xi tiles chosen by the autotuner).
+captured into one hipGraph launch, Xi tiles chosen by the autotuner) and again
+with one persistent launch per K group.
 
 Checks:
   - every solve's labels equal those of its own potts_min_partition call
@@ -34,13 +35,19 @@ def c4(ctx):
     for o in outs:
         o.close()
     print(f"C4 batched iterations: {ms:.1f} ms, tiles {S.potts_batch_tiles(ctx)}", flush=True)
-    yield vols, probs, labels
+    outs, ms = S.potts_batch(ctx, probs, TAU, TAU, ITERS, graph=True, autotune=True, persistent=True)
+    persistent = [o.to_host() for o in outs]
+    for o in outs:
+        o.close()
+    print(f"C4 persistent iterations: {ms:.1f} ms, tiles {S.potts_batch_tiles(ctx)}, "
+          f"workgroups per CU {S.potts_batch_occupancy(ctx)}", flush=True)
+    yield vols, probs, labels, persistent
     for g in dev:
         g.close()
 
 
 def test_c4_shape(c4):
-    vols, probs, _ = c4
+    vols, probs, _, _ = c4
     assert len(vols) == 64 and len(probs) == 576
     for v in vols:
         assert 48 <= min(v.n, v.height, v.width) and max(v.n, v.height, v.width) <= 192
@@ -50,7 +57,7 @@ def test_c4_shape(c4):
 
 
 def test_c4_every_solve_equals_its_single_solve(ctx, c4):
-    _, probs, labels = c4
+    _, probs, labels, _ = c4
     bad = []
     for i, ((g, c, lam), got) in enumerate(zip(probs, labels)):
         ref = S.potts_min_partition(ctx, g, c, lam, TAU, TAU, ITERS)
@@ -72,7 +79,7 @@ def _samples(vols):
 
 
 def test_c4_samples_bitwise_vs_fp32_mirror(c4):
-    vols, probs, labels = c4
+    vols, probs, labels, _ = c4
     for i, lam in _samples(vols):
         v = vols[i]
         lay = O.Layout.from_mapping(v.field.mapping)
@@ -82,3 +89,12 @@ def test_c4_samples_bitwise_vs_fp32_mirror(c4):
         assert np.array_equal(got[m].view(np.uint32), want[m].view(np.uint32)), (i, v.c.size, lam)
         print(f"volume {i} ({v.n}x{v.height}x{v.width}, {'box' if v.box else 'cylinder'}, K={v.c.size}), "
               f"lambda {lam}: labels == FP32 mirror")
+
+
+def test_c4_persistent_equals_graph(c4):
+    """The same 576 solves as one persistent launch per K group (tiles from a
+    work queue, grid barriers, autotuned tiles and workgroups per CU)."""
+    _, probs, labels, persistent = c4
+    bad = [i for i, (a, b, (g, _, _)) in enumerate(zip(labels, persistent, probs))
+           if not np.array_equal(a[g.mapping.valid_mask()], b[g.mapping.valid_mask()])]
+    assert not bad, f"{len(bad)} persistent solves differ: {bad[:5]}"
