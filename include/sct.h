@@ -364,9 +364,11 @@ int sct_bimm_nll(const sct_uv_samples *s, int K, const double *w, const double *
 /* ---- diagnostics (no counterpart in the reference) ------------------------ */
 /* Checks on the GPU that the Dykstra projection's range-specific correctly
  * rounded square root (x in [1, 2^64)) and quotient (b in [1, 2^64),
- * a in [2^-23, b)), and the simplex projection's quotients by 3, 5 and 7 (any
- * finite y other than -0), equal IEEE sqrtf and division on n random
- * operands; *bad_sqrt and *bad_div receive the numbers of differing results. */
+ * a in [2^-23, b)), the same quotient on the histograms' range (b in
+ * [2^-32, 2^32], |a| in [b 2^-40, b]), and the simplex projection's quotients
+ * by 3, 5 and 7 (any finite y other than -0), equal IEEE sqrtf and division
+ * on n random operands; *bad_sqrt and *bad_div receive the numbers of
+ * differing results. */
 int sct_selftest_cr_math(sct_ctx *ctx, uint64_t n, uint64_t seed, uint64_t *bad_sqrt, uint64_t *bad_div);
 
 /* Rounds x[n] to the stored element of Xi and V-bar for storage

@@ -588,20 +588,22 @@ def calc_hist(ctx: Context, f: DeviceField, *args, seed: int = 0, counts: bool =
     counts=True returns integer counts in place of the density.
     """
     L = N.lib()
+    if len(args) in (2, 4):
+        nbins = int(args[-2])
+        cnt = np.empty(nbins, np.uint32)
+        # the density is computed only when it is what the caller asked for
+        den = None if counts else np.empty(nbins, np.float64)
+        den_p = None if den is None else den.ctypes.data_as(N.f64p)
     if len(args) == 2:
-        nbins, delta = int(args[0]), float(args[1])
-        cnt = np.zeros(nbins, np.uint32)
-        den = np.zeros(nbins, np.float64)
+        delta = float(args[1])
         a, b = C.c_float(), C.c_float()
         ctx.check(L.sct_calc_hist_auto(ctx.handle, f.handle, nbins, delta, seed, cnt.ctypes.data_as(N.u32p),
-                                       den.ctypes.data_as(N.f64p), C.byref(a), C.byref(b)))
+                                       den_p, C.byref(a), C.byref(b)))
         return (cnt if counts else den), a.value, b.value
     if len(args) == 4:
-        rs, re_, nbins, delta = float(args[0]), float(args[1]), int(args[2]), float(args[3])
-        cnt = np.zeros(nbins, np.uint32)
-        den = np.zeros(nbins, np.float64)
+        rs, re_, delta = float(args[0]), float(args[1]), float(args[3])
         ctx.check(L.sct_calc_hist(ctx.handle, f.handle, rs, re_, nbins, delta, seed,
-                                  cnt.ctypes.data_as(N.u32p), den.ctypes.data_as(N.f64p)))
+                                  cnt.ctypes.data_as(N.u32p), den_p))
         return cnt if counts else den
     raise TypeError("calc_hist(ctx, f, nbins, delta) or calc_hist(ctx, f, rs, re, nbins, delta)")
 

@@ -6,6 +6,7 @@
 // results do not depend on scheduling.  Waves stream 64 consecutive n of one
 // line; work is distributed over (line, 64-chunk) pairs.
 #include "sct_internal.hpp"
+#include "potts_math.hpp"
 
 #include <algorithm>
 #include <cfloat>
@@ -13,6 +14,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 
 namespace {
 
@@ -190,15 +192,42 @@ struct HistArgs {
     uint32_t nbins;
     uint32_t k0, k1;        // dither key
     uint32_t lines_per_block;
+    uint32_t fast;          // hist_frac may take the short division (re - rs in range)
 };
 
-// bin = clamp(int(round(nbins * t - 0.5f)), 0, nbins - 1), t = (x - rs) / (re - rs)
-// (calc_hist.cpp:61-63); nbins * t - 0.5f is one fused multiply-add.
-__device__ __forceinline__ int32_t hist_bin(float x, float rs, float re, uint32_t nbins)
+// re - rs in [2^-32, 2^32] (as the device computes it): the range of
+// denominators for which hist_frac's short division is exact.
+inline uint32_t short_div_ok(float rs, float re)
 {
-    const float t = (x - rs) / (re - rs);
-    const int32_t b = int32_t(roundf(__fmaf_rn(float(nbins), t, -0.5f)));
-    return max(0, min(int32_t(nbins) - 1, b));
+    const float d = re - rs;
+    return d >= 0x1p-32f && d <= 0x1p32f;
+}
+
+// t = a / b (a = x - range start, b = range length) for hist_bin.  With b in
+// [2^-32, 2^32] (fast) and |a| >= b 2^-40 the compiler's IEEE division neither
+// scales its operands (v_div_scale) nor fixes up its result (v_div_fixup), so
+// its core -- reciprocal, a Newton step, the quotient and two residual
+// corrections, div_cr_normal -- is the correctly rounded quotient on its own;
+// for smaller |a| both give |t| < 2^-39 (or a t of a's sign), which bins to 0
+// whatever its last bits.  x is in the range, so a <= b.
+__device__ __forceinline__ float hist_frac(float a, float b, bool fast)
+{
+    return fast ? sct::div_cr_normal(a, b) : a / b;
+}
+
+// bin = clamp(int(round(nbins * t - 0.5f)), 0, nbins - 1), t = (x - rs) / (re - rs)
+// (calc_hist.cpp:61-63); nbins * t - 0.5f is one fused multiply-add.  For x
+// in the range (callers test it first) t <= 1, so y = nbins t - 0.5 < nbins:
+// the clamp at 0 goes first, y' = max(y, 0) (y in [-0.5, 0) rounds to -1 or
+// 0, both clamped to 0; NaN, from rs = re, gives 0 as its conversion would),
+// and round half away from zero of y' >= 0 is trunc(y') + (fract(y') >= 0.5),
+// fract(y') = y' - floor(y') being exact.
+__device__ __forceinline__ int32_t hist_bin(float x, float rs, float re, uint32_t nbins, bool fast)
+{
+    const float t = hist_frac(x - rs, re - rs, fast);
+    const float y = fmaxf(__fmaf_rn(float(nbins), t, -0.5f), 0.0f);
+    const int32_t b = int32_t(y) + int32_t(__builtin_amdgcn_fractf(y) >= 0.5f);
+    return min(int32_t(nbins) - 1, b);
 }
 
 // calc_hist_kernel (calc_hist.cpp:23-80).  MODE 0: u32 counters in LDS
@@ -225,7 +254,7 @@ __global__ __launch_bounds__(256) void hist_kernel(VoxGeom g, const float *__res
             x = x + (-half + a.delta * (float(r >> 8) * (1.0f / 16777216.0f)));
         }
         if (a.rs - half <= x && x + half <= a.re) {
-            const int32_t b = hist_bin(x, a.rs, a.re, a.nbins);
+            const int32_t b = hist_bin(x, a.rs, a.re, a.nbins, a.fast);
             if constexpr (MODE == 0) atomicAdd(&sh[b], 1u);
             else atomicAdd(&counts[b], 1u);
         }
@@ -240,64 +269,304 @@ __global__ __launch_bounds__(256) void hist_kernel(VoxGeom g, const float *__res
     }
 }
 
-// Histograms of more than kSmallBins bins: the bins are cut into partitions of
-// kPartBins u32 LDS counters (128 KB) and the lines into slices; workgroup
-// (slice, partition) counts its slice's voxels that fall in its partition and
-// stores the counts as the slice's row of `partial` (no atomics on device
-// memory); hist_sum_kernel adds the rows.  Each voxel is read once per
-// partition.  1024-thread workgroups: one per CU (the LDS), 16 waves.
+// ---- wide histograms: 16-bit counters in LDS ---------------------------------
+// Histograms of more than kSmallBins bins cut the bins into partitions of up
+// to kWideBins counters and the lines into slices.  Workgroup (slice,
+// partition) counts the voxels of its slice that fall in its partition with
+// 16-bit counters in LDS, two per 32-bit word (65536 bins in 128 KB, so a
+// histogram of up to 65536 bins reads every voxel once), and stores them as
+// the (slice, partition) block of `rows`; wide_sum_kernel adds the blocks.
+// 1024-thread workgroups, one per CU (the LDS): 16 waves.
+//
+// Counters stay within 16 bits by settling: the adds (LDS atomics whose
+// result is not read) run in periods of at most 32 voxels per thread, 32768
+// adds per workgroup; between two barriers after each period, settle() takes
+// kSpill = 2^15 off every counter at kSpill or more and appends its bin to the
+// workgroup's spill list, an entry per kSpill counts.  Every counter starts a
+// period below 2^15 and gains at most 32768 in it, so it never passes 65535
+// and never carries into its neighbour.  An entry stands for kSpill adds, so
+// a slice of at most kWideSliceVoxels voxel slots fills at most kSpillCap.
 constexpr uint32_t kSmallBins = 8192;
-constexpr uint32_t kPartBins = 32768;
+constexpr uint32_t kWideBins = 65536;                  // counters per partition
+constexpr uint32_t kWideWords = kWideBins / 2;         // their LDS words
+constexpr uint32_t kSpill = 32768;
+constexpr uint32_t kSpillCap = 256;
+constexpr uint64_t kWideSliceVoxels = uint64_t(kSpillCap) * kSpill;   // 2^23
+constexpr uint32_t kPeriodVoxels = 32;                 // per thread between settles
+// A (slice, partition) block of rows: the counter words, the spill count and
+// the spill bins (a multiple of 256 bytes, so every block is 16-byte aligned)
+constexpr uint32_t kWideRowWords = kWideWords + 512;
 constexpr uint32_t kMaxParts = 8;
 constexpr int kPartThreads = 1024;
+static_assert(kWideWords + 1 + kSpillCap <= kWideRowWords, "spill list fits the block");
+static_assert(uint64_t(kPartThreads) * kPeriodVoxels <= 65536 - kSpill, "a period keeps counters in 16 bits");
 
-template <bool DITHER>
-__global__ __launch_bounds__(kPartThreads) void hist_part_kernel(VoxGeom g, const float *__restrict__ u, HistArgs a,
-                                                                 uint32_t parts, uint32_t *__restrict__ partial)
+// Counter words of nb counters, a multiple of 4 (settle reads 16 bytes).
+__host__ __device__ inline uint32_t wide_words(uint32_t nb) { return ((nb + 1) / 2 + 3) & ~3u; }
+
+// LDS words of a wide kernel: the counter words, the spill list, its count
+// (132 KB of the CU's 160 KB).
+constexpr uint32_t kWideLdsWords = kWideWords + kSpillCap + 1;
+
+struct WideHist {
+    uint32_t *w, *list, *n;
+    uint32_t nw;
+    __device__ WideHist(uint32_t *sh, uint32_t nb)
+        : w(sh), list(sh + wide_words(nb)), n(sh + wide_words(nb) + kSpillCap), nw(wide_words(nb)) {}
+    // Followed by a barrier before the first add.
+    __device__ void clear() const
+    {
+        for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x) w[i] = 0;
+        if (threadIdx.x == 0) *n = 0;
+    }
+    __device__ __forceinline__ void add(uint32_t b) const { atomicAdd(&w[b >> 1], 1u << ((b & 1u) << 4)); }
+    // Called by every thread of the workgroup after each period.
+    __device__ void settle() const
+    {
+        __syncthreads();
+        for (uint32_t i = 4 * threadIdx.x; i < nw; i += 4 * blockDim.x) {
+            const uint4 q = *reinterpret_cast<const uint4 *>(w + i);
+            if (!((q.x | q.y | q.z | q.w) & 0x80008000u)) continue;
+            const uint32_t qw[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                uint32_t x = qw[k];
+#pragma unroll
+                for (uint32_t half = 0; half < 2; ++half)
+                    if ((x >> (16 * half)) & kSpill) {
+                        x -= kSpill << (16 * half);
+                        list[atomicAdd(n, 1u)] = 2 * (i + k) + half;
+                    }
+                w[i + k] = x;
+            }
+        }
+        __syncthreads();
+    }
+    // After a barrier that follows the last add: the block of rows.
+    __device__ void store(uint32_t *blk) const
+    {
+        for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x) blk[i] = w[i];
+        const uint32_t ns = *n;
+        if (threadIdx.x == 0) blk[kWideWords] = ns;
+        for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) blk[kWideWords + 1 + i] = list[i];
+    }
+};
+
+// Workgroups of a wide kernel: (slice, partition) pairs, the partitions of a
+// slice on one XCD (workgroups go to the 8 XCDs round-robin) so that its
+// voxels come from that XCD's L2 after the first partition reads them.
+inline uint32_t wide_grid(uint32_t used, uint32_t parts)
 {
-    extern __shared__ uint32_t sh[];
-    const uint32_t part = blockIdx.x % parts, slice = blockIdx.x / parts;
-    const uint32_t b0 = part * kPartBins, nb = min(kPartBins, a.nbins - b0);
-    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) sh[i] = 0;
+    return parts == 1 ? used : ((used + 7) / 8) * 8 * parts;
+}
+
+__device__ __forceinline__ bool wide_block(uint32_t parts, uint32_t used, uint32_t &slice, uint32_t &part)
+{
+    const uint32_t b = blockIdx.x;
+    if (parts == 1) {
+        slice = b;
+        part = 0;
+    } else {
+        part = (b / 8) % parts;
+        slice = b % 8 + 8 * (b / (8 * parts));
+    }
+    return slice < used;
+}
+
+constexpr uint32_t kNoBin = ~0u;
+
+// Counts the voxels of lines [l0, l1) into h: bin_of(x, line, n) is a voxel's
+// counter in h (kNoBin for none), x[k] its value in array k.  The lines are cut
+// into chunks of 64 x VEC voxels (VEC consecutive elements per lane: one
+// 16-byte load per array when VEC = 16 / sizeof(T), else one element); every
+// round each wave takes CPR chunks (four, two for two arrays), issues the
+// next round's loads and then counts this round's voxels; settle() follows every kPeriodVoxels voxels per
+// lane.  All waves run the same rounds, those past the last chunk on empty
+// ones (a load clamped to the slice, no voxel counted).  Lanes past a line's
+// end load its first elements.  Called by all kPartThreads threads.
+template <uint32_t VEC, typename T, int NA, typename F>
+__device__ __forceinline__ void count_lines(const WideHist &h, const VoxGeom &g, const T *const (&arr)[NA],
+                                            uint32_t l0, uint32_t l1, F &&bin_of)
+{
+    constexpr uint32_t CH = kWave * VEC, NW = kPartThreads / kWave;
+    constexpr uint32_t CPR = NA == 1 ? 4 : 2;   // chunks per wave and round (registers of two arrays)
+    constexpr uint32_t kRoundsPerSettle = kPeriodVoxels / (CPR * VEC);
+    static_assert(kRoundsPerSettle >= 1 && kPeriodVoxels % (CPR * VEC) == 0, "whole rounds per period");
+    const uint32_t lane = threadIdx.x % kWave;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint32_t cpl = (g.len + CH - 1) / CH;   // chunks per line
+    const uint32_t total = l0 < l1 ? (l1 - l0) * cpl : 0;
+    const uint32_t rounds = (total + CPR * NW - 1) / (CPR * NW);
+    if (!rounds) return;   // uniform over the workgroup
+    struct Batch {
+        T v[CPR][NA][VEC];
+        uint32_t line[CPR], n[CPR];   // n <= len; n = len: nothing to count
+    };
+    auto load = [&](uint32_t r, Batch &bt) {
+#pragma unroll
+        for (uint32_t k = 0; k < CPR; ++k) {
+            const uint32_t c = (CPR * r + k) * NW + w;
+            const uint32_t ce = min(c, total - 1);
+            const uint32_t li = ce / cpl;
+            const uint32_t nn = (ce - li * cpl) * CH + lane * VEC;
+            bt.line[k] = l0 + li;
+            bt.n[k] = c < total && nn < g.len ? nn : g.len;
+            const uint64_t o = uint64_t(bt.line[k]) * g.stride + (nn < g.len ? nn : 0);
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                if constexpr (VEC * sizeof(T) == 16) {
+                    const uint4 q = *reinterpret_cast<const uint4 *>(arr[a] + o);
+                    __builtin_memcpy(bt.v[k][a], &q, 16);
+                } else {
+#pragma unroll
+                    for (uint32_t j = 0; j < VEC; ++j) bt.v[k][a][j] = nn + j < g.len ? arr[a][o + j] : T(0);
+                }
+            }
+        }
+    };
+    auto count = [&](const Batch &bt, auto full) {
+#pragma unroll
+        for (uint32_t k = 0; k < CPR; ++k)
+#pragma unroll
+            for (uint32_t j = 0; j < VEC; ++j) {
+                if (!full && j >= g.len - bt.n[k]) continue;   // n <= len
+                T x[NA];
+#pragma unroll
+                for (int a = 0; a < NA; ++a) x[a] = bt.v[k][a][j];
+                const uint32_t b = bin_of(x, bt.line[k], bt.n[k] + j);
+                if (b != kNoBin) h.add(b);
+            }
+    };
+    Batch cur;
+    load(0, cur);
+    for (uint32_t r = 0; r < rounds; ++r) {
+        Batch nxt;
+        load(r + 1, nxt);   // past the last round: clamped, never counted
+        bool full = true;
+#pragma unroll
+        for (uint32_t k = 0; k < CPR; ++k) full = full && cur.n[k] + VEC <= g.len;
+        if (__all(full))   // every voxel of the wave's chunks is in its line
+            count(cur, std::true_type());
+        else
+            count(cur, std::false_type());
+        if ((r + 1) % kRoundsPerSettle == 0) h.settle();
+        cur = nxt;
+    }
+}
+
+// count_lines with 16-byte loads when the line stride allows them.
+template <typename T, int NA, typename F>
+__device__ __forceinline__ void count_slice(const WideHist &h, const VoxGeom &g, const T *const (&arr)[NA],
+                                            uint32_t l0, uint32_t l1, F &&bin_of)
+{
+    constexpr uint32_t VEC = 16 / sizeof(T);
+    if (g.stride % VEC == 0)
+        count_lines<VEC>(h, g, arr, l0, l1, bin_of);
+    else
+        count_lines<kPeriodVoxels / 4>(h, g, arr, l0, l1, bin_of);
+}
+
+// ONE: a single partition (nbins <= kWideBins), whose bins need no test.
+template <bool DITHER, bool ONE>
+__global__ __launch_bounds__(kPartThreads) void hist_wide_kernel(VoxGeom g, const float *__restrict__ u, HistArgs a,
+                                                                 uint32_t parts, uint32_t used,
+                                                                 uint32_t *__restrict__ rows)
+{
+    uint32_t slice, part;
+    if (!wide_block(parts, used, slice, part)) return;
+    __shared__ uint32_t sh[kWideLdsWords];
+    const uint32_t b0 = part * kWideBins, nb = min(kWideBins, a.nbins - b0);
+    const WideHist h(sh, nb);
+    h.clear();
     __syncthreads();
     const uint32_t l0 = slice * a.lines_per_block;
     const uint32_t l1 = min(g.lines, l0 + a.lines_per_block);
     const float half = a.delta / 2;
-    wave_lines(g, u, l0 + threadIdx.x / kWave, l1, blockDim.x / kWave, [&](float v, uint32_t line, uint32_t n) {
+    const float *const arr[1] = {u};
+    count_slice<float, 1>(h, g, arr, l0, l1, [&](const float (&x)[1], uint32_t line, uint32_t n) -> uint32_t {
+        float v = x[0];
         if constexpr (DITHER) {
             const uint32_t r = philox_first(line, n, a.k0, a.k1);
             v = v + (-half + a.delta * (float(r >> 8) * (1.0f / 16777216.0f)));
+            if (!(a.rs - half <= v && v + half <= a.re)) return kNoBin;
+        } else {
+            if (!(a.rs <= v && v <= a.re)) return kNoBin;   // delta = 0: v + 0 compares as v
         }
-        if (a.rs - half <= v && v + half <= a.re) {
-            const uint32_t b = uint32_t(hist_bin(v, a.rs, a.re, a.nbins)) - b0;
-            if (b < nb) atomicAdd(&sh[b], 1u);
-        }
+        const uint32_t b = uint32_t(hist_bin(v, a.rs, a.re, a.nbins, a.fast)) - b0;
+        if constexpr (ONE) return b;
+        return b < nb ? b : kNoBin;
     });
     __syncthreads();
-    uint32_t *row = partial + uint64_t(slice) * a.nbins + b0;
-    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) row[i] = sh[i];
+    h.store(rows + (uint64_t(slice) * parts + part) * kWideRowWords);
 }
 
-__global__ __launch_bounds__(256) void hist_sum_kernel(const uint32_t *__restrict__ partial, uint32_t slices,
-                                                       uint32_t nbins, uint32_t *__restrict__ counts)
+// Sums the blocks of `rows` of `slices` slices into out[nbins]: workgroup b
+// takes bins [512 b, 512 b + 512) (one partition), lane l of wave w their
+// 8 bins from 8 l on (one 16-byte load per block) of slices w, w + 4, ...;
+// then the spill entries of every slice that fall in the range.
+template <typename T>
+__global__ __launch_bounds__(256) void wide_sum_kernel(const uint32_t *__restrict__ rows, uint32_t slices,
+                                                       uint32_t parts, uint32_t nbins, T *__restrict__ out)
 {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nbins) return;
-    uint32_t s = 0;
-    for (uint32_t k = 0; k < slices; ++k) s += partial[uint64_t(k) * nbins + b];
-    counts[b] = s;
+    __shared__ T acc[4][512];
+    __shared__ T tot[512];
+    const uint32_t part = blockIdx.x / (kWideBins / 512), c0 = (blockIdx.x % (kWideBins / 512)) * 512;
+    const uint32_t lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    T s[8] = {};
+#pragma unroll 4
+    for (uint32_t k = w; k < slices; k += 4) {
+        const uint4 q = *reinterpret_cast<const uint4 *>(rows + (uint64_t(k) * parts + part) * kWideRowWords + c0 / 2 +
+                                                         4 * lane);
+        const uint32_t qw[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            s[2 * j] += T(qw[j] & 0xffffu);
+            s[2 * j + 1] += T(qw[j] >> 16);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[w][8 * lane + j] = s[j];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 512; i += blockDim.x) tot[i] = acc[0][i] + acc[1][i] + acc[2][i] + acc[3][i];
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < slices; k += blockDim.x) {
+        const uint32_t *blk = rows + (uint64_t(k) * parts + part) * kWideRowWords;
+        const uint32_t ns = blk[kWideWords];
+        for (uint32_t e = 0; e < ns; ++e) {
+            const uint32_t b = blk[kWideWords + 1 + e] - c0;
+            if (b < 512) atomicAdd(&tot[b], T(kSpill));
+        }
+    }
+    __syncthreads();
+    const uint32_t gb = part * kWideBins + c0;
+    for (uint32_t i = threadIdx.x; i < 512 && gb + i < nbins; i += blockDim.x) out[gb + i] = tot[i];
 }
 
-// hist_sum_kernel into 64-bit counts: a slice's row stays 32-bit (a slice holds
-// far fewer than 2^32 voxels), the sum over slices need not.
-__global__ __launch_bounds__(256) void hist_sum64_kernel(const uint32_t *__restrict__ partial, uint32_t slices,
-                                                         uint32_t nbins, uint64_t *__restrict__ counts)
+// Slices of lines for the wide kernels: a workgroup per CU over all
+// partitions where the lines allow, at most kWideSliceVoxels voxel slots per
+// slice, and the blocks of rows within max_words words.  ok is false when
+// these cannot all hold (a line longer than kWideSliceVoxels, more than
+// kMaxParts partitions, or too little room for the rows).
+struct WidePlan {
+    bool ok = false;
+    uint32_t parts = 0, lps = 0, used = 0;   // partitions, lines per slice, slices
+    uint64_t words = 0;                      // of rows
+};
+
+WidePlan wide_plan(const sct_ctx *ctx, const VoxGeom &g, uint64_t nbins, uint64_t max_words)
 {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nbins) return;
-    uint64_t s = 0;
-    for (uint32_t k = 0; k < slices; ++k) s += partial[uint64_t(k) * nbins + b];
-    counts[b] = s;
+    WidePlan p;
+    p.parts = uint32_t((nbins + kWideBins - 1) / kWideBins);
+    if (!g.lines || !g.len || g.len > kWideSliceVoxels || p.parts > kMaxParts) return p;
+    const uint32_t lps_cap = uint32_t(kWideSliceVoxels / g.len);
+    uint64_t slices = std::max<uint32_t>(1, std::min<uint32_t>(g.lines, std::max<uint32_t>(ctx->cu_count, 1) / p.parts));
+    slices = std::min<uint64_t>(slices, max_words / (uint64_t(p.parts) * kWideRowWords));
+    if (!slices) return p;
+    p.lps = std::min<uint32_t>(uint32_t((g.lines + slices - 1) / slices), lps_cap);
+    p.used = (g.lines + p.lps - 1) / p.lps;
+    p.words = uint64_t(p.used) * p.parts * kWideRowWords;
+    p.ok = p.words <= max_words;
+    return p;
 }
 
 // grad_kernel of grad_magn.cpp:17-58 for one voxel: x its value's
@@ -380,6 +649,7 @@ __global__ __launch_bounds__(256) void grad_magn_kernel(VoxGeom g, const float *
 struct UvArgs {
     float u0, u1, v0, v1;
     uint32_t nbins, hstride;
+    uint32_t fast_u, fast_v;   // short_div_ok of the two ranges
 };
 
 // counts[bin] += 1 for every lane of the wave with bin >= 0.  The lanes that
@@ -422,8 +692,8 @@ __global__ __launch_bounds__(256) void uv_hist_kernel(VoxGeom g, const float *__
             const uint64_t o = uint64_t(line) * g.stride + n;
             const float x = u[o], y = v[o];
             if (a.u0 <= x && x <= a.u1 && a.v0 <= y && y <= a.v1) {
-                const int32_t xb = hist_bin(x, a.u0, a.u1, a.nbins);
-                const int32_t yb = hist_bin(y, a.v0, a.v1, a.nbins);
+                const int32_t xb = hist_bin(x, a.u0, a.u1, a.nbins, a.fast_u);
+                const int32_t yb = hist_bin(y, a.v0, a.v1, a.nbins, a.fast_v);
                 bin = (long long)yb * a.hstride + xb;
             }
         }
@@ -431,12 +701,12 @@ __global__ __launch_bounds__(256) void uv_hist_kernel(VoxGeom g, const float *__
     }
 }
 
-// uv_hist with up to kPartBins * kMaxParts counters: workgroup (slice,
-// partition) counts the slice's voxels whose (v, u) bin falls in its 32768
-// counters of LDS and stores them as its slice's row; hist_sum_kernel adds
-// the rows.  Hot bins (the phases' intensities at small gradients) then cost
-// LDS atomics within a workgroup instead of device atomics from every CU on
-// one address.
+// uv_hist with up to kWideBins * kMaxParts counters: workgroup (slice,
+// partition) counts the slice's voxels whose (v, u) bin falls in its 65536
+// 16-bit counters of LDS and stores them as its (slice, partition) block of
+// rows; wide_sum_kernel adds the blocks.  Hot bins (the phases' intensities
+// at small gradients) then cost LDS atomics within a workgroup instead of
+// device atomics from every CU on one address.
 // The joint bin of every voxel (~0u outside the ranges), for the partitioned
 // count when more than kBinPassParts partitions would each recompute it.
 constexpr uint32_t kBinPassParts = 2;
@@ -450,46 +720,49 @@ __global__ __launch_bounds__(256) void uv_bin_kernel(VoxGeom g, const float *__r
                         const float x = xy[0], y = xy[1];
                         uint32_t b = ~0u;
                         if (a.u0 <= x && x <= a.u1 && a.v0 <= y && y <= a.v1)
-                            b = uint32_t(hist_bin(y, a.v0, a.v1, a.nbins)) * a.hstride +
-                                uint32_t(hist_bin(x, a.u0, a.u1, a.nbins));
+                            b = uint32_t(hist_bin(y, a.v0, a.v1, a.nbins, a.fast_v)) * a.hstride +
+                                uint32_t(hist_bin(x, a.u0, a.u1, a.nbins, a.fast_u));
                         bins[uint64_t(line) * g.stride + n] = b;
                     });
 }
 
-// BINS: u holds uv_bin_kernel's bins (read as 32-bit words), v is unused.
+// BINS: bins holds uv_bin_kernel's bins and u, v are unused; otherwise the
+// workgroup computes the bins from u and v.
 template <bool BINS>
-__global__ __launch_bounds__(kPartThreads) void uv_part_kernel(VoxGeom g, const float *__restrict__ u,
-                                                               const float *__restrict__ v, UvArgs a, uint32_t parts,
-                                                               uint32_t lines_per_slice, uint32_t total,
-                                                               uint32_t *__restrict__ partial)
+__global__ __launch_bounds__(kPartThreads) void uv_wide_kernel(VoxGeom g, const float *__restrict__ u,
+                                                               const float *__restrict__ v,
+                                                               const uint32_t *__restrict__ bins, UvArgs a,
+                                                               uint32_t parts, uint32_t used, uint32_t lines_per_slice,
+                                                               uint32_t total, uint32_t *__restrict__ rows)
 {
-    extern __shared__ uint32_t sh[];
-    const uint32_t part = blockIdx.x % parts, slice = blockIdx.x / parts;
-    const uint32_t b0 = part * kPartBins, nb = min(kPartBins, total - b0);
-    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) sh[i] = 0;
+    uint32_t slice, part;
+    if (!wide_block(parts, used, slice, part)) return;
+    __shared__ uint32_t sh[kWideLdsWords];
+    const uint32_t b0 = part * kWideBins, nb = min(kWideBins, total - b0);
+    const WideHist h(sh, nb);
+    h.clear();
     __syncthreads();
     const uint32_t l0 = slice * lines_per_slice;
     const uint32_t l1 = min(g.lines, l0 + lines_per_slice);
     if constexpr (BINS) {
-        wave_lines(g, u, l0 + threadIdx.x / kWave, l1, blockDim.x / kWave, [&](float w, uint32_t, uint32_t) {
-            const uint32_t b = __float_as_uint(w) - b0;   // ~0u - b0 >= nb
-            if (b < nb) atomicAdd(&sh[b], 1u);
+        const uint32_t *const arr[1] = {bins};
+        count_slice<uint32_t, 1>(h, g, arr, l0, l1, [&](const uint32_t (&x)[1], uint32_t, uint32_t) -> uint32_t {
+            const uint32_t b = x[0] - b0;   // ~0u - b0 >= nb
+            return b < nb ? b : kNoBin;
         });
     } else {
         const float *const arr[2] = {u, v};
-        wave_lines_n<2>(g, arr, l0 + threadIdx.x / kWave, l1, blockDim.x / kWave,
-                        [&](const float (&xy)[2], uint32_t, uint32_t) {
-                            const float x = xy[0], y = xy[1];
-                            if (!(a.u0 <= x && x <= a.u1 && a.v0 <= y && y <= a.v1)) return;
-                            const uint32_t xb = uint32_t(hist_bin(x, a.u0, a.u1, a.nbins));
-                            const uint32_t yb = uint32_t(hist_bin(y, a.v0, a.v1, a.nbins));
-                            const uint32_t b = yb * a.hstride + xb - b0;
-                            if (b < nb) atomicAdd(&sh[b], 1u);
-                        });
+        count_slice<float, 2>(h, g, arr, l0, l1, [&](const float (&xy)[2], uint32_t, uint32_t) -> uint32_t {
+            const float x = xy[0], y = xy[1];
+            if (!(a.u0 <= x && x <= a.u1 && a.v0 <= y && y <= a.v1)) return kNoBin;
+            const uint32_t xb = uint32_t(hist_bin(x, a.u0, a.u1, a.nbins, a.fast_u));
+            const uint32_t yb = uint32_t(hist_bin(y, a.v0, a.v1, a.nbins, a.fast_v));
+            const uint32_t b = yb * a.hstride + xb - b0;
+            return b < nb ? b : kNoBin;
+        });
     }
     __syncthreads();
-    uint32_t *row = partial + uint64_t(slice) * total + b0;
-    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) row[i] = sh[i];
+    h.store(rows + (uint64_t(slice) * parts + part) * kWideRowWords);
 }
 
 int run_minmax(sct_ctx *ctx, const sct_field *f, bool positive, float *mn, float *mx)
@@ -549,29 +822,33 @@ __global__ __launch_bounds__(256) void fill_zero_kernel(uint32_t *__restrict__ p
 }
 
 // How calc_hist counts nbins bins of g: mode 0 (LDS counters, device atomic
-// flush), 3 (bin partitions, per-slice partial rows, then a summing kernel)
-// or 2 (device atomics).  Mode 3 takes one workgroup per CU over all
-// partitions (the LDS holds one), fewer when the partial rows must fit in
+// flush), 3 (16-bit LDS counters by partitions of kWideBins bins, a block of
+// rows per slice and partition, then wide_sum_kernel) or 2 (device atomics,
+// when mode 3's plan does not hold).  The rows of mode 3 take at most
 // max_partial words.
 struct HistPlan {
     int mode = 0;
-    uint32_t parts = 1, used = 0;   // partitions, slices (rows of partial)
+    uint32_t parts = 1, used = 0;   // partitions, slices
     uint64_t partial_words = 0;
 };
 
 HistPlan hist_plan(const sct_ctx *ctx, const VoxGeom &g, HistArgs &a, uint64_t max_partial)
 {
     HistPlan p;
-    p.parts = (a.nbins + kPartBins - 1) / kPartBins;
-    p.mode = a.nbins <= kSmallBins ? 0 : (p.parts <= kMaxParts ? 3 : 2);
+    p.mode = a.nbins <= kSmallBins ? 0 : 3;
     if (!g.lines || !g.len) return p;
     if (p.mode == 3) {
-        uint64_t slices = std::max<uint32_t>(1, std::min<uint32_t>(g.lines, ctx->cu_count / p.parts));
-        slices = std::max<uint64_t>(1, std::min<uint64_t>(slices, max_partial / a.nbins));
-        a.lines_per_block = uint32_t((g.lines + slices - 1) / slices);
-        p.used = (g.lines + a.lines_per_block - 1) / a.lines_per_block;
-        p.partial_words = uint64_t(p.used) * a.nbins;
-    } else {
+        const WidePlan w = wide_plan(ctx, g, a.nbins, max_partial);
+        if (w.ok) {
+            a.lines_per_block = w.lps;
+            p.parts = w.parts;
+            p.used = w.used;
+            p.partial_words = w.words;
+            return p;
+        }
+        p.mode = 2;
+    }
+    {
         const uint32_t blocks = std::max<uint32_t>(1, ctx->cu_count * 4);
         a.lines_per_block = std::max<uint32_t>(1, (g.lines + blocks - 1) / blocks);
     }
@@ -591,16 +868,16 @@ hipError_t hist_passes(sct_ctx *ctx, const sct_field *f, const VoxGeom &g, const
         if (e != hipSuccess || !g.lines || !g.len) return e;
     }
     if (p.mode == 3) {
-        const size_t lds = sizeof(uint32_t) * std::min(nbins, kPartBins);
         auto launch = [&](auto kernel) -> hipError_t {
-            hipError_t ea = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-            if (ea != hipSuccess) return ea;
-            kernel<<<p.used * p.parts, kPartThreads, lds, ctx->stream>>>(g, (const float *)f->data, a, p.parts, partial);
+            kernel<<<wide_grid(p.used, p.parts), kPartThreads, 0, ctx->stream>>>(g, (const float *)f->data, a,
+                                                                                 p.parts, p.used, partial);
             return hipGetLastError();
         };
-        hipError_t e = dither ? launch(hist_part_kernel<true>) : launch(hist_part_kernel<false>);
+        hipError_t e = p.parts == 1 ? (dither ? launch(hist_wide_kernel<true, true>) : launch(hist_wide_kernel<false, true>))
+                                    : (dither ? launch(hist_wide_kernel<true, false>)
+                                              : launch(hist_wide_kernel<false, false>));
         if (e != hipSuccess) return e;
-        hist_sum_kernel<<<(nbins + 255) / 256, 256, 0, ctx->stream>>>(partial, p.used, nbins, d);
+        wide_sum_kernel<uint32_t><<<(nbins + 511) / 512, 256, 0, ctx->stream>>>(partial, p.used, p.parts, nbins, d);
         return hipGetLastError();
     }
     const size_t lds = p.mode == 0 ? sizeof(uint32_t) * nbins : 0;
@@ -633,6 +910,7 @@ HistArgs hist_args(float rs, float re, uint32_t nbins, float delta, uint64_t see
     a.k0 = uint32_t(seed);
     a.k1 = uint32_t(seed >> 32);
     a.lines_per_block = 1;
+    a.fast = short_div_ok(rs, re);
     return a;
 }
 
@@ -677,12 +955,13 @@ extern "C" int sct_calc_hist(sct_ctx *ctx, const sct_field *f, float rs, float r
 // ---- calc_hist with no allocation ------------------------------------------
 // The device temporaries of sct_calc_hist_noalloc live in one module-scope
 // array per GPU, `scratch`: the counts (SCT_NOALLOC_MAX_BINS words) and then
-// the partial rows of the partitioned mode (kNoAllocPartial words).  Each
+// the blocks of rows of the partitioned mode (kNoAllocPartial words).  Each
 // GPU holds its own copy at its own address, found once per device with
 // hipGetSymbolAddress when a context is created on it; host code uses that
 // address, never the array's name.
-constexpr uint64_t kNoAllocPartial = uint64_t(4) << 20;   // 16 MiB of partial rows
-static_assert(SCT_NOALLOC_MAX_BINS <= kNoAllocPartial, "one partial row must fit");
+constexpr uint64_t kNoAllocPartial = uint64_t(4) << 20;   // 16 MiB of rows
+static_assert(uint64_t(SCT_NOALLOC_MAX_BINS + kWideBins - 1) / kWideBins * kWideRowWords <= kNoAllocPartial,
+              "the blocks of one slice must fit");
 static __device__ __align__(256) uint32_t scratch[SCT_NOALLOC_MAX_BINS + kNoAllocPartial];
 
 namespace {
@@ -790,21 +1069,18 @@ extern "C" int sct_uv_hist(sct_ctx *ctx, const sct_field *u, const sct_field *v,
     a.v1 = v_end;
     a.nbins = nbins;
     a.hstride = ((nbins + 31u) / 32u) * 32u;   // 128-byte rows, uv-hist-kernel.cpp:115-121
+    a.fast_u = short_div_ok(u_start, u_end);
+    a.fast_v = short_div_ok(v_start, v_end);
     const size_t total = size_t(nbins - 1) * a.hstride + nbins;
     if (stride) *stride = a.hstride;
     const VoxGeom g = vox_geom(u);
-    const uint32_t parts = uint32_t((total + kPartBins - 1) / kPartBins);
-    // counts d, then (partitioned mode) a row of partial counts per slice of
-    // lines, in the context's scratch
+    // counts d, then (partitioned mode) the blocks of rows, in the context's
+    // scratch
     const size_t dbytes = (sizeof(uint32_t) * total + 255) & ~size_t(255);
-    const bool part_mode = g.lines && g.len && parts <= kMaxParts;
-    uint32_t lps = 0, used = 0;
-    if (part_mode) {
-        const uint32_t slices = std::max<uint32_t>(1, std::min<uint32_t>(g.lines, ctx->cu_count / parts));
-        lps = (g.lines + slices - 1) / slices;
-        used = (g.lines + lps - 1) / lps;
-    }
-    const size_t pbytes = (sizeof(uint32_t) * uint64_t(used) * total + 255) & ~size_t(255);
+    const WidePlan wp = wide_plan(ctx, g, total, ~uint64_t(0));
+    const bool part_mode = wp.ok;
+    const uint32_t parts = wp.parts, lps = wp.lps, used = part_mode ? wp.used : 0;
+    const size_t pbytes = part_mode ? sizeof(uint32_t) * wp.words : 0;
     void *mem = nullptr, *host = nullptr;
     int rc = sct_ctx_scratch(ctx, dbytes + pbytes, &mem);
     if (!rc) rc = sct_ctx_host_scratch(ctx, sizeof(uint32_t) * total, &host);
@@ -823,38 +1099,37 @@ extern "C" int sct_uv_hist(sct_ctx *ctx, const sct_field *u, const sct_field *v,
         }
     }
     const bool bin_pass = bins != nullptr;
-    hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t) * total, ctx->stream);
-    if (e == hipSuccess && part_mode) {
-        // counters in LDS, by partitions of the bins (see uv_part_kernel)
-        const size_t lds = sizeof(uint32_t) * std::min<size_t>(total, kPartBins);
+    hipError_t e = hipSuccess;
+    if (part_mode) {
+        // 16-bit counters in LDS, by partitions of the bins (see uv_wide_kernel)
+        const uint32_t grid = wide_grid(used, parts);
         if (bin_pass) {
             uv_bin_kernel<<<stream_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const float *)u->data,
                                                                          (const float *)v->data, a, bins);
             e = hipGetLastError();
-            if (e == hipSuccess)
-                e = hipFuncSetAttribute((const void *)uv_part_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        int(lds));
             if (e == hipSuccess) {
-                uv_part_kernel<true><<<used * parts, kPartThreads, lds, ctx->stream>>>(
-                    g, (const float *)bins, nullptr, a, parts, lps, uint32_t(total), partial);
+                uv_wide_kernel<true><<<grid, kPartThreads, 0, ctx->stream>>>(
+                    g, nullptr, nullptr, bins, a, parts, used, lps, uint32_t(total), partial);
                 e = hipGetLastError();
             }
         } else {
-            e = hipFuncSetAttribute((const void *)uv_part_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    int(lds));
-            if (e == hipSuccess) {
-                uv_part_kernel<false><<<used * parts, kPartThreads, lds, ctx->stream>>>(
-                    g, (const float *)u->data, (const float *)v->data, a, parts, lps, uint32_t(total), partial);
-                e = hipGetLastError();
-            }
-        }
-        if (e == hipSuccess) {
-            hist_sum_kernel<<<uint32_t((total + 255) / 256), 256, 0, ctx->stream>>>(partial, used, uint32_t(total), d);
+            uv_wide_kernel<false><<<grid, kPartThreads, 0, ctx->stream>>>(
+                g, (const float *)u->data, (const float *)v->data, nullptr, a, parts, used, lps, uint32_t(total),
+                partial);
             e = hipGetLastError();
         }
-    } else if (e == hipSuccess && g.lines && g.len) {
-        uv_hist_kernel<<<stream_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const float *)u->data, (const float *)v->data, a, d);
-        e = hipGetLastError();
+        if (e == hipSuccess) {
+            wide_sum_kernel<uint32_t><<<uint32_t((total + 511) / 512), 256, 0, ctx->stream>>>(partial, used, parts,
+                                                                                               uint32_t(total), d);
+            e = hipGetLastError();
+        }
+    } else {
+        e = hipMemsetAsync(d, 0, sizeof(uint32_t) * total, ctx->stream);
+        if (e == hipSuccess && g.lines && g.len) {
+            uv_hist_kernel<<<stream_grid(ctx, g), 256, 0, ctx->stream>>>(g, (const float *)u->data,
+                                                                          (const float *)v->data, a, d);
+            e = hipGetLastError();
+        }
     }
     if (e == hipSuccess) e = hipMemcpyAsync(host, d, sizeof(uint32_t) * total, hipMemcpyDeviceToHost, ctx->stream);
     if (bins) {
@@ -939,47 +1214,26 @@ __global__ __launch_bounds__(256) void convert_kernel(VoxGeom g, const T *__rest
 }
 
 // histogram_kernel of find_intensity_range.cpp:22-46 (65536 bins, one per u16
-// value) by bin partitions, as
-// hist_part_kernel: workgroup (slice of lines, partition of 32768 bins) counts
-// its slice's voxels whose value (the bin) falls in its partition in LDS and
-// stores the counts as its slice's row of `partial`; hist_sum_kernel adds the
-// rows.  A wave walks a line at a time, each lane 4 values per trip (one
-// 8-byte load when the stride is a multiple of 4).
-__global__ __launch_bounds__(kPartThreads) void u16_part_kernel(VoxGeom g, const uint16_t *__restrict__ in,
-                                                                uint32_t lines_per_slice, uint32_t *__restrict__ partial)
+// value) with the wide histogram: workgroup (slice of lines) counts its
+// slice's values in 65536 16-bit LDS counters and stores them as its block of
+// rows; wide_sum_kernel adds the blocks into 64-bit counts.  Each lane takes 8
+// values per chunk (one 16-byte load when the stride is a multiple of 8).
+__global__ __launch_bounds__(kPartThreads) void u16_wide_kernel(VoxGeom g, const uint16_t *__restrict__ in,
+                                                                uint32_t used, uint32_t lines_per_slice,
+                                                                uint32_t *__restrict__ rows)
 {
-    extern __shared__ uint32_t sh[];
-    const uint32_t part = blockIdx.x % 2, slice = blockIdx.x / 2;
-    const uint32_t b0 = part * kPartBins;
-    for (uint32_t i = threadIdx.x; i < kPartBins; i += blockDim.x) sh[i] = 0;
+    uint32_t slice, part;
+    if (!wide_block(1, used, slice, part)) return;
+    __shared__ uint32_t sh[kWideLdsWords];
+    const WideHist h(sh, kWideBins);
+    h.clear();
     __syncthreads();
     const uint32_t l0 = slice * lines_per_slice, l1 = min(g.lines, l0 + lines_per_slice);
-    const uint32_t lane = threadIdx.x % kWave, nw = blockDim.x / kWave;
-    auto count = [&](uint32_t v) {
-        const uint32_t b = v - b0;
-        if (b < kPartBins) atomicAdd(&sh[b], 1u);
-    };
-    for (uint32_t line = l0 + threadIdx.x / kWave; line < l1; line += nw) {
-        const uint16_t *row = in + uint64_t(line) * g.stride;
-        if (g.stride % 4 == 0) {
-            for (uint32_t n0 = lane * 4; n0 < g.len; n0 += kWave * 4) {
-                if (n0 + 4 <= g.len) {
-                    const uint2 q = *reinterpret_cast<const uint2 *>(row + n0);
-                    count(q.x & 0xffffu);
-                    count(q.x >> 16);
-                    count(q.y & 0xffffu);
-                    count(q.y >> 16);
-                } else {
-                    for (uint32_t n = n0; n < g.len; ++n) count(row[n]);
-                }
-            }
-        } else {
-            for (uint32_t n = lane; n < g.len; n += kWave) count(row[n]);
-        }
-    }
+    const uint16_t *const arr[1] = {in};
+    count_slice<uint16_t, 1>(h, g, arr, l0, l1,
+                             [&](const uint16_t (&x)[1], uint32_t, uint32_t) -> uint32_t { return x[0]; });
     __syncthreads();
-    uint32_t *out = partial + uint64_t(slice) * 65536u + b0;
-    for (uint32_t i = threadIdx.x; i < kPartBins; i += blockDim.x) out[i] = sh[i];
+    h.store(rows + uint64_t(slice) * kWideRowWords);
 }
 
 // The same histogram with device atomics, for a device without 128 KB of LDS
@@ -1033,9 +1287,9 @@ extern "C" int sct_convert_to_fp(sct_ctx *ctx, const sct_field *in, float lo, fl
 // find_intensity_range(ctx, field, tail_thr), find_intensity_range.cpp:63-110:
 // the 65536-bin histogram of a u16 field and the tail cut.  hist (optional)
 // receives the 65536 counts.  Counts are 64-bit, as the reference's
-// (find_intensity_range.cpp:48-60): per-slice rows are 32-bit, so a slice of
-// lines must hold fewer than 2^32 voxels (SCT_ERR_SIZE otherwise; a slice is
-// 1/128 of the volume on MI355X).  Synchronises.
+// (find_intensity_range.cpp:48-60): the blocks of a slice hold 16-bit
+// counters and spill entries of at most 2^23 voxels, the sums 64-bit ones.
+// Synchronises.
 extern "C" int sct_find_intensity_range(sct_ctx *ctx, const sct_field *f, double tail_thr, uint16_t *lo,
                                         uint16_t *hi, uint64_t *hist)
 {
@@ -1043,35 +1297,31 @@ extern "C" int sct_find_intensity_range(sct_ctx *ctx, const sct_field *f, double
     if (f->dtype != SCT_U16) return sct_fail(ctx, SCT_ERR_INVALID, "u16 field required");
     constexpr uint32_t nbins = 65536;
     const VoxGeom g = vox_geom(f);
-    // counts d, then a row of partial counts per slice of lines (two
-    // partitions of 32768 bins), in the context's scratch
-    const uint32_t slices = std::max<uint32_t>(1, std::min<uint32_t>(std::max<uint32_t>(g.lines, 1), ctx->cu_count / 2));
-    const uint32_t lps = std::max<uint32_t>(1, (g.lines + slices - 1) / slices);
-    const uint32_t used = (g.lines + lps - 1) / lps;
-    if (uint64_t(lps) * g.len > 0xffffffffull)
-        return sct_fail(ctx, SCT_ERR_SIZE, "find_intensity_range: 2^32 or more voxels in a slice of lines");
+    // counts d, then the blocks of rows of the slices of lines, in the
+    // context's scratch
+    const WidePlan wp = wide_plan(ctx, g, nbins, ~uint64_t(0));
     const size_t dbytes = sizeof(uint64_t) * nbins;
     uint64_t *h = nullptr;
     void *mem = nullptr;
-    int rc = sct_ctx_scratch(ctx, dbytes + sizeof(uint32_t) * uint64_t(used) * nbins, &mem);
+    int rc = sct_ctx_scratch(ctx, dbytes + (wp.ok ? sizeof(uint32_t) * wp.words : 0), &mem);
     if (!rc) rc = sct_ctx_host_scratch(ctx, sizeof(uint64_t) * nbins, (void **)&h);
     if (rc) return rc;
     uint64_t *d = (uint64_t *)mem;
     uint32_t *partial = (uint32_t *)((char *)mem + dbytes);
-    hipError_t e = hipMemsetAsync(d, 0, sizeof(uint64_t) * nbins, ctx->stream);
-    if (e == hipSuccess && g.lines && g.len) {
-        const size_t lds = sizeof(uint32_t) * kPartBins;
-        if (hipFuncSetAttribute((const void *)u16_part_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                int(lds)) == hipSuccess) {
-            u16_part_kernel<<<used * 2, kPartThreads, lds, ctx->stream>>>(g, (const uint16_t *)f->data, lps, partial);
+    hipError_t e = hipSuccess;
+    if (wp.ok) {
+        u16_wide_kernel<<<wp.used, kPartThreads, 0, ctx->stream>>>(g, (const uint16_t *)f->data, wp.used, wp.lps,
+                                                                   partial);
+        e = hipGetLastError();
+        if (e == hipSuccess) {
+            wide_sum_kernel<unsigned long long><<<nbins / 512, 256, 0, ctx->stream>>>(partial, wp.used, 1, nbins,
+                                                                                       (unsigned long long *)d);
             e = hipGetLastError();
-            if (e == hipSuccess) {
-                hist_sum64_kernel<<<nbins / 256, 256, 0, ctx->stream>>>(partial, used, nbins, d);
-                e = hipGetLastError();
-            }
-        } else {
-            // no 128 KB of LDS: device atomics
-            (void)hipGetLastError();
+        }
+    } else {
+        // no voxels, or lines too long for a slice: device atomics
+        e = hipMemsetAsync(d, 0, sizeof(uint64_t) * nbins, ctx->stream);
+        if (e == hipSuccess && g.lines && g.len) {
             const uint32_t blocks = std::max<uint32_t>(1, ctx->cu_count * 4);
             const uint32_t lpb = std::max<uint32_t>(1, (g.lines + blocks - 1) / blocks);
             u16_hist_kernel<<<(g.lines + lpb - 1) / lpb, 256, 0, ctx->stream>>>(

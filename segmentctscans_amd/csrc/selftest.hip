@@ -1,6 +1,7 @@
 // selftest.hip -- GPU check of the range-specific correctly rounded helpers
-// of the Dykstra projection and of the simplex projection's small-constant
-// quotients against the compiler's IEEE sqrtf and division.
+// of the Dykstra projection, of the simplex projection's small-constant
+// quotients and of the histograms' bin fraction against the compiler's IEEE
+// sqrtf and division.
 #include "sct_internal.hpp"
 #include "potts_math.hpp"
 
@@ -17,7 +18,9 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x)
 }
 
 // Operands: x, b uniform over the float bit patterns of [1, 2^64); a uniform
-// over the patterns of [2^-23, b).  Counts results that differ in any bit.
+// over the patterns of [2^-23, b); for the histograms' fraction, b uniform
+// over the patterns of [2^-32, 2^32] and |a| over those of [b 2^-40, b], of
+// either sign.  Counts results that differ in any bit.
 __global__ __launch_bounds__(256) void cr_math_kernel(uint64_t n, uint64_t seed, unsigned long long *bad)
 {
     constexpr uint32_t lo = 0x3f800000u, hi = 0x5f800000u, a_lo = 0x34000000u;
@@ -30,6 +33,14 @@ __global__ __launch_bounds__(256) void cr_math_kernel(uint64_t n, uint64_t seed,
         const float a = __uint_as_float(a_lo + uint32_t(h2 % (b_bits - a_lo)));
         bs += __float_as_uint(sct::sqrt_cr_ge1(x)) != __float_as_uint(sqrtf(x));
         bd += __float_as_uint(sct::div_cr_normal(a, b)) != __float_as_uint(a / b);
+        // the histograms' fraction (x - start) / (end - start) on its short path
+        const uint64_t h3 = splitmix64(~seed ^ (2 * i)), h4 = splitmix64(~seed ^ (2 * i + 1));
+        constexpr uint32_t hb_lo = 0x2f800000u, hb_hi = 0x4f800000u;   // 2^-32, 2^32
+        const uint32_t hb = hb_lo + uint32_t(h3 % (hb_hi - hb_lo + 1));
+        const uint32_t ha_lo = hb - (40u << 23);                       // b 2^-40, a normal float
+        const float hbf = __uint_as_float(hb);
+        const float haf = __uint_as_float((ha_lo + uint32_t(h4 % (hb - ha_lo + 1))) | uint32_t(h3 >> 63) << 31);
+        bd += __float_as_uint(sct::div_cr_normal(haf, hbf)) != __float_as_uint(haf / hbf);
         // the small-constant quotients on any finite operand other than -0
         const float y = __uint_as_float(uint32_t(h2 >> 32));
         if (isfinite(y) && __float_as_uint(y) != 0x80000000u) {

@@ -81,7 +81,7 @@ def test_tv_then_65536_bin_histogram_bit_exact(ctx, ph128):
         assert cnt.sum() == np.count_nonzero((got[m] >= mn) & (got[m] <= mx))
 
 
-@pytest.mark.parametrize("nbins", [1, 256, 1000, 1024, 8192, 8193, 32768, 65536, 100000, 300000])
+@pytest.mark.parametrize("nbins", [1, 256, 1000, 1024, 8192, 8193, 32768, 65536, 100000, 300000, 600000])
 def test_calc_hist_modes_bit_exact(ctx, ph128, nbins):
     lay = O.Layout.from_mapping(ph128.field.mapping)
     g = S.upload(ctx, ph128.field)
@@ -144,10 +144,12 @@ def test_grad_magn_uv_hist_bit_exact(ctx, ph128):
     h2, s2 = S.uv_hist(ctx, u, v, 0.2, 0.6, 0.0, 0.5, 100)
     w2, _ = O.uv_hist(lay, ph128.field.data, vh, 0.2, 0.6, 0.0, 0.5, 100)
     assert np.array_equal(h2.reshape(-1)[: w2.size], w2)
-    # more bins than the LDS partitions hold: device atomics, wave-aggregated
-    h3, _ = S.uv_hist(ctx, u, v, u0, u1, v0, v1, 600)
-    w3, _ = O.uv_hist(lay, ph128.field.data, vh, u0, u1, v0, v1, 600)
-    assert np.array_equal(h3.reshape(-1)[: w3.size], w3)
+    # 6 wide partitions; then more bins than 8 partitions hold: device
+    # atomics, wave-aggregated
+    for nb in (600, 800):
+        h3, _ = S.uv_hist(ctx, u, v, u0, u1, v0, v1, nb)
+        w3, _ = O.uv_hist(lay, ph128.field.data, vh, u0, u1, v0, v1, nb)
+        assert np.array_equal(h3.reshape(-1)[: w3.size], w3)
 
 
 def test_download_size_mismatch_raises(ctx, ph24):
@@ -167,7 +169,9 @@ def test_device_memory(ctx):
 def test_dykstra_cr_sqrt_and_division_match_ieee(ctx):
     """The Dykstra sweep's square root and quotient skip the compiler's range
     scaling; on their ranges (x, b in [1, 2^64), a in [2^-23, b)) they must be
-    IEEE correctly rounded: 2^26 random operands each, bit for bit."""
+    IEEE correctly rounded: 2^26 random operands each, bit for bit.  The same
+    quotient bins the histograms' values (b in [2^-32, 2^32], |a| in
+    [b 2^-40, b], both signs), checked the same way."""
     import ctypes as C
     from segmentctscans_amd import _native as N
     bs, bd = C.c_uint64(), C.c_uint64()
