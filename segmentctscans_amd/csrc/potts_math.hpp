@@ -194,7 +194,9 @@ __device__ __forceinline__ int dykstra(float (&q)[K][3], DykstraLimits lim)
 // True when Dykstra stops after its first sweep: with zero corrections every
 // pair is already feasible, the sweep changes q only by adding +0 to the
 // first index of each pair (turning -0 into +0), and its correction norm is 0.
-// In that case q is updated accordingly.
+// In that case q is updated accordingly.  The sweep's differences are
+// (q_i2 - q_i1) + 0; only their squares enter the test, which the + 0 (a sign
+// of zero) does not change, so it is left out here.
 template <int K>
 __device__ __forceinline__ bool dykstra_first_sweep_done(float (&q)[K][3], DykstraLimits lim)
 {
@@ -203,9 +205,9 @@ __device__ __forceinline__ bool dykstra_first_sweep_done(float (&q)[K][3], Dykst
     for (int i1 = 0; i1 < K - 1; ++i1) {
 #pragma unroll
         for (int i2 = i1 + 1; i2 < K; ++i2) {
-            const float h0 = (q[i2][0] - q[i1][0]) + 0.0f;
-            const float h1 = (q[i2][1] - q[i1][1]) + 0.0f;
-            const float h2 = (q[i2][2] - q[i1][2]) + 0.0f;
+            const float h0 = q[i2][0] - q[i1][0];
+            const float h1 = q[i2][1] - q[i1][1];
+            const float h2 = q[i2][2] - q[i1][2];
             ok = ok && !(__fmaf_rn(h2, h2, __fmaf_rn(h1, h1, h0 * h0)) > lim.t1);
         }
     }

@@ -181,13 +181,16 @@ struct TwoPassArgs {
 };
 
 constexpr int kXiVoxelsPerLane = 4;  // per lane and workgroup: kXiVoxelsPerLane / VPL chunks per wave
+// Chunks per wave of the Xi kernel.
+template <typename St>
+constexpr int xi_cpw() { return kXiVoxelsPerLane / vpl_of<St>(); }
 constexpr int kXiBlockWaves = 4;     // default workgroup of the Xi kernel (its "tile": 64 BW lanes x 4 voxels)
 constexpr int kXiThreads = kXiBlockWaves * 64;
 constexpr int kXiWorkWaves = 4;      // at most this many waves drain the work list
 // Sweeps an item of the work list gets in the first stage; the few that need
-// more (about 16% of the queued items, up to 20 sweeps) are parked and finish
-// in a second stage on the fewest waves, so the long tail of sweep counts no
-// longer sets the time of every draining wave.
+// more (about a quarter of the queued items, up to 20 sweeps) are parked and
+// finish in a second stage on the fewest waves, so the long tail of sweep
+// counts no longer sets the time of every draining wave.
 constexpr int kXiFirstSweeps = 2;
 
 // Per-voxel record of the Xi step's Dykstra, written by the REC variant of
@@ -261,8 +264,13 @@ __device__ __forceinline__ void xi_tile(const Lines &lines, const TwoPassArgs<K,
                                         DykstraLimits lim, SweepRecord rec, uint32_t blk)
 {
     constexpr int KM = K - 1, NP = K * (K - 1) / 2;
-    constexpr int VPL = vpl_of<St>(), CPW = kXiVoxelsPerLane / VPL;
-    constexpr int kXiItems = 128 * BW;     // work-list capacity per workgroup
+    constexpr int VPL = vpl_of<St>(), CPW = xi_cpw<St>();
+    // Work list: a region of kWaveItems items per wave, filled by that wave
+    // alone (a slot is the region's start + the wave's running count + the
+    // lane's rank), so no atomics; a voxel past its region (more than half of
+    // a wave's 64 x 4 voxels queued) is solved in place.
+    constexpr int kWaveItems = 128;
+    constexpr int kXiItems = kWaveItems * BW;
     using P = Packed<St, VPL>;
     // Parking pays for K <= 3; for K >= 4 the corrections of the parked
     // items (NP = 6..15 pairs) cost LDS, hence workgroups per CU.
@@ -272,15 +280,16 @@ __device__ __forceinline__ void xi_tile(const Lines &lines, const TwoPassArgs<K,
     __shared__ float rb[kXiLong][NP][3];   // Dykstra corrections of the parked items
     __shared__ int lt[kXiLong];            // parked item: index | sweeps << 16
     __shared__ uint8_t sq[REC ? kXiItems : 1];   // sweeps of the work-list items
-    __shared__ int nq, next, nl;
+    __shared__ int nwi[BW];                // items in each wave's region
+    __shared__ int next, nl;
     const int lane = threadIdx.x % kWave, w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     if (threadIdx.x == 0) {
-        nq = 0;
         next = 0;
         nl = 0;
     }
     __syncthreads();
     const uint64_t total = lines.count();
+    int queued_n = 0;   // items this wave has queued (uniform)
     float res[CPW][VPL][KM][3];
     int pos[CPW][VPL];
     int sfull[CPW][VPL];   // REC: sweeps of voxels not on the work list
@@ -339,13 +348,10 @@ __device__ __forceinline__ void xi_tile(const Lines &lines, const TwoPassArgs<K,
             xi_input<K>(vbj, vbn, nx, xj, tau2, q);
             const bool queued = valid && !dykstra_first_sweep_done<K>(q, lim);
             const uint64_t mask = __ballot(queued);
-            int p = 0;
-            if (mask) {
-                const int leader = __ffsll((unsigned long long)mask) - 1;
-                if (lane == leader) p = atomicAdd(&nq, __popcll(mask));
-                p = __shfl(p, leader, kWave) + __popcll(mask & ((1ull << lane) - 1ull));
-            }
-            if (queued && p < kXiItems) {
+            const int rank = queued_n + __popcll(mask & ((1ull << lane) - 1ull));
+            const int p = w * kWaveItems + rank;
+            queued_n += __popcll(mask);
+            if (queued && rank < kWaveItems) {
 #pragma unroll
                 for (int i = 0; i < K; ++i)
 #pragma unroll
@@ -360,8 +366,23 @@ __device__ __forceinline__ void xi_tile(const Lines &lines, const TwoPassArgs<K,
                 for (int ax = 0; ax < 3; ++ax) res[k][j][i][ax] = valid ? q[i][ax] : 0.0f;
         }
     }
+    if (lane == 0) nwi[w] = min(queued_n, kWaveItems);
     __syncthreads();   // work list complete
-    const int nwl = min(nq, kXiItems);
+    // item t of the drain: the regions' items one after the other
+    int first[BW + 1];
+    first[0] = 0;
+#pragma unroll
+    for (int i = 0; i < BW; ++i) first[i + 1] = first[i] + nwi[i];
+    const int nwl = first[BW];
+    auto slot_of = [&](int t) {
+        int r = 0;
+#pragma unroll
+        for (int i = 1; i < BW; ++i) r += t >= first[i];
+        int s = 0;
+#pragma unroll
+        for (int i = 0; i < BW; ++i) s = i == r ? r * kWaveItems + t - first[i] : s;
+        return s;
+    };
     if (w < kXiWorkWaves && w * kWave < nwl) {
         float q[K][3], r[NP][3];
         int sw = 0, t = 0;
@@ -375,8 +396,9 @@ __device__ __forceinline__ void xi_tile(const Lines &lines, const TwoPassArgs<K,
                 if (lane == leader) base = atomicAdd(&next, __popcll(mask));
                 base = __shfl(base, leader, kWave);
                 if (need) {
-                    t = base + __popcll(mask & ((1ull << lane) - 1ull));
-                    if (t < nwl) {
+                    const int g = base + __popcll(mask & ((1ull << lane) - 1ull));
+                    if (g < nwl) {
+                        t = slot_of(g);
 #pragma unroll
                         for (int i = 0; i < K; ++i)
 #pragma unroll
@@ -603,7 +625,7 @@ __global__ __launch_bounds__(256) void potts_labels_kernel(Lines lines, TwoPassA
 template <typename St, int BW = kXiBlockWaves>
 inline uint32_t xi_grid(uint64_t chunks)
 {
-    constexpr uint64_t cpw = kXiVoxelsPerLane / vpl_of<St>();
+    constexpr uint64_t cpw = xi_cpw<St>();
     return uint32_t(((chunks + cpw - 1) / cpw + BW - 1) / BW);
 }
 inline uint32_t wave_grid(uint64_t chunks) { return uint32_t((chunks + 3) / 4); }
