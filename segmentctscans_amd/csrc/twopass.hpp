@@ -311,18 +311,21 @@ __device__ __forceinline__ void xi_tile(const Lines &lines, const TwoPassArgs<K,
         nv[k] = n0 < ch.len ? int(min(uint32_t(VPL), ch.len - n0)) : 0;
         const uint32_t w0 = ch.w + l0;
         wofs[k] = nv[k] ? w0 : 0;
+        // Every lane loads, from an address inside its line: a lane past the
+        // line's end reads the chunk's first vector, a missing neighbour line
+        // the voxel's own; neither value is used (the lane's voxels are not
+        // valid, and xi_input skips a missing neighbour).
+        const uint32_t wl = nv[k] ? w0 : ch.w;
+        const uint32_t wr = nv[k] && ch.nr != kNone ? ch.nr + l0 : wl;
+        const uint32_t wc = nv[k] && ch.nc != kNone ? ch.nc + l0 : wl;
         float vb[KM][VPL], vr[KM][VPL], vc[KM][VPL], x[KM][3][VPL], vn[KM][VPL];
 #pragma unroll
         for (int i = 0; i < KM; ++i) {
+            P::ld(a.vbar[i] + wl, vb[i]);
+            P::ld(a.vbar[i] + wr, vr[i]);
+            P::ld(a.vbar[i] + wc, vc[i]);
 #pragma unroll
-            for (int j = 0; j < VPL; ++j) vb[i][j] = vr[i][j] = vc[i][j] = 0.0f;
-            if (nv[k]) {
-                P::ld(a.vbar[i] + w0, vb[i]);
-                if (ch.nr != kNone) P::ld(a.vbar[i] + ch.nr + l0, vr[i]);
-                if (ch.nc != kNone) P::ld(a.vbar[i] + ch.nc + l0, vc[i]);
-#pragma unroll
-                for (int ax = 0; ax < 3; ++ax) P::ld(a.xi[3 * i + ax] + w0, x[i][ax]);
-            }
+            for (int ax = 0; ax < 3; ++ax) P::ld(a.xi[3 * i + ax] + wl, x[i][ax]);
             // V-bar at n + 1: the next element of the lane, the next lane's first,
             // or (last lane) the first of the next chunk
 #pragma unroll
