@@ -223,18 +223,26 @@ __global__ __launch_bounds__((FusedCfg<K, St>::W * 64), 1) void potts_fused_kern
         const bool own = nv && cc.live(rr), below = nv && cc.live(rr - 1);
         const uint32_t off = own ? cc.line(rr) * a.wstride + lofs : 0;
         const uint32_t offb = below ? cc.line(rr - 1) * a.wstride + lofs : 0;
+        // Every lane loads (an absent voxel from offset 0, inside every array)
+        // and keeps zero where the voxel is absent: no branches between the loads.
+        auto keep = [&](bool ok, float(&x)[VPL]) {
 #pragma unroll
-        for (int k = 0; k < NC; ++k) {
-            if (own) gld<St, VPL>(pt->xi_in[k] + off, pxi[k]);
-            else zero(pxi[k]);
-        }
+            for (int j = 0; j < VPL; ++j) x[j] = ok ? x[j] : 0.0f;
+        };
+#pragma unroll
+        for (int k = 0; k < NC; ++k) gld<St, VPL>(pt->xi_in[k] + off, pxi[k]);
 #pragma unroll
         for (int i = 0; i < KM; ++i) {
-            if (below) gld<St, VPL>(pt->xi_in[3 * i + 1] + offb, pxr[i]);
-            else zero(pxr[i]);
-            if (own) gld<StorageF32, VPL>(pt->V_in[i] + off, pv[i]);
-            else zero(pv[i]);
+            gld<St, VPL>(pt->xi_in[3 * i + 1] + offb, pxr[i]);
+            gld<StorageF32, VPL>(pt->V_in[i] + off, pv[i]);
             pxn[i] = (own && lane == 0 && n0 > 0) ? gld1<St>(pt->xi_in[3 * i] + off - 1) : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < NC; ++k) keep(own, pxi[k]);
+#pragma unroll
+        for (int i = 0; i < KM; ++i) {
+            keep(below, pxr[i]);
+            keep(own, pv[i]);
         }
         const uint32_t uo = own ? cc.line(rr) * a.in_stride + a.s0 + lofs : 0;
 #pragma unroll
