@@ -184,6 +184,12 @@ constexpr int kXiVoxelsPerLane = 4;  // per lane and workgroup: kXiVoxelsPerLane
 // Chunks per wave of the Xi kernel.
 template <typename St>
 constexpr int xi_cpw() { return kXiVoxelsPerLane / vpl_of<St>(); }
+// Waves per SIMD the Xi kernel is compiled for (launch-bounds hint): with
+// 16-bit storage at K <= 3 the step is bound by how many workgroups overlap
+// (7 per CU instead of 6: 7.8 -> 7.3 ms on the 1426 x 1024^2 volume); FP32
+// keeps the compiler's choice (6; 7 would spill and ran 11% slower).
+template <int K, typename St>
+constexpr int xi_waves_per_simd() { return K <= 3 && sizeof(typename St::T) == 2 ? 7 : 1; }
 constexpr int kXiBlockWaves = 4;     // default workgroup of the Xi kernel (its "tile": 64 BW lanes x 4 voxels)
 constexpr int kXiThreads = kXiBlockWaves * 64;
 constexpr int kXiWorkWaves = 4;      // at most this many waves drain the work list
@@ -267,9 +273,11 @@ __device__ __forceinline__ void xi_tile(const Lines &lines, const TwoPassArgs<K,
     constexpr int VPL = vpl_of<St>(), CPW = xi_cpw<St>();
     // Work list: a region of kWaveItems items per wave, filled by that wave
     // alone (a slot is the region's start + the wave's running count + the
-    // lane's rank), so no atomics; a voxel past its region (more than half of
-    // a wave's 64 x 4 voxels queued) is solved in place.
-    constexpr int kWaveItems = 128;
+    // lane's rank), so no atomics; a voxel past its region (more than 120 or
+    // 128 of a wave's 64 x 4 voxels queued) is solved in place.  120 for
+    // 16-bit storage at K <= 3 keeps the LDS within a seventh of the CU's
+    // (xi_waves_per_simd).
+    constexpr int kWaveItems = xi_waves_per_simd<K, St>() == 7 ? 120 : 128;
     constexpr int kXiItems = kWaveItems * BW;
     using P = Packed<St, VPL>;
     // Parking pays for K <= 3; for K >= 4 the corrections of the parked
@@ -495,8 +503,10 @@ __device__ __forceinline__ void xi_tile(const Lines &lines, const TwoPassArgs<K,
 }
 
 template <int K, typename St, typename Lines, int BW = kXiBlockWaves, bool REC = false>
-__global__ __launch_bounds__(256) void potts_xi_kernel(Lines lines, TwoPassArgs<K, St> a, float tau2,
-                                                       DykstraLimits lim, SweepRecord rec = {})
+__global__ __launch_bounds__(256, (xi_waves_per_simd<K, St>())) void potts_xi_kernel(Lines lines,
+                                                                                      TwoPassArgs<K, St> a,
+                                                                                      float tau2, DykstraLimits lim,
+                                                                                      SweepRecord rec = {})
 {
     xi_tile<K, St, Lines, BW, REC>(lines, a, tau2, lim, rec, xcd_block(blockIdx.x, gridDim.x, 64 / BW));
 }
