@@ -471,7 +471,7 @@ def batch_workload(S, seed: int, iters: int, autotune: bool = True) -> dict:
     tiles = S.potts_batch_tiles(ctx)
     log(f"batched, {'autotuned' if autotune else 'default'} tiles {tiles}: iterations {ms:.1f} ms (events), "
         f"call {wall_batch:.2f} s")
-    # one persistent launch per K group (tiles from a work queue, grid barriers)
+    # one persistent launch per K group (tiles from one ordered ticket queue)
     t = time.perf_counter()
     outs_p, ms_p = S.potts_batch(ctx, probs, tau, tau, iters, graph=True, autotune=autotune, persistent=True)
     wall_p = time.perf_counter() - t

@@ -267,7 +267,8 @@ typedef struct sct_potts_problem {
 #define SCT_BATCH_AUTOTUNE 2 /* pick each K group's Xi workgroup tile (and, persistent, its
                                workgroups per CU) by timing (cached per context) */
 #define SCT_BATCH_PERSISTENT 4 /* each K group's iterations in one persistent launch: tiles
-                                  from a work queue, a grid-wide barrier after each half-step */
+                                  from one ordered ticket queue; a half-step's tiles start once
+                                  every tile of the half-step before is complete */
 
 /* Solves n problems, iters primal-dual iterations each, with common tau1 and
  * tau2 and storage (SCT_STORAGE_*): out[p] receives a new field of c[argmax]
@@ -285,9 +286,9 @@ int sct_potts_batch_tiles(sct_ctx *ctx, int *waves);
 /* Fixes the tile of each K (waves[K - 2] in {1, 2, 4}; 0 leaves K to the
  * default or the autotuner) for the following sct_potts_batch calls on ctx. */
 int sct_potts_batch_set_tiles(sct_ctx *ctx, const int *waves);
-/* The workgroups per CU (0: one fewer than fit, within 24 waves and 8 workgroups per CU) of the persistent launches of the
- * last sct_potts_batch call on ctx, for K = 2..6 (0 without
- * SCT_BATCH_PERSISTENT).  No counterpart in the reference. */
+/* The workgroups per CU of the persistent launches of the last
+ * sct_potts_batch call on ctx, for K = 2..6 (0 without SCT_BATCH_PERSISTENT).
+ * No counterpart in the reference. */
 int sct_potts_batch_occupancy(sct_ctx *ctx, int *per_cu);
 
 /* ---- adjoint (no counterpart in the reference) ---------------------------- */

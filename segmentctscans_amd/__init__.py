@@ -351,8 +351,8 @@ def potts_batch(ctx: Context, problems, tau1: float, tau2: float, iters: int, *,
     label fields (one per problem, as potts_min_partition would return them)
     and the event time of the iterations in ms; graph=True runs all
     iterations as one captured hipGraph launch; persistent=True runs each K
-    group's iterations in one persistent launch (tiles from a work queue, a
-    grid barrier after each half-step); autotune=True first picks each K
+    group's iterations in one persistent launch (tiles from one ordered
+    ticket queue, each half-step waiting for the one before); autotune=True first picks each K
     group's Xi workgroup tile and, persistent, its workgroups per CU by timing
     (potts_batch_tiles and potts_batch_occupancy report them)."""
     n = len(problems)
@@ -391,8 +391,7 @@ def potts_batch_tiles(ctx: Context) -> dict:
 
 def potts_batch_occupancy(ctx: Context) -> dict:
     """{K: workgroups per CU} of the persistent launches of the last
-    potts_batch call (0: one fewer than fit, within 24 waves and 8 workgroups per CU, and for every K of a call without
-    persistent=True)."""
+    potts_batch call (0 for every K of a call without persistent=True)."""
     w = (C.c_int * 5)()
     ctx.check(N.lib().sct_potts_batch_occupancy(ctx.handle, w))
     t = potts_batch_tiles(ctx)

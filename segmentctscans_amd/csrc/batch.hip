@@ -11,13 +11,14 @@
 // iterations of all groups are captured once into a hipGraph and run as a
 // single graph launch.  With SCT_BATCH_PERSISTENT each group runs all its
 // iterations in one persistent launch: its workgroups take Xi tiles, then V
-// tiles, from a ticket counter, with a grid-wide barrier after each
-// half-step.  Every solve computes exactly what sct_potts_min_partition
+// tiles, from one ordered sequence of tickets, and start a half-step's tiles
+// once every ticket of the half-step before is complete.  Every solve computes exactly what sct_potts_min_partition
 // computes for it (same per-tile code, same arithmetic).
 #include "potts.hpp"
 #include "twopass.hpp"
 
 #include <algorithm>
+#include <cstddef>
 #include <map>
 #include <memory>
 #include <vector>
@@ -30,7 +31,8 @@ namespace {
 struct Group {
     int K = 0;
     int bw = kXiBlockWaves;            // Xi workgroup tile: waves per workgroup
-    int occ = 0;                       // persistent launch: workgroups per CU (0: as many as allowed)
+    int occ = 0;                       // persistent launch: workgroups per CU (0: as many as fit, -1: one fewer)
+    int per_cu = 0;                    // persistent launch: workgroups per CU of the last launch
     std::vector<int> probs;            // problem indices
     std::vector<uint64_t> r_off;       // result offset of each solve in R
     uint64_t wspan = 0, n_chunks = 0, r_total = 0, u_total = 0;
@@ -140,56 +142,135 @@ TwoPassArgs<K, St> group_args(const Group &G)
     return a;
 }
 
-// Grid-wide barrier and tile tickets of a persistent launch (zeroed by
+// Tile tickets and completion count of a persistent launch (zeroed by
 // sync_reset_kernel before it).
+constexpr size_t kPersistArgBytes = 1024;
 struct GridSync {
-    unsigned count, gen, err, pad;
-    unsigned long long ticket;
+    unsigned long long ticket;   // next ticket to hand out
+    unsigned long long done;     // tickets completed and published
+    unsigned err, pad[11];
+    alignas(16) unsigned char args[kPersistArgBytes];   // the launch's PersistArgs
 };
+
+// Arguments of a persistent launch, kept in device memory: for each ticket its
+// workgroups read the pointers of that half-step afresh (scalar loads), so
+// neither the other half-step's pointers nor the last ticket's are held in
+// registers across the ticket loop.
+template <int K, typename St>
+struct PersistArgs {
+    BatchLines L;
+    TwoPassArgs<K, St> a;
+    SolveTable<K> prm;
+};
+static_assert(sizeof(PersistArgs<6, StorageF32>) <= kPersistArgBytes, "persistent argument table size");
+
+template <typename X>
+using ConstAS = __attribute__((address_space(4))) X;
+template <typename X>
+using GlobalAS = __attribute__((address_space(1))) X;
+
+// p, which points to global memory: the cast there and back lets the compiler
+// use global rather than flat accesses through pointers read from the table.
+template <typename X>
+__device__ __forceinline__ X *global_ptr(X *p)
+{
+    return (X *)(GlobalAS<X> *)p;
+}
+
+// The argument table, read afresh: the empty asm makes its address opaque, so
+// its loads are neither hoisted out of the ticket loop nor kept across it.
+template <int K, typename St>
+__device__ __forceinline__ const ConstAS<PersistArgs<K, St>> *fresh_args(const PersistArgs<K, St> *p)
+{
+    const ConstAS<PersistArgs<K, St>> *q = (const ConstAS<PersistArgs<K, St>> *)p;
+    asm volatile("" : "+s"(q));
+    return q;
+}
+
+template <int K, typename St>
+__device__ __forceinline__ BatchLines lines_of(const ConstAS<PersistArgs<K, St>> *p)
+{
+    return BatchLines{global_ptr(p->L.chunks), global_ptr(p->L.lines), p->L.n_chunks};
+}
+
+template <int K, typename St>
+__device__ __forceinline__ TwoPassArgs<K, St> args_of(const ConstAS<PersistArgs<K, St>> *p)
+{
+    TwoPassArgs<K, St> a;
+#pragma unroll
+    for (int j = 0; j < 3 * (K - 1); ++j) {
+        a.xi[j] = global_ptr(p->a.xi[j]);
+        a.xi_out[j] = global_ptr(p->a.xi_out[j]);
+    }
+#pragma unroll
+    for (int i = 0; i < K - 1; ++i) {
+        a.vbar[i] = global_ptr(p->a.vbar[i]);
+        a.V[i] = global_ptr(p->a.V[i]);
+        a.vbar_out[i] = global_ptr(p->a.vbar_out[i]);
+        a.V_out[i] = global_ptr(p->a.V_out[i]);
+    }
+    a.u = global_ptr(p->a.u);
+    return a;
+}
 
 // A waiting workgroup gives up after this many polls (of the order of a
 // second; a half-step of the batched workloads takes milliseconds), flags the
 // launch as failed and leaves the kernel, as does every workgroup that then
-// finds the flag set, so a launch whose workgroups are not all resident ends
-// with an error instead of waiting for ever.
+// finds the flag set, so a launch that stops making progress ends with an
+// error instead of waiting for ever.
 constexpr uint32_t kSpinLimit = 1u << 21;
 
-__global__ void sync_reset_kernel(GridSync *g)
+// Zeroes the counters and stores the launch's argument table.
+template <int K, typename St>
+__global__ void sync_reset_kernel(GridSync *g, PersistArgs<K, St> args)
 {
-    if (threadIdx.x == 0) *g = GridSync{0, 0, 0, 0, 0};
+    if (threadIdx.x == 0) {
+        g->ticket = 0;
+        g->done = 0;
+        g->err = 0;
+        *reinterpret_cast<PersistArgs<K, St> *>(g->args) = args;
+    }
 }
 
-// Every workgroup's writes before the barrier are visible to every
-// workgroup after it: each wave's writes are complete within the workgroup
-// (workgroup-scope release), then one thread releases them at agent scope
-// (the workgroups of one launch span all XCDs; one L2 write-back per
-// workgroup rather than one per wave) and, after the barrier, acquires at
-// agent scope for the workgroup.  Returns false (in every thread of the
-// workgroup) when the launch has failed.
-__device__ __forceinline__ bool grid_sync(GridSync *g, unsigned nblocks)
+// Adds the n tickets this workgroup completed since its last call to the
+// completion count, its writes visible to every workgroup that then reads the
+// count: each wave's writes are complete within the workgroup (workgroup-scope
+// release), then one thread releases them at agent scope (the workgroups of
+// one launch span all XCDs; one L2 write-back per workgroup and half-step).
+__device__ __forceinline__ void publish_done(GridSync *g, unsigned n)
 {
-    __shared__ unsigned ok;
+    if (!n) return;   // uniform over the workgroup
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __syncthreads();
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const unsigned gen = __hip_atomic_load(&g->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned arrived = __hip_atomic_fetch_add(&g->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived + 1 == nblocks) {
-            __hip_atomic_store(&g->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&g->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            uint32_t polls = 0;
-            while (__hip_atomic_load(&g->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                __builtin_amdgcn_s_sleep(2);
-                if (__hip_atomic_load(&g->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                if (++polls == kSpinLimit) {
-                    __hip_atomic_store(&g->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
+        __hip_atomic_fetch_add(&g->done, (unsigned long long)n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Waits until the completion count reaches target, then acquires what was
+// published for the workgroup.  One thread polls with relaxed loads and
+// acquires once, after the wait, so the polling invalidates no cache.
+// Returns false (in every thread of the workgroup) when the launch has failed.
+__device__ __forceinline__ bool wait_done(GridSync *g, unsigned long long target)
+{
+    __shared__ unsigned ok;
+    if (threadIdx.x == 0) {
+        uint32_t polls = 0;
+        unsigned good = 1;
+        while (__hip_atomic_load(&g->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
+            if (__hip_atomic_load(&g->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                good = 0;
+                break;
+            }
+            if (++polls == kSpinLimit) {
+                __hip_atomic_store(&g->err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                good = 0;
+                break;
             }
         }
-        ok = __hip_atomic_load(&g->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+        ok = good;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
@@ -197,72 +278,113 @@ __device__ __forceinline__ bool grid_sync(GridSync *g, unsigned nblocks)
     return ok != 0u;
 }
 
-// All iterations of one K group in one launch.  A workgroup of BW waves takes
-// tickets from the work queue, each worth kTilesPerTicket tiles: in the Xi
-// half-step a tile is the two-pass Xi kernel's workgroup of chunks, in the V
-// half-step BW chunks, one per wave.  The tickets of half-step h are
-// [base_h, base_h + tickets_h), and each workgroup draws one more to find the
-// half-step done, so base_{h+1} = base_h + tickets_h + gridDim.x.
-constexpr unsigned kTilesPerTicket = 8;
-
-template <int K, typename St, int BW>
-__global__ __launch_bounds__(64 * BW) void potts_batch_persistent_kernel(BatchLines L, TwoPassArgs<K, St> a,
-                                                                         SolveTable<K> prm, float tau1, float tau2,
-                                                                         DykstraLimits lim, int iters, GridSync *gs)
+// x, the same in every lane of the wave, as a scalar.
+__device__ __forceinline__ unsigned long long uniform64(unsigned long long x)
 {
-    constexpr uint64_t cpt = uint64_t(kXiVoxelsPerLane / vpl_of<St>()) * BW;   // chunks per Xi tile
-    const unsigned long long nxi = (L.count() + cpt - 1) / cpt, nvt = (L.count() + BW - 1) / BW;
-    const unsigned long long kxi = (nxi + kTilesPerTicket - 1) / kTilesPerTicket,
-                             kvt = (nvt + kTilesPerTicket - 1) / kTilesPerTicket;
-    __shared__ unsigned long long tk;
-    unsigned long long base = 0;
-    auto next_ticket = [&]() -> unsigned long long {
-        if (threadIdx.x == 0) tk = atomicAdd(&gs->ticket, 1ull) - base;
-        __syncthreads();
-        const unsigned long long t = tk;
-        __syncthreads();
-        return t;
-    };
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    for (int it = 0; it < iters; ++it) {
-        for (unsigned long long k = next_ticket(); k < kxi; k = next_ticket()) {
-            const unsigned long long t1 = min(nxi, (k + 1) * kTilesPerTicket);
-            for (unsigned long long t = k * kTilesPerTicket; t < t1; ++t)
-                xi_tile<K, St, BatchLines, BW, false>(L, a, tau2, lim, SweepRecord{}, uint32_t(t));
-        }
-        base += kxi + gridDim.x;
-        if (!grid_sync(gs, gridDim.x)) return;
-        for (unsigned long long k = next_ticket(); k < kvt; k = next_ticket()) {
-            const unsigned long long t1 = min(nvt, (k + 1) * kTilesPerTicket);
-            for (unsigned long long t = k * kTilesPerTicket; t < t1; ++t)
-                v_chunk<K, St, BatchLines, SolveTable<K>>(L, a, prm, tau1, t * BW + w);
-        }
-        base += kvt + gridDim.x;
-        if (!grid_sync(gs, gridDim.x)) return;
-    }
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(x)), hi = __builtin_amdgcn_readfirstlane(uint32_t(x >> 32));
+    return (unsigned long long)hi << 32 | lo;
 }
 
-constexpr int kMaxPerCu = 8, kMaxWavesPerCu = 24;
+// All iterations of one K group in one launch.  The work is one sequence of
+// tickets, each worth kTilesPerTicket tiles: per iteration the Xi half-step's
+// tickets (a tile is the two-pass Xi kernel's workgroup of chunks), then the
+// V half-step's (a tile is BW chunks, one per wave).  A workgroup draws
+// tickets in order; before its first tile of a half-step it publishes the
+// tickets it has completed and waits until every ticket of the half-steps
+// before is complete.  Those tickets were drawn earlier, by workgroups that
+// are running, so the wait always ends, whether or not the whole grid is
+// resident at once.
+constexpr unsigned kTilesPerTicket = 8;
+
+// Waves per SIMD the persistent kernel is compiled for (launch-bounds hint).
+template <int K>
+constexpr int persist_waves_per_simd() { return K == 2 ? 8 : K == 3 ? 6 : 1; }
 
 template <int K, typename St, int BW>
-int launch_persistent(sct_ctx *ctx, const Group &G, const BatchLines &L, const TwoPassArgs<K, St> &a,
+__global__ __launch_bounds__(64 * BW, (persist_waves_per_simd<K>())) void potts_batch_persistent_kernel(const PersistArgs<K, St> *pa, float tau1,
+                                                                         float tau2, DykstraLimits lim, int iters,
+                                                                         GridSync *gs)
+{
+    constexpr uint64_t cpt = uint64_t(xi_cpw<St>()) * BW;   // chunks per Xi tile
+    const uint64_t count = fresh_args(pa)->L.n_chunks;
+    const unsigned long long nxi = (count + cpt - 1) / cpt, nvt = (count + BW - 1) / BW;
+    const unsigned long long kxi = (nxi + kTilesPerTicket - 1) / kTilesPerTicket,
+                             kvt = (nvt + kTilesPerTicket - 1) / kTilesPerTicket;
+    const unsigned long long per_it = kxi + kvt, total = per_it * (unsigned long long)iters;
+    __shared__ unsigned long long tk;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    unsigned long long reached = 0;   // completion count this workgroup has waited for
+    unsigned mine = 0;                // tickets completed here, not yet published
+    while (true) {
+        if (threadIdx.x == 0) tk = atomicAdd(&gs->ticket, 1ull);
+        __syncthreads();
+        // the same in every lane: kept in scalar registers, so that the tile
+        // and chunk indices derived from it stay scalar too
+        const unsigned long long t = uniform64(tk);
+        __syncthreads();
+        if (t >= total) break;
+        const unsigned long long it = t / per_it, r = t - it * per_it;
+        const bool xi = r < kxi;
+        const unsigned long long need = it * per_it + (xi ? 0 : kxi);   // tickets of the half-steps before
+        if (need > reached) {
+            publish_done(gs, mine);
+            mine = 0;
+            if (!wait_done(gs, need)) return;
+            reached = need;
+        }
+        if (xi) {
+            const auto *p = fresh_args(pa);
+            const BatchLines L = lines_of(p);
+            const TwoPassArgs<K, St> a = args_of(p);
+            const unsigned long long t1 = min(nxi, (r + 1) * kTilesPerTicket);
+#pragma unroll 1
+            for (unsigned long long u = r * kTilesPerTicket; u < t1; ++u)
+                xi_tile<K, St, BatchLines, BW, false, true>(L, a, tau2, lim, SweepRecord{}, uint32_t(u));
+        } else {
+            const auto *p = fresh_args(pa);
+            const BatchLines L = lines_of(p);
+            const TwoPassArgs<K, St> a = args_of(p);
+            const SolveTable<K> prm{global_ptr(p->prm.p)};
+            const unsigned long long k = r - kxi, t1 = min(nvt, (k + 1) * kTilesPerTicket);
+#pragma unroll 1
+            for (unsigned long long u = k * kTilesPerTicket; u < t1; ++u)
+                v_chunk<K, St, BatchLines, SolveTable<K>, true>(L, a, prm, tau1, u * BW + w);
+        }
+        ++mine;
+    }
+    publish_done(gs, mine);
+}
+
+template <int K, typename St, int BW>
+int launch_persistent(sct_ctx *ctx, Group &G, const BatchLines &L, const TwoPassArgs<K, St> &a,
                       const SolveTable<K> &prm, float tau1, float tau2, int iters, GridSync *gs)
 {
     const void *fn = (const void *)potts_batch_persistent_kernel<K, St, BW>;
     int fit = 0;
     SCT_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, fn, 64 * BW, 0));
     if (fit < 1) return sct_fail(ctx, SCT_ERR_HIP, "persistent batch kernel does not fit a CU");
-    // One workgroup per CU fewer than the occupancy query reports, at most 24
-    // waves and 8 workgroups per CU.  On MI355X these grids did not all become
-    // resident at once (their barriers timed out): 4-wave workgroups at 5 and
-    // at 8 per CU (every slot the query reports), 2-wave ones at 14 per CU,
-    // 1-wave ones at 31 per CU; within these limits they all did.
-    const int most = std::max(1, std::min(std::min(fit - 1, kMaxWavesPerCu / BW), kMaxPerCu));
-    const int per_cu = G.occ > 0 ? std::min(G.occ, most) : most;
+    // The query counts more workgroups than a CU holds at once for these
+    // kernels (24 of 4 waves for K = 3, whose 80 VGPRs allow 6 waves per
+    // SIMD); bound it by the kernel's VGPRs (512 per SIMD lane, in granules of
+    // 8, at most 8 waves per SIMD, 4 SIMDs per CU) and its LDS.
+    hipFuncAttributes fa{};
+    SCT_HIP(ctx, hipFuncGetAttributes(&fa, fn));
+    int lds_cu = 0;
+    SCT_HIP(ctx, hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, ctx->device));
+    if (fa.numRegs > 0) fit = std::min(fit, std::min(8, 512 / ((fa.numRegs + 7) / 8 * 8)) * 4 / BW);
+    if (fa.sharedSizeBytes > 0 && lds_cu > 0) fit = std::min(fit, int(size_t(lds_cu) / fa.sharedSizeBytes));
+    fit = std::max(fit, 1);
+    // As many workgroups per CU as fit, or one fewer (the autotuner's
+    // choice): no wait depends on a workgroup that has not started, so the
+    // grid need not be resident all at once.
+    const int per_cu = G.occ > 0 ? std::min(G.occ, fit) : std::max(1, fit + std::max(G.occ, -1));
+    G.per_cu = per_cu;
     const uint32_t grid = uint32_t(per_cu) * std::max<uint32_t>(1, ctx->cu_count);
-    sync_reset_kernel<<<1, 64, 0, ctx->stream>>>(gs);
-    potts_batch_persistent_kernel<K, St, BW><<<grid, 64 * BW, 0, ctx->stream>>>(L, a, prm, tau1, tau2,
-                                                                                dykstra_limits(), iters, gs);
+    sync_reset_kernel<K, St><<<1, 64, 0, ctx->stream>>>(gs, PersistArgs<K, St>{L, a, prm});
+    const auto *pa = reinterpret_cast<const PersistArgs<K, St> *>(reinterpret_cast<const char *>(gs) +
+                                                                   offsetof(GridSync, args));
+    potts_batch_persistent_kernel<K, St, BW><<<grid, 64 * BW, 0, ctx->stream>>>(pa, tau1, tau2, dykstra_limits(),
+                                                                                iters, gs);
     SCT_HIP(ctx, hipGetLastError());
     return SCT_OK;
 }
@@ -326,15 +448,15 @@ int run_phase(sct_ctx *ctx, const sct_potts_problem *pr, Group &G, int storage, 
     }
 }
 
-// The grid barrier of each group's persistent launches: a launch that gave up
-// waiting flags its GridSync.
+// The completion count of each group's persistent launches: a launch that
+// gave up waiting flags its GridSync.
 int check_sync(sct_ctx *ctx, const GridSync *gs, size_t n)
 {
     std::vector<GridSync> h(n);
     SCT_HIP(ctx, hipMemcpyAsync(h.data(), gs, sizeof(GridSync) * n, hipMemcpyDeviceToHost, ctx->stream));
     SCT_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (const GridSync &g : h)
-        if (g.err) return sct_fail(ctx, SCT_ERR_HIP, "persistent batch launch: a grid barrier timed out");
+        if (g.err) return sct_fail(ctx, SCT_ERR_HIP, "persistent batch launch: a wait for the previous half-step timed out");
     return SCT_OK;
 }
 
@@ -343,12 +465,11 @@ int check_sync(sct_ctx *ctx, const GridSync *gs, size_t n)
 // initial state, so the result is the untuned one.  The configurations are
 // the Xi workgroup tile (BW = 4, 2, 1 waves: 1024, 512, 256 voxels and work
 // lists of 512, 256, 128 entries) and, for the persistent launch, the
-// workgroups per CU (1, 2, or one fewer than fit within 24 waves and 8
-// workgroups per CU).  Choices are cached in the
+// workgroups per CU (as many as fit, or one fewer).  Choices are cached in the
 // context per (K, storage, launch form, size class of the group).
 constexpr int kTuneIters = 3;
 constexpr int kTiles[3] = {4, 2, 1};
-constexpr int kOcc[3] = {0, 2, 1};
+constexpr int kOcc[2] = {0, -1};
 
 uint64_t tile_key(const Group &G, int storage, bool persistent)
 {
@@ -374,7 +495,7 @@ int autotune(sct_ctx *ctx, const sct_potts_problem *pr, std::vector<Group> &grou
         auto hit = ctx->tiles.find(key);
         if (hit != ctx->tiles.end()) {
             G.bw = forced ? forced : (hit->second & 0xff);
-            G.occ = hit->second >> 8;
+            G.occ = (hit->second >> 8) - 1;   // stored + 1 (occ >= -1)
             continue;
         }
         float best = 0;
@@ -382,7 +503,7 @@ int autotune(sct_ctx *ctx, const sct_potts_problem *pr, std::vector<Group> &grou
         bool first = true;
         for (int i = 0; i < 3 && !rc; ++i) {
             if (forced && kTiles[i] != forced) continue;
-            for (int j = 0; j < (persistent ? 3 : 1) && !rc; ++j) {
+            for (int j = 0; j < (persistent ? 2 : 1) && !rc; ++j) {
                 G.bw = kTiles[i];
                 G.occ = persistent ? kOcc[j] : 0;
                 // one untimed iteration, then kTuneIters timed ones
@@ -410,7 +531,7 @@ int autotune(sct_ctx *ctx, const sct_potts_problem *pr, std::vector<Group> &grou
         }
         G.bw = best_bw;
         G.occ = best_occ;
-        if (!rc && !forced) ctx->tiles[key] = best_bw | (best_occ << 8);
+        if (!rc && !forced) ctx->tiles[key] = best_bw | ((best_occ + 1) << 8);
         if (!rc) rc = run_phase(ctx, pr, G, storage, 1, t1, t2);   // back to the initial state
     }
     for (auto e : ev)
@@ -529,13 +650,13 @@ extern "C" int sct_potts_batch(sct_ctx *ctx, int n, const sct_potts_problem *pro
     if (!rc && persistent) rc = sct_alloc(ctx, (void **)&gs, sizeof(GridSync) * std::max<size_t>(1, groups.size()));
     if (!rc && (flags & SCT_BATCH_AUTOTUNE) && iters > 0)
         rc = autotune(ctx, problems, groups, storage, tau1, tau2, gs);
+    if (!rc) rc = run_iterations(ctx, problems, groups, storage, iters, tau1, tau2, (flags & SCT_BATCH_GRAPH) != 0,
+                                 gs, elapsed_ms);
     for (int K = 2; K <= 6; ++K) ctx->last_tiles[K - 2] = ctx->last_occ[K - 2] = 0;
     for (const Group &G : groups) {
         ctx->last_tiles[G.K - 2] = G.n_chunks ? G.bw : 0;
-        if (persistent && G.n_chunks) ctx->last_occ[G.K - 2] = G.occ;
+        if (persistent && G.n_chunks) ctx->last_occ[G.K - 2] = G.per_cu;
     }
-    if (!rc) rc = run_iterations(ctx, problems, groups, storage, iters, tau1, tau2, (flags & SCT_BATCH_GRAPH) != 0,
-                                 gs, elapsed_ms);
     for (Group &G : groups)
         if (!rc) rc = run_phase(ctx, problems, G, storage, 3, tau1, tau2);
     for (Group &G : groups) {

@@ -180,6 +180,18 @@ struct TwoPassArgs {
     }
 };
 
+// threadIdx.x; with OPAQUE, hidden from the compiler.  A loop over tiles (the
+// persistent batch kernel) reads the lane through the opaque form, so the
+// values a tile derives from it are computed per tile rather than hoisted out
+// of the loop and held in registers across both half-steps.
+template <bool OPAQUE>
+__device__ __forceinline__ uint32_t thread_id()
+{
+    uint32_t t = threadIdx.x;
+    if constexpr (OPAQUE) asm volatile("" : "+v"(t));
+    return t;
+}
+
 constexpr int kXiVoxelsPerLane = 4;  // per lane and workgroup: kXiVoxelsPerLane / VPL chunks per wave
 // Chunks per wave of the Xi kernel.
 template <typename St>
@@ -265,7 +277,7 @@ __global__ __launch_bounds__(256) void potts_xi_zero_kernel(Lines lines, TwoPass
 // BW: waves per workgroup (launch 64 BW threads); the workgroup covers 256 BW
 // voxels and its work list holds 128 BW of them.  REC: also store the
 // SweepRecord of every voxel (the solver runs without).
-template <int K, typename St, typename Lines, int BW, bool REC>
+template <int K, typename St, typename Lines, int BW, bool REC, bool OPAQUE = false>
 __device__ __forceinline__ void xi_tile(const Lines &lines, const TwoPassArgs<K, St> &a, float tau2,
                                         DykstraLimits lim, SweepRecord rec, uint32_t blk)
 {
@@ -290,8 +302,9 @@ __device__ __forceinline__ void xi_tile(const Lines &lines, const TwoPassArgs<K,
     __shared__ uint8_t sq[REC ? kXiItems : 1];   // sweeps of the work-list items
     __shared__ int nwi[BW];                // items in each wave's region
     __shared__ int next, nl;
-    const int lane = threadIdx.x % kWave, w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    if (threadIdx.x == 0) {
+    const uint32_t tid = thread_id<OPAQUE>();
+    const int lane = int(tid % kWave), w = __builtin_amdgcn_readfirstlane(int(tid / kWave));
+    if (tid == 0) {
         next = 0;
         nl = 0;
     }
@@ -518,14 +531,14 @@ __global__ __launch_bounds__(256, (xi_waves_per_simd<K, St>())) void potts_xi_ke
 // neighbour lines (5% faster on 1426 x 1024^2; the Xi step's stores gain
 // nothing from it).
 // The V step of chunk gw (one wave); returns the largest sum_k dV_k^2 of its voxels.
-template <int K, typename St, typename Lines, typename Params>
+template <int K, typename St, typename Lines, typename Params, bool OPAQUE = false>
 __device__ __forceinline__ float v_chunk(const Lines &lines, const TwoPassArgs<K, St> &a, const Params &prm,
                                          float tau1, uint64_t gw)
 {
     constexpr int KM = K - 1, VPL = vpl_of<St>();
     using P = Packed<St, VPL>;
     using PF = Packed<StorageF32, VPL>;
-    const int lane = threadIdx.x % kWave;
+    const int lane = int(thread_id<OPAQUE>() % kWave);
     float d = 0.0f;
     if (gw < lines.count()) {   // uniform over the wave
         const Chunk ch = lines.template chunk<VPL>(gw);

@@ -105,9 +105,9 @@ def test_autotuned_batch_equals_untuned(ctx, volumes):
 @pytest.mark.parametrize("storage", ["fp32", "bf16"])
 @pytest.mark.parametrize("graph", [False, True])
 def test_persistent_batch_equals_single_solves(ctx, volumes, storage, graph):
-    """One persistent launch per K group (tiles from a work queue, a grid
-    barrier after each half-step) gives every problem its own
-    potts_min_partition labels."""
+    """One persistent launch per K group (tiles from one ordered ticket
+    queue, each half-step waiting for the one before) gives every problem its
+    own potts_min_partition labels."""
     probs = _problems(volumes)
     outs, ms = S.potts_batch(ctx, probs, TAU, TAU, 30, storage=storage, graph=graph, persistent=True)
     assert len(outs) == len(probs) and ms > 0
@@ -141,7 +141,7 @@ def test_persistent_autotune(ctx, volumes):
     probs = _problems(volumes)
     a, _ = S.potts_batch(ctx, probs, TAU, TAU, 15, graph=True, autotune=True, persistent=True)
     tiles, occ = S.potts_batch_tiles(ctx), S.potts_batch_occupancy(ctx)
-    assert set(tiles.values()) <= {1, 2, 4} and set(occ.values()) <= {0, 1, 2}
+    assert set(tiles.values()) <= {1, 2, 4} and all(1 <= v <= 32 for v in occ.values())
     assert set(occ) == {c.size for (_, c, _) in probs}
     b, _ = S.potts_batch(ctx, probs, TAU, TAU, 15, graph=True)
     for x, y, (g, _, _) in zip(a, b, probs):

@@ -92,8 +92,8 @@ def test_c4_samples_bitwise_vs_fp32_mirror(c4):
 
 
 def test_c4_persistent_equals_graph(c4):
-    """The same 576 solves as one persistent launch per K group (tiles from a
-    work queue, grid barriers, autotuned tiles and workgroups per CU)."""
+    """The same 576 solves as one persistent launch per K group (tiles from
+    one ordered ticket queue, autotuned tiles and workgroups per CU)."""
     _, probs, labels, persistent = c4
     bad = [i for i, (a, b, (g, _, _)) in enumerate(zip(labels, persistent, probs))
            if not np.array_equal(a[g.mapping.valid_mask()], b[g.mapping.valid_mask()])]
