@@ -286,6 +286,12 @@ int sct_potts_batch_tiles(sct_ctx *ctx, int *waves);
 /* Fixes the tile of each K (waves[K - 2] in {1, 2, 4}; 0 leaves K to the
  * default or the autotuner) for the following sct_potts_batch calls on ctx. */
 int sct_potts_batch_set_tiles(sct_ctx *ctx, const int *waves);
+/* Fixes the workgroups per CU of each K's persistent launches (per_cu[K - 2] in
+ * 1..64; 0 leaves K to the default or the autotuner) for the following
+ * sct_potts_batch calls on ctx.  A count above what a CU holds gives a grid
+ * that is not resident all at once, which the launch handles (its waits only
+ * involve workgroups that have started).  No counterpart in the reference. */
+int sct_potts_batch_set_occupancy(sct_ctx *ctx, const int *per_cu);
 /* The workgroups per CU of the persistent launches of the last
  * sct_potts_batch call on ctx, for K = 2..6 (0 without SCT_BATCH_PERSISTENT).
  * No counterpart in the reference. */

@@ -382,6 +382,14 @@ def potts_batch_set_tiles(ctx: Context, tiles: dict) -> None:
     ctx.check(N.lib().sct_potts_batch_set_tiles(ctx.handle, w))
 
 
+def potts_batch_set_occupancy(ctx: Context, per_cu: dict) -> None:
+    """Fix the workgroups per CU ({K: 1..64}) of later persistent potts_batch
+    launches; an empty dict returns every K to the default or the autotuner.
+    Counts above what a CU holds give grids that are not resident at once."""
+    w = (C.c_int * 5)(*[int(per_cu.get(K, 0)) for K in range(2, 7)])
+    ctx.check(N.lib().sct_potts_batch_set_occupancy(ctx.handle, w))
+
+
 def potts_batch_tiles(ctx: Context) -> dict:
     """{K: waves per Xi workgroup} used by the last potts_batch call."""
     w = (C.c_int * 5)()

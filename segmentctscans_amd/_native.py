@@ -103,6 +103,7 @@ SIGNATURES = {
     "sct_potts_batch_tiles": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "sct_potts_batch_set_tiles": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "sct_potts_batch_occupancy": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "sct_potts_batch_set_occupancy": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "sct_potts_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
                                   C.POINTER(C.c_void_p), C.POINTER(C.c_float)]),
     "sct_potts_solver_gap": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

@@ -295,6 +295,7 @@ __device__ __forceinline__ unsigned long long uniform64(unsigned long long x)
 // are running, so the wait always ends, whether or not the whole grid is
 // resident at once.
 constexpr unsigned kTilesPerTicket = 8;
+constexpr int kMaxForcedPerCu = 64;   // sct_potts_batch_set_occupancy's limit
 
 // Waves per SIMD the persistent kernel is compiled for (launch-bounds hint).
 template <int K>
@@ -377,7 +378,9 @@ int launch_persistent(sct_ctx *ctx, Group &G, const BatchLines &L, const TwoPass
     // As many workgroups per CU as fit, or one fewer (the autotuner's
     // choice): no wait depends on a workgroup that has not started, so the
     // grid need not be resident all at once.
-    const int per_cu = G.occ > 0 ? std::min(G.occ, fit) : std::max(1, fit + std::max(G.occ, -1));
+    const int forced = ctx->forced_occ[K - 2];
+    const int per_cu = forced > 0 ? forced
+                     : G.occ > 0 ? std::min(G.occ, fit) : std::max(1, fit + std::max(G.occ, -1));
     G.per_cu = per_cu;
     const uint32_t grid = uint32_t(per_cu) * std::max<uint32_t>(1, ctx->cu_count);
     sync_reset_kernel<K, St><<<1, 64, 0, ctx->stream>>>(gs, PersistArgs<K, St>{L, a, prm});
@@ -503,7 +506,7 @@ int autotune(sct_ctx *ctx, const sct_potts_problem *pr, std::vector<Group> &grou
         bool first = true;
         for (int i = 0; i < 3 && !rc; ++i) {
             if (forced && kTiles[i] != forced) continue;
-            for (int j = 0; j < (persistent ? 2 : 1) && !rc; ++j) {
+            for (int j = 0; j < (persistent && !ctx->forced_occ[G.K - 2] ? 2 : 1) && !rc; ++j) {
                 G.bw = kTiles[i];
                 G.occ = persistent ? kOcc[j] : 0;
                 // one untimed iteration, then kTuneIters timed ones
@@ -600,6 +603,16 @@ extern "C" int sct_potts_batch_set_tiles(sct_ctx *ctx, const int *waves)
         if (waves[k] != 0 && waves[k] != 1 && waves[k] != 2 && waves[k] != 4)
             return sct_fail(ctx, SCT_ERR_INVALID, "tiles are 1, 2 or 4 waves (0: not set)");
     for (int k = 0; k < 5; ++k) ctx->forced_tiles[k] = waves[k];
+    return SCT_OK;
+}
+
+extern "C" int sct_potts_batch_set_occupancy(sct_ctx *ctx, const int *per_cu)
+{
+    if (!ctx || !per_cu) return SCT_ERR_INVALID;
+    for (int k = 0; k < 5; ++k)
+        if (per_cu[k] < 0 || per_cu[k] > kMaxForcedPerCu)
+            return sct_fail(ctx, SCT_ERR_INVALID, "workgroups per CU are 1 to 64 (0: not set)");
+    for (int k = 0; k < 5; ++k) ctx->forced_occ[k] = per_cu[k];
     return SCT_OK;
 }
 

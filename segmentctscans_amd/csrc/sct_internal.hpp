@@ -92,6 +92,7 @@ struct sct_ctx {
     int last_tiles[5] = {};          // tiles of the last sct_potts_batch call, K = 2..6
     int last_occ[5] = {};            // its persistent launches' workgroups per CU (0: no persistent launch)
     int forced_tiles[5] = {};        // tiles set by sct_potts_batch_set_tiles (0: not set)
+    int forced_occ[5] = {};          // workgroups per CU set by sct_potts_batch_set_occupancy (0: not set)
     void *scratch = nullptr;         // device scratch of the reductions (sct_ctx_scratch)
     size_t scratch_bytes = 0;
     void *host_scratch = nullptr;    // pinned host staging of their results

@@ -153,3 +153,29 @@ def test_persistent_autotune(ctx, volumes):
     assert S.potts_batch_tiles(ctx) == tiles and S.potts_batch_occupancy(ctx) == occ
     for x in c2:
         x.close()
+
+
+def test_persistent_grid_larger_than_resident(ctx, volumes):
+    """A persistent launch of 32 workgroups per CU, several times what a CU
+    holds (at most 8 of these four-wave workgroups), so most of the grid starts
+    only as resident workgroups leave.  A workgroup waits only for tickets that
+    started workgroups hold, so the launch completes, with the graph launch's
+    labels.  Counts outside 0..64 are rejected."""
+    probs = _problems(volumes)
+    ks = sorted({c.size for (_, c, _) in probs})
+    with pytest.raises(S.SctError):
+        S.potts_batch_set_occupancy(ctx, {ks[0]: 65})
+    ref, _ = S.potts_batch(ctx, probs, TAU, TAU, 15, graph=True)
+    S.potts_batch_set_tiles(ctx, {K: 4 for K in ks})
+    S.potts_batch_set_occupancy(ctx, {K: 32 for K in ks})
+    try:
+        out, ms = S.potts_batch(ctx, probs, TAU, TAU, 15, graph=False, persistent=True)
+        assert S.potts_batch_occupancy(ctx) == {K: 32 for K in ks} and ms > 0
+    finally:
+        S.potts_batch_set_occupancy(ctx, {})
+        S.potts_batch_set_tiles(ctx, {})
+    for x, y, (g, _, _) in zip(ref, out, probs):
+        m = g.mapping.valid_mask()
+        assert np.array_equal(x.to_host()[m], y.to_host()[m])
+        x.close()
+        y.close()
