@@ -285,6 +285,13 @@ __device__ __forceinline__ unsigned long long uniform64(unsigned long long x)
     return (unsigned long long)hi << 32 | lo;
 }
 
+constexpr unsigned kTilesPerTicket = 8;
+constexpr int kMaxForcedPerCu = 64;   // sct_potts_batch_set_occupancy's limit
+
+// Waves per SIMD the persistent kernel is compiled for (launch-bounds hint).
+template <int K>
+constexpr int persist_waves_per_simd() { return K == 2 ? 8 : K == 3 ? 6 : 1; }
+
 // All iterations of one K group in one launch.  The work is one sequence of
 // tickets, each worth kTilesPerTicket tiles: per iteration the Xi half-step's
 // tickets (a tile is the two-pass Xi kernel's workgroup of chunks), then the
@@ -294,17 +301,9 @@ __device__ __forceinline__ unsigned long long uniform64(unsigned long long x)
 // before is complete.  Those tickets were drawn earlier, by workgroups that
 // are running, so the wait always ends, whether or not the whole grid is
 // resident at once.
-constexpr unsigned kTilesPerTicket = 8;
-constexpr int kMaxForcedPerCu = 64;   // sct_potts_batch_set_occupancy's limit
-
-// Waves per SIMD the persistent kernel is compiled for (launch-bounds hint).
-template <int K>
-constexpr int persist_waves_per_simd() { return K == 2 ? 8 : K == 3 ? 6 : 1; }
-
 template <int K, typename St, int BW>
-__global__ __launch_bounds__(64 * BW, (persist_waves_per_simd<K>())) void potts_batch_persistent_kernel(const PersistArgs<K, St> *pa, float tau1,
-                                                                         float tau2, DykstraLimits lim, int iters,
-                                                                         GridSync *gs)
+__global__ __launch_bounds__(64 * BW, (persist_waves_per_simd<K>())) void potts_batch_persistent_kernel(
+    const PersistArgs<K, St> *pa, float tau1, float tau2, DykstraLimits lim, int iters, GridSync *gs)
 {
     constexpr uint64_t cpt = uint64_t(xi_cpw<St>()) * BW;   // chunks per Xi tile
     const uint64_t count = fresh_args(pa)->L.n_chunks;
