@@ -1,6 +1,6 @@
 """Batched Potts solves (no counterpart in the reference): every problem of a
 batch gets exactly the labels its own potts_min_partition call returns --
-mixed shapes (cylinders and boxes), K = 2..5, two lambdas per volume, FP32 and
+mixed shapes (cylinders and boxes), K = 2..6, two lambdas per volume, FP32 and
 BF16 storage, iterations launched directly or as one captured hipGraph."""
 import numpy as np
 import pytest
@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 TAU = float(np.float32(1.0 / np.sqrt(12.0)))
 SHAPES = [  # n, height, width, box?, K
     (20, 18, 22, False, 3), (33, 25, 25, True, 4), (48, 30, 20, False, 5),
-    (17, 40, 31, True, 3), (26, 26, 26, False, 2), (70, 12, 9, True, 4)]
+    (17, 40, 31, True, 3), (26, 26, 26, False, 2), (70, 12, 9, True, 4), (24, 21, 19, False, 6)]
 
 
 @pytest.fixture(scope="module")
